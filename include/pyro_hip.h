@@ -1,0 +1,121 @@
+/*
+ * pyro_hip.h -- C ABI of the MI355X (gfx950) per-timestep fire solver.
+ *
+ * Drop-in boundary for the wgpu dispatch of the reference's Sim
+ * (src/sim.rs).  A Rust FFI crate wraps these calls as Sim::new / step /
+ * read_stats / read_scalars with the reference's signatures (see
+ * INTEGRATION.md); C++ and Python hosts call them directly.
+ *
+ * Conventions
+ *  - Return 0 on success, a negative PYRO_E_* code on failure; the message is
+ *    available from pyro_last_error() (thread-local).
+ *  - The library owns all device memory; host pointers are borrowed for the
+ *    duration of a call only.
+ *  - A PyroSim handle is single-threaded.  Work is ordered on the handle's
+ *    own HIP stream; pyro_sim_step() returns once the step is enqueued
+ *    (like queue.submit in sim.rs:418) and errors of an enqueued step are
+ *    reported by the next synchronising call.
+ *  - Field layouts are the reference's: vec4 fields N x 16 B, velocity
+ *    components (nx+1)(ny+1)(nz+1) x f32 with x fastest, flags N x u32.
+ */
+#ifndef PYRO_HIP_H
+#define PYRO_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PYRO_PARAMS_BYTES 576 /* scene.rs:426-477 */
+#define PYRO_STATS_N 1040     /* sim.rs:12 STATS_N */
+
+#define PYRO_OK 0
+#define PYRO_E_INVALID (-1)   /* bad argument / size mismatch */
+#define PYRO_E_OOM (-2)       /* device allocation failed */
+#define PYRO_E_HIP (-3)       /* HIP runtime error (incl. a failed earlier launch) */
+#define PYRO_E_NONFINITE (-4) /* a finiteness check found NaN/Inf */
+#define PYRO_E_STATE (-5)     /* call not valid in the current state */
+#define PYRO_E_NODEVICE (-6)  /* no usable GPU */
+
+/* Field ids for read/write_field (superset of sim.rs:494 read_scalars). */
+enum PyroField {
+  PYRO_FIELD_S0 = 0,   /* (T, Y_F, Y_O2, Y_s)           N x vec4 */
+  PYRO_FIELD_S1 = 1,   /* (Y_m, q, S, 0)                N x vec4 */
+  PYRO_FIELD_S0B = 2,  /* advected s0 (before react)    N x vec4 */
+  PYRO_FIELD_S1B = 3,
+  PYRO_FIELD_H0 = 4,   /* first-order predictor of s0   N x vec4 */
+  PYRO_FIELD_H1 = 5,
+  PYRO_FIELD_SOL = 6,      /* sol[sol_cur]   (slot, fuel, char, Ts) */
+  PYRO_FIELD_SOL_OUT = 7,  /* sol[1-sol_cur] */
+  PYRO_FIELD_PYR = 8,      /* (m_v, m_c, exposed faces, 0) */
+  PYRO_FIELD_FLAGS = 9,    /* N x u32, 1 = solid */
+  PYRO_FIELD_CURL = 10,    /* (w, |w|) */
+  PYRO_FIELD_U = 11, PYRO_FIELD_V = 12, PYRO_FIELD_W = 13,    /* vel[cur] */
+  PYRO_FIELD_UB = 14, PYRO_FIELD_VB = 15, PYRO_FIELD_WB = 16, /* vel[1-cur] */
+  PYRO_FIELD_HU = 17, PYRO_FIELD_HV = 18, PYRO_FIELD_HW = 19, /* predictor */
+  PYRO_FIELD_LEVEL_BASE = 32 /* + 4*level + {0 phi, 1 b, 2 flags(u32)} */
+};
+
+/* Stages of one step, in the dispatch order of sim.rs:396-413. */
+enum PyroStage {
+  PYRO_STAGE_SOLID = 0, PYRO_STAGE_CURL, PYRO_STAGE_ADV_VEL1, PYRO_STAGE_ADV_VEL2,
+  PYRO_STAGE_ADV_S1, PYRO_STAGE_ADV_S2, PYRO_STAGE_REACT, PYRO_STAGE_FORCES,
+  PYRO_STAGE_DIVERGENCE, PYRO_STAGE_VCYCLES, PYRO_STAGE_RESID, PYRO_STAGE_PROJECT,
+  PYRO_STAGE_COUNT
+};
+
+/* pyro_sim_create flags */
+#define PYRO_CREATE_NO_GRAPH 1u /* launch kernels directly instead of replaying hipGraphs */
+
+typedef struct PyroSim PyroSim;
+
+const char *pyro_last_error(void);
+
+/* Sim::new (sim.rs:108-232).  params: the 576-byte Params blob of
+ * Scene::params (scene.rs:523-618); sol_init: N x vec4 from Scene::voxelize
+ * (scene.rs:621-660); s0_init: N x vec4 or NULL for ambient (sim.rs:134);
+ * pressure_cycles: SimCfg.pressure_cycles (sim.rs:229); ign_until:
+ * Ignition.until or 0 (sim.rs:204). */
+int pyro_sim_create(const void *params, size_t params_bytes, const float *sol_init,
+                    const float *s0_init, uint32_t pressure_cycles, float ign_until,
+                    uint32_t flags, PyroSim **out);
+int pyro_sim_destroy(PyroSim *sim);
+
+/* Sim::step (sim.rs:246-423): one timestep; stats are produced when
+ * want_stats != 0 and read back by pyro_sim_read_stats. */
+int pyro_sim_step(PyroSim *sim, float dt, int want_stats);
+/* Sim::read_stats (sim.rs:510-530): the 1040 slots of the last stats step
+ * in FP64 (slot map of stats.wgsl:2-3; 10 = sum r^2, 11 = sum b^2). */
+int pyro_sim_read_stats(PyroSim *sim, double *out, size_t n);
+/* Sim::read_scalars (sim.rs:494-508) and every other field, blocking. */
+int pyro_sim_read_field(PyroSim *sim, int field, void *dst, size_t bytes);
+int pyro_sim_write_field(PyroSim *sim, int field, const void *src, size_t bytes);
+/* Sim::cells (sim.rs:234), Sim::time / dims / cycles / levels. */
+int pyro_sim_info(const PyroSim *sim, uint32_t dims[3], uint64_t *cells, double *time,
+                  uint32_t *frame, uint32_t *levels);
+int pyro_sim_level_dims(const PyroSim *sim, int level, uint32_t dims[3]);
+int pyro_sim_sync(PyroSim *sim);
+/* Device pointer of a field for interop (renderer hand-off); *bytes = size. */
+int pyro_sim_device_ptr(PyroSim *sim, int field, void **dptr, size_t *bytes);
+
+/* Fine-grained hooks (per-kernel parity tests, profiling). begin_step sets
+ * dt/time/ign_on exactly as step does; run_stage enqueues one stage with the
+ * current ping-pong roles; end_step finalises stats and swaps roles. */
+int pyro_sim_begin_step(PyroSim *sim, float dt, int want_stats);
+int pyro_sim_run_stage(PyroSim *sim, int stage);
+int pyro_sim_end_step(PyroSim *sim);
+/* Time `reps` launches of one stage with HIP events on the sim stream
+ * (average ms per launch; state is advanced reps times). */
+int pyro_sim_time_stage(PyroSim *sim, int stage, int reps, float *avg_ms);
+/* C1 harness: vel[cur] += amp * U[-1,1] (PCG hash of component, face, seed). */
+int pyro_sim_add_velocity_noise(PyroSim *sim, float amp, uint32_t seed);
+/* Scan the state for NaN/Inf; returns PYRO_E_NONFINITE if any. */
+int pyro_sim_check_finite(PyroSim *sim);
+
+/* Test hook: apply the device's exp (fn 0), sqrt (1) or division in/in2 (2). */
+int pyro_debug_math(int fn, const float *in, const float *in2, float *out, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

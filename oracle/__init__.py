@@ -1,0 +1,263 @@
+"""CPU oracle of pyro's per-timestep solver -- TEST INFRASTRUCTURE ONLY.
+
+ctypes wrapper of ``oracle/liboracle.so`` (built from ``pyro_oracle.c`` by
+``oracle/build.py``).  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import this package, and only as the
+checker / CPU baseline; the product (``pyro_amd``) never does.
+
+``OSim`` exposes the same methods as ``pyro_amd.Sim`` so parity tests drive
+both with identical calls.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+_lib = None
+
+FIELDS = {
+    "s0": 0, "s1": 1, "s0b": 2, "s1b": 3, "h0": 4, "h1": 5, "sol": 6, "sol_out": 7,
+    "pyr": 8, "flags": 9, "curl": 10, "u": 11, "v": 12, "w": 13, "ub": 14, "vb": 15,
+    "wb": 16, "hu": 17, "hv": 18, "hw": 19,
+}
+LEVEL_BASE = 32
+STAGES = [
+    "solid", "curl", "adv_vel1", "adv_vel2", "adv_s1", "adv_s2", "react", "forces",
+    "divergence", "vcycles", "resid", "project",
+]
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        path = HERE / "liboracle.so"
+        if not path.exists():
+            raise RuntimeError(f"{path} missing: run `python oracle/build.py`")
+        L = C.CDLL(str(path))
+        V = C.c_void_p
+        L.osim_create.restype = V
+        L.osim_create.argtypes = [V, C.c_size_t, V, V, C.c_uint32, C.c_float]
+        L.osim_destroy.argtypes = [V]
+        L.osim_levels.argtypes = [V, V]
+        L.osim_step.argtypes = [V, C.c_float, C.c_int]
+        L.osim_begin_step.argtypes = [V, C.c_float, C.c_int]
+        L.osim_run_stage.argtypes = [V, C.c_int]
+        L.osim_end_step.argtypes = [V]
+        L.osim_read_field.argtypes = [V, C.c_int, V, C.c_size_t]
+        L.osim_write_field.argtypes = [V, C.c_int, V, C.c_size_t]
+        L.osim_read_stats.argtypes = [V, V]
+        L.osim_time.argtypes = [V]
+        L.osim_time.restype = C.c_double
+        L.osim_add_velocity_noise.argtypes = [V, C.c_float, C.c_uint32]
+        L.osim_set_threads.argtypes = [C.c_int]
+        L.oracle_expf.argtypes = [C.c_float]
+        L.oracle_expf.restype = C.c_float
+        L.oracle_noise.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+        L.oracle_noise.restype = C.c_float
+        L.omg_create.restype = V
+        L.omg_create.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32, V]
+        L.omg_destroy.argtypes = [V]
+        L.omg_level_dims.argtypes = [V, C.c_int, V]
+        L.omg_set.argtypes = [V, C.c_int, V, V]
+        L.omg_get.argtypes = [V, C.c_int, V, V, V]
+        L.omg_smooth.argtypes = [V, C.c_int, C.c_uint32, C.c_float]
+        L.omg_restrict.argtypes = [V, C.c_int]
+        L.omg_prolong.argtypes = [V, C.c_int]
+        L.omg_vcycles.argtypes = [V, C.c_uint32, C.c_uint32]
+        L.omg_resid_norms.argtypes = [V, V, V]
+        L.oracle_poisson_cg64.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, V, V, V,
+                                          C.c_double, C.c_int, V]
+        L.oracle_residual64.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, V, V, V, V]
+        _lib = L
+    return _lib
+
+
+def set_threads(n: int) -> None:
+    lib().osim_set_threads(int(n))
+
+
+def expf(x: np.ndarray) -> np.ndarray:
+    f = lib().oracle_expf
+    x = np.asarray(x, dtype=np.float32).ravel()
+    return np.array([f(float(v)) for v in x], dtype=np.float32)
+
+
+def noise(comp: int, idx: int, seed: int) -> float:
+    return lib().oracle_noise(comp, idx, seed)
+
+
+class OSim:
+    """CPU restatement of one Sim (same methods as pyro_amd.Sim)."""
+
+    def __init__(self, params, sol_init, s0_init=None, pressure_cycles=4, ign_until=0.0):
+        blob = params.bytes() if hasattr(params, "bytes") else bytes(params)
+        self.dims = tuple(int(v) for v in np.frombuffer(blob, dtype=np.uint32, count=3))
+        self.dx = float(np.frombuffer(blob, dtype=np.float32, count=1, offset=16)[0])
+        sol = np.ascontiguousarray(sol_init, dtype=np.float32)
+        s0p = None
+        if s0_init is not None:
+            self._s0 = np.ascontiguousarray(s0_init, dtype=np.float32)
+            s0p = self._s0.ctypes.data
+        buf = (C.c_uint8 * 576).from_buffer_copy(blob)
+        self._h = lib().osim_create(buf, 576, sol.ctypes.data, s0p, int(pressure_cycles), float(ign_until))
+        if not self._h:
+            raise RuntimeError("osim_create failed")
+        self.cycles = pressure_cycles
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().osim_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, dt, want_stats=False):
+        lib().osim_step(self._h, float(dt), int(bool(want_stats)))
+
+    def begin_step(self, dt, want_stats=False):
+        lib().osim_begin_step(self._h, float(dt), int(bool(want_stats)))
+
+    def run_stage(self, stage):
+        sid = STAGES.index(stage) if isinstance(stage, str) else int(stage)
+        if lib().osim_run_stage(self._h, sid) != 0:
+            raise ValueError(stage)
+
+    def end_step(self):
+        lib().osim_end_step(self._h)
+
+    @property
+    def time(self):
+        return lib().osim_time(self._h)
+
+    @property
+    def levels(self):
+        return lib().osim_levels(self._h, None)
+
+    def level_dims(self, level):
+        d = (C.c_uint32 * 24)()
+        lib().osim_levels(self._h, d)
+        return (d[3 * level], d[3 * level + 1], d[3 * level + 2])
+
+    def _shape(self, fid):
+        nx, ny, nz = self.dims
+        if fid in (0, 1, 2, 3, 4, 5, 6, 7, 8, 10):
+            return (nz, ny, nx, 4), np.float32
+        if fid == 9:
+            return (nz, ny, nx), np.uint32
+        if 11 <= fid <= 19:
+            return (nz + 1, ny + 1, nx + 1), np.float32
+        l, q = divmod(fid - LEVEL_BASE, 4)
+        d = self.level_dims(l)
+        return (d[2], d[1], d[0]), (np.uint32 if q == 2 else np.float32)
+
+    def read_field(self, name):
+        fid = FIELDS[name] if isinstance(name, str) else int(name)
+        shape, dt = self._shape(fid)
+        out = np.zeros(shape, dtype=dt)
+        if lib().osim_read_field(self._h, fid, out.ctypes.data, out.nbytes) != 0:
+            raise ValueError(f"bad field {name}")
+        return out
+
+    def write_field(self, name, arr):
+        fid = FIELDS[name] if isinstance(name, str) else int(name)
+        shape, dt = self._shape(fid)
+        a = np.ascontiguousarray(arr, dtype=dt)
+        if lib().osim_write_field(self._h, fid, a.ctypes.data, a.nbytes) != 0:
+            raise ValueError(f"bad field {name}")
+
+    def read_stats_raw(self):
+        out = np.zeros(1040, dtype=np.float64)
+        lib().osim_read_stats(self._h, out.ctypes.data)
+        return out
+
+    def add_velocity_noise(self, amp, seed):
+        lib().osim_add_velocity_noise(self._h, float(amp), int(seed))
+
+
+class OMg:
+    """Stand-alone multigrid hierarchy of the oracle (mg.wgsl operator)."""
+
+    def __init__(self, dims, h, levels, flags0=None):
+        nx, ny, nz = dims
+        self.dims = dims
+        fp = None
+        if flags0 is not None:
+            self._f = np.ascontiguousarray(flags0, dtype=np.uint32)
+            fp = self._f.ctypes.data
+        self._h = lib().omg_create(nx, ny, nz, float(h), int(levels), fp)
+
+    def __del__(self):
+        try:
+            lib().omg_destroy(self._h)
+        except Exception:
+            pass
+
+    @property
+    def levels(self):
+        d = (C.c_uint32 * 3)()
+        return lib().omg_level_dims(self._h, -1, d)
+
+    def level_dims(self, l):
+        d = (C.c_uint32 * 3)()
+        lib().omg_level_dims(self._h, l, d)
+        return (d[0], d[1], d[2])
+
+    def set(self, l, phi=None, b=None):
+        pp = None if phi is None else np.ascontiguousarray(phi, dtype=np.float32)
+        bp = None if b is None else np.ascontiguousarray(b, dtype=np.float32)
+        lib().omg_set(self._h, l, None if pp is None else pp.ctypes.data, None if bp is None else bp.ctypes.data)
+
+    def get(self, l):
+        nx, ny, nz = self.level_dims(l)
+        phi = np.zeros((nz, ny, nx), np.float32)
+        b = np.zeros_like(phi)
+        f = np.zeros((nz, ny, nx), np.uint32)
+        lib().omg_get(self._h, l, phi.ctypes.data, b.ctypes.data, f.ctypes.data)
+        return phi, b, f
+
+    def smooth(self, l, iters, omega):
+        lib().omg_smooth(self._h, l, iters, omega)
+
+    def restrict(self, l):
+        lib().omg_restrict(self._h, l)
+
+    def prolong(self, l):
+        lib().omg_prolong(self._h, l)
+
+    def vcycles(self, cycles, coarse_iters=80):
+        lib().omg_vcycles(self._h, cycles, coarse_iters)
+
+    def resid_norms(self):
+        r2 = C.c_double()
+        b2 = C.c_double()
+        lib().omg_resid_norms(self._h, C.byref(r2), C.byref(b2))
+        return r2.value, b2.value
+
+
+def poisson_cg64(flags, b, h, tol=1e-12, max_iter=100000, phi0=None):
+    """FP64 Jacobi-PCG solve of the mg.wgsl operator; returns (phi, iters, rel_resid)."""
+    nz, ny, nx = b.shape
+    f = np.ascontiguousarray(flags, dtype=np.uint32)
+    bb = np.ascontiguousarray(b, dtype=np.float64)
+    phi = np.zeros_like(bb) if phi0 is None else np.ascontiguousarray(phi0, dtype=np.float64).copy()
+    rel = C.c_double()
+    it = lib().oracle_poisson_cg64(nx, ny, nz, float(h), f.ctypes.data, bb.ctypes.data, phi.ctypes.data,
+                                   float(tol), int(max_iter), C.byref(rel))
+    return phi, it, rel.value
+
+
+def residual64(flags, b, phi, h):
+    nz, ny, nx = b.shape
+    f = np.ascontiguousarray(flags, dtype=np.uint32)
+    bb = np.ascontiguousarray(b, dtype=np.float64)
+    pp = np.ascontiguousarray(phi, dtype=np.float64)
+    r = np.zeros_like(bb)
+    lib().oracle_residual64(nx, ny, nz, float(h), f.ctypes.data, bb.ctypes.data, pp.ctypes.data, r.ctypes.data)
+    return r

@@ -1,0 +1,269 @@
+// k_advect.hip -- MacCormack advection on the staggered MAC grid (gfx950).
+//
+// Reference passes: advect.wgsl adv_vel1 (:4-20), adv_vel2 (:22-70),
+// adv_s1 (:72-91), adv_s2 (:93-125), with the samplers of tmpl_comp.wgsl
+// (:2-25), tmpl_scalar.wgsl (:2-33) and the RK2 traces of vel.wgsl (:2-19).
+// Velocity samples are trilinear gathers from the padded component arrays;
+// the gathers hit L1/L2 (each face's stencil overlaps its neighbours'), the
+// compulsory HBM traffic is one read of each input and one write per output.
+#include "kernels.hpp"
+
+namespace pyro {
+
+struct VelPtrs {
+  const float *__restrict__ u;
+  const float *__restrict__ v;
+  const float *__restrict__ w;
+};
+
+// Trilinear sample of a staggered component whose nodes are integer points
+// of g (tmpl_comp.wgsl:2-15); hi = largest valid node index.
+__device__ __forceinline__ float triC(const Geo &g, const float *__restrict__ X, float gx0, float gy0, float gz0,
+                                      int hx, int hy, int hz) {
+  float gx = clampf_(gx0, 0.0f, (float)hx), gy = clampf_(gy0, 0.0f, (float)hy), gz = clampf_(gz0, 0.0f, (float)hz);
+  int bx = (int)floorf(gx), by = (int)floorf(gy), bz = (int)floorf(gz);
+  bx = bx > hx - 1 ? hx - 1 : bx;
+  by = by > hy - 1 ? hy - 1 : by;
+  bz = bz > hz - 1 ? hz - 1 : bz;
+  float fx = gx - (float)bx, fy = gy - (float)by, fz = gz - (float)bz;
+  uint32_t sy = (uint32_t)g.vx, sz = (uint32_t)g.vx * (uint32_t)g.vy;
+  uint32_t i = vidx(g, bx, by, bz);
+  float c00 = mixf_(X[i], X[i + 1], fx);
+  float c10 = mixf_(X[i + sy], X[i + sy + 1], fx);
+  float c01 = mixf_(X[i + sz], X[i + sz + 1], fx);
+  float c11 = mixf_(X[i + sy + sz], X[i + sy + sz + 1], fx);
+  return mixf_(mixf_(c00, c10, fy), mixf_(c01, c11, fy), fz);
+}
+// 8-corner min/max at the sample's base cell (tmpl_comp.wgsl:16-25).
+__device__ __forceinline__ void mmC(const Geo &g, const float *__restrict__ X, float gx0, float gy0, float gz0,
+                                    int hx, int hy, int hz, float &lo, float &hi) {
+  float gx = clampf_(gx0, 0.0f, (float)hx), gy = clampf_(gy0, 0.0f, (float)hy), gz = clampf_(gz0, 0.0f, (float)hz);
+  int bx = (int)floorf(gx), by = (int)floorf(gy), bz = (int)floorf(gz);
+  bx = bx > hx - 1 ? hx - 1 : bx;
+  by = by > hy - 1 ? hy - 1 : by;
+  bz = bz > hz - 1 ? hz - 1 : bz;
+  uint32_t sy = (uint32_t)g.vx, sz = (uint32_t)g.vx * (uint32_t)g.vy;
+  uint32_t i = vidx(g, bx, by, bz);
+  const uint32_t o[8] = {0, 1, sy, sy + 1, sz, sz + 1, sy + sz, sy + sz + 1};
+  float l = 1e30f, h = -1e30f;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    float v = X[i + o[j]];
+    l = fmin_(l, v);
+    h = fmax_(h, v);
+  }
+  lo = l;
+  hi = h;
+}
+
+// velAt / traceBack / traceFwd (vel.wgsl:2-19) in cell units.
+__device__ __forceinline__ void velAt(const Geo &g, const VelPtrs &V, float px, float py, float pz, float &ux,
+                                      float &uy, float &uz) {
+  ux = triC(g, V.u, px, py - 0.5f, pz - 0.5f, g.nx, g.ny - 1, g.nz - 1);
+  uy = triC(g, V.v, px - 0.5f, py, pz - 0.5f, g.nx - 1, g.ny, g.nz - 1);
+  uz = triC(g, V.w, px - 0.5f, py - 0.5f, pz, g.nx - 1, g.ny - 1, g.nz);
+}
+template <int SIGN>
+__device__ __forceinline__ void trace(const Geo &g, const VelPtrs &V, float k, float x, float y, float z, float &ox,
+                                      float &oy, float &oz) {
+  float u1x, u1y, u1z, u2x, u2y, u2z;
+  velAt(g, V, x, y, z, u1x, u1y, u1z);
+  float hk = 0.5f * k;
+  if (SIGN < 0) {
+    velAt(g, V, x - hk * u1x, y - hk * u1y, z - hk * u1z, u2x, u2y, u2z);
+    ox = x - k * u2x; oy = y - k * u2y; oz = z - k * u2z;
+  } else {
+    velAt(g, V, x + hk * u1x, y + hk * u1y, z + hk * u1z, u2x, u2y, u2z);
+    ox = x + k * u2x; oy = y + k * u2y; oz = z + k * u2z;
+  }
+}
+
+// ---------------------------------------------------------------- adv_vel1
+__global__ __launch_bounds__(NT) void k_adv_vel1(const Params *__restrict__ P, Geo g, VelPtrs V,
+                                                  float *__restrict__ hu, float *__restrict__ hv,
+                                                  float *__restrict__ hw) {
+  uint32_t i = blockIdx.x * NT + threadIdx.x;
+  if (i >= g.nv) return;
+  int x, y, z;
+  decode_face(g, i, x, y, z);
+  const float k = P->dt / P->dx;
+  float bx, by, bz;
+  if (x <= g.nx && y <= g.ny - 1 && z <= g.nz - 1) {
+    trace<-1>(g, V, k, (float)x, (float)y + 0.5f, (float)z + 0.5f, bx, by, bz);
+    hu[i] = triC(g, V.u, bx, by - 0.5f, bz - 0.5f, g.nx, g.ny - 1, g.nz - 1);
+  }
+  if (x <= g.nx - 1 && y <= g.ny && z <= g.nz - 1) {
+    trace<-1>(g, V, k, (float)x + 0.5f, (float)y, (float)z + 0.5f, bx, by, bz);
+    hv[i] = triC(g, V.v, bx - 0.5f, by, bz - 0.5f, g.nx - 1, g.ny, g.nz - 1);
+  }
+  if (x <= g.nx - 1 && y <= g.ny - 1 && z <= g.nz) {
+    trace<-1>(g, V, k, (float)x + 0.5f, (float)y + 0.5f, (float)z, bx, by, bz);
+    hw[i] = triC(g, V.w, bx - 0.5f, by - 0.5f, bz, g.nx - 1, g.ny - 1, g.nz);
+  }
+}
+void launch_adv_vel1(const Fields &f, hipStream_t s) {
+  VelPtrs V{f.va[0], f.va[1], f.va[2]};
+  k_adv_vel1<<<nblocks(f.g.nv), NT, 0, s>>>(f.P, f.g, V, f.hat[0], f.hat[1], f.hat[2]);
+}
+
+// ---------------------------------------------------------------- adv_vel2
+__device__ __forceinline__ bool solidC(const Geo &g, const uint8_t *__restrict__ fl, int x, int y, int z) {
+  return inb(g, x, y, z) && fl[sidx(g, x, y, z)] == 1;
+}
+
+__global__ __launch_bounds__(NT) void k_adv_vel2(const Params *__restrict__ P, Geo g, VelPtrs V,
+                                                  const float *__restrict__ hu, const float *__restrict__ hv,
+                                                  const float *__restrict__ hw, float *__restrict__ ou,
+                                                  float *__restrict__ ov, float *__restrict__ ow,
+                                                  const uint8_t *__restrict__ fl) {
+  uint32_t i = blockIdx.x * NT + threadIdx.x;
+  if (i >= g.nv) return;
+  int x, y, z;
+  decode_face(g, i, x, y, z);
+  const float k = P->dt / P->dx;
+  const bool sc = solidC(g, fl, x, y, z);
+#pragma unroll 1
+  for (int c = 0; c < 3; c++) {
+    const int hx = g.nx - 1 + (c == 0), hy = g.ny - 1 + (c == 1), hz = g.nz - 1 + (c == 2);
+    if (x > hx || y > hy || z > hz) continue;
+    float *O = c == 0 ? ou : (c == 1 ? ov : ow);
+    if ((c == 1 && y == 0) || sc || solidC(g, fl, x - (c == 0), y - (c == 1), z - (c == 2))) {
+      O[i] = 0.0f;
+      continue;
+    }
+    const float offx = c == 0 ? 0.0f : 0.5f, offy = c == 1 ? 0.0f : 0.5f, offz = c == 2 ? 0.0f : 0.5f;
+    const float *Vc = c == 0 ? V.u : (c == 1 ? V.v : V.w);
+    const float *Hc = c == 0 ? hu : (c == 1 ? hv : hw);
+    float px = (float)x + offx, py = (float)y + offy, pz = (float)z + offz;
+    float bx, by, bz, fx, fy, fz;
+    trace<-1>(g, V, k, px, py, pz, bx, by, bz);
+    trace<1>(g, V, k, px, py, pz, fx, fy, fz);
+    bx = bx - offx; by = by - offy; bz = bz - offz;
+    fx = fx - offx; fy = fy - offy; fz = fz - offz;
+    float lo, hi;
+    mmC(g, Vc, bx, by, bz, hx, hy, hz, lo, hi);
+    float t = triC(g, Hc, fx, fy, fz, hx, hy, hz);
+    O[i] = clampf_(Hc[i] + 0.5f * (Vc[i] - t), lo, hi);
+  }
+}
+void launch_adv_vel2(const Fields &f, hipStream_t s) {
+  VelPtrs V{f.va[0], f.va[1], f.va[2]};
+  k_adv_vel2<<<nblocks(f.g.nv), NT, 0, s>>>(f.P, f.g, V, f.hat[0], f.hat[1], f.hat[2], f.vb[0], f.vb[1], f.vb[2],
+                                            f.flags);
+}
+
+// ---------------------------------------------------------------- scalars
+// Base cell and weights of a cell-centred trilinear sample (tmpl_scalar.wgsl:2-15).
+struct SSample {
+  uint32_t i;
+  float fx, fy, fz;
+};
+__device__ __forceinline__ SSample sbase(const Geo &g, float gx0, float gy0, float gz0) {
+  float gx = clampf_(gx0, 0.0f, (float)(g.nx - 1)), gy = clampf_(gy0, 0.0f, (float)(g.ny - 1)),
+        gz = clampf_(gz0, 0.0f, (float)(g.nz - 1));
+  int bx = (int)floorf(gx), by = (int)floorf(gy), bz = (int)floorf(gz);
+  bx = bx > g.nx - 2 ? g.nx - 2 : bx;
+  by = by > g.ny - 2 ? g.ny - 2 : by;
+  bz = bz > g.nz - 2 ? g.nz - 2 : bz;
+  SSample s;
+  s.fx = gx - (float)bx;
+  s.fy = gy - (float)by;
+  s.fz = gz - (float)bz;
+  s.i = sidx(g, bx, by, bz);
+  return s;
+}
+__device__ __forceinline__ float4 mix4(float4 a, float4 b, float t) {
+  return make_float4(mixf_(a.x, b.x, t), mixf_(a.y, b.y, t), mixf_(a.z, b.z, t), mixf_(a.w, b.w, t));
+}
+__device__ __forceinline__ float4 triS(const Geo &g, const float4 *__restrict__ X, const SSample &s) {
+  uint32_t sy = (uint32_t)g.nx, sz = (uint32_t)g.nx * (uint32_t)g.ny, i = s.i;
+  float4 c00 = mix4(X[i], X[i + 1], s.fx);
+  float4 c10 = mix4(X[i + sy], X[i + sy + 1], s.fx);
+  float4 c01 = mix4(X[i + sz], X[i + sz + 1], s.fx);
+  float4 c11 = mix4(X[i + sy + sz], X[i + sy + sz + 1], s.fx);
+  return mix4(mix4(c00, c10, s.fy), mix4(c01, c11, s.fy), s.fz);
+}
+__device__ __forceinline__ void mmS(const Geo &g, const float4 *__restrict__ X, const SSample &s, float4 &lo,
+                                    float4 &hi) {
+  uint32_t sy = (uint32_t)g.nx, sz = (uint32_t)g.nx * (uint32_t)g.ny, i = s.i;
+  const uint32_t o[8] = {0, 1, sy, sy + 1, sz, sz + 1, sy + sz, sy + sz + 1};
+  float4 l = make_float4(1e30f, 1e30f, 1e30f, 1e30f), h = make_float4(-1e30f, -1e30f, -1e30f, -1e30f);
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    float4 v = X[i + o[j]];
+    l.x = fmin_(l.x, v.x); l.y = fmin_(l.y, v.y); l.z = fmin_(l.z, v.z); l.w = fmin_(l.w, v.w);
+    h.x = fmax_(h.x, v.x); h.y = fmax_(h.y, v.y); h.z = fmax_(h.z, v.z); h.w = fmax_(h.w, v.w);
+  }
+  lo = l;
+  hi = h;
+}
+// Distance beyond the open side/top boundaries (common.wgsl:79-82).
+__device__ __forceinline__ float outside(const Geo &g, float x, float y, float z) {
+  return fmax_(fmax_(-0.5f - x, x - ((float)g.nx - 0.5f)),
+               fmax_(fmax_(-0.5f - z, z - ((float)g.nz - 0.5f)), y - ((float)g.ny - 0.5f)));
+}
+
+__global__ __launch_bounds__(NT) void k_adv_s1(const Params *__restrict__ P, Geo g, VelPtrs V,
+                                                const float4 *__restrict__ s0, const float4 *__restrict__ s1,
+                                                float4 *__restrict__ h0, float4 *__restrict__ h1) {
+  uint32_t i = blockIdx.x * NT + threadIdx.x;
+  if (i >= g.n) return;
+  int x, y, z;
+  decode_cell(g, i, x, y, z);
+  const float k = P->dt / P->dx;
+  float bx, by, bz;
+  trace<-1>(g, V, k, (float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f, bx, by, bz);
+  bx = bx - 0.5f; by = by - 0.5f; bz = bz - 0.5f;
+  SSample s = sbase(g, bx, by, bz);
+  float w = clampf_(outside(g, bx, by, bz), 0.0f, 1.0f);
+  float4 a = triS(g, s0, s);
+  float4 c = triS(g, s1, s);
+  h0[i] = make_float4(mixf_(a.x, P->t0, w), mixf_(a.y, 0.0f, w), mixf_(a.z, P->yo2, w), mixf_(a.w, 0.0f, w));
+  h1[i] = make_float4(mixf_(c.x, 0.0f, w), mixf_(c.y, 0.0f, w), mixf_(c.z, 0.0f, w), mixf_(c.w, 0.0f, w));
+}
+void launch_adv_s1(const Fields &f, hipStream_t s) {
+  VelPtrs V{f.va[0], f.va[1], f.va[2]};
+  k_adv_s1<<<nblocks(f.g.n), NT, 0, s>>>(f.P, f.g, V, f.s0, f.s1, f.h0, f.h1);
+}
+
+__device__ __forceinline__ float mcl(float r, float s, float t, float lo, float hi) {
+  return clampf_(r + 0.5f * (s - t), lo, hi);
+}
+
+__global__ __launch_bounds__(NT) void k_adv_s2(const Params *__restrict__ P, Geo g, VelPtrs V,
+                                                const float4 *__restrict__ s0, const float4 *__restrict__ s1,
+                                                const float4 *__restrict__ h0, const float4 *__restrict__ h1,
+                                                float4 *__restrict__ o0, float4 *__restrict__ o1) {
+  uint32_t i = blockIdx.x * NT + threadIdx.x;
+  if (i >= g.n) return;
+  int x, y, z;
+  decode_cell(g, i, x, y, z);
+  const float k = P->dt / P->dx;
+  float px = (float)x + 0.5f, py = (float)y + 0.5f, pz = (float)z + 0.5f;
+  float bx, by, bz, fx, fy, fz;
+  trace<-1>(g, V, k, px, py, pz, bx, by, bz);
+  trace<1>(g, V, k, px, py, pz, fx, fy, fz);
+  bx = bx - 0.5f; by = by - 0.5f; bz = bz - 0.5f;
+  fx = fx - 0.5f; fy = fy - 0.5f; fz = fz - 0.5f;
+  float4 r0 = h0[i], r1 = h1[i];
+  if (outside(g, bx, by, bz) <= 0.0f && outside(g, fx, fy, fz) <= 0.0f) {
+    SSample sb = sbase(g, bx, by, bz), sf = sbase(g, fx, fy, fz);
+    float4 l0, u0, l1, u1;
+    mmS(g, s0, sb, l0, u0);
+    mmS(g, s1, sb, l1, u1);
+    float4 t0 = triS(g, h0, sf), t1 = triS(g, h1, sf);
+    float4 a = s0[i], c = s1[i];
+    r0 = make_float4(mcl(r0.x, a.x, t0.x, l0.x, u0.x), mcl(r0.y, a.y, t0.y, l0.y, u0.y),
+                     mcl(r0.z, a.z, t0.z, l0.z, u0.z), mcl(r0.w, a.w, t0.w, l0.w, u0.w));
+    r1 = make_float4(mcl(r1.x, c.x, t1.x, l1.x, u1.x), mcl(r1.y, c.y, t1.y, l1.y, u1.y),
+                     mcl(r1.z, c.z, t1.z, l1.z, u1.z), mcl(r1.w, c.w, t1.w, l1.w, u1.w));
+  }
+  o0[i] = r0;
+  o1[i] = r1;
+}
+void launch_adv_s2(const Fields &f, hipStream_t s) {
+  VelPtrs V{f.va[0], f.va[1], f.va[2]};
+  k_adv_s2<<<nblocks(f.g.n), NT, 0, s>>>(f.P, f.g, V, f.s0, f.s1, f.h0, f.h1, f.s0b, f.s1b);
+}
+
+}  // namespace pyro

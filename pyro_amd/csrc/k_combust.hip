@@ -1,0 +1,391 @@
+// k_combust.hip -- pyrolysis (solid) and gas-phase reaction (react) on gfx950.
+//
+// Reference passes: combust.wgsl solid (:3-137) with rayIrr (:18-39) and
+// faceIrr (:41-48); react (:139-273).  Statistics (stats.wgsl slot map) are
+// reduced per block in FP64 in a fixed order and combined by
+// k_stats_finalize, so every stats step is bit-reproducible.
+#include "kernels.hpp"
+
+namespace pyro {
+
+__device__ __forceinline__ float rhoOf(const Params *__restrict__ P, float T) {
+  return P->rho0 * P->t0 / fmax_(T, 200.0f);
+}
+__device__ __forceinline__ float kapP(const Params *__restrict__ P, float T, float yo, float ys) {
+  float fv = rhoOf(P, T) * fmax_(ys, 0.0f) / RHO_SOOT;
+  return 1862.0f * fv * T + P->kap_gas * clampf_(1.0f - yo / P->yo2, 0.0f, 1.0f);
+}
+__device__ __forceinline__ float cpOf(float T) { return clampf_(1000.0f + 0.24f * (T - 300.0f), 1000.0f, 1450.0f); }
+// Material slot of a SOL cell; slots beyond the 8-entry table are clamped to
+// the last entry so that malformed input cannot read past Params.
+__device__ __forceinline__ uint32_t slotOf(float4 s) {
+  uint32_t k = (uint32_t)s.x;
+  return k > 7u ? 7u : k;
+}
+__device__ __forceinline__ uint32_t kindOf(const Params *__restrict__ P, float4 s) {
+  return (uint32_t)(P->mats[slotOf(s)].a[3] + 0.5f);
+}
+__device__ __forceinline__ bool isSolidS(const Params *__restrict__ P, float4 s) {
+  if (kindOf(P, s) == 2u) return s.y > 0.5f;
+  return s.y + s.z > P->burnout;
+}
+
+// Radiative transfer along one ray of 10 strided steps (combust.wgsl:18-39).
+__device__ float rayIrr(const Params *__restrict__ P, const Geo &g, const float4 *__restrict__ solIn,
+                        const float4 *__restrict__ s0, int px, int py, int pz, int dx_, int dy_, int dz_) {
+  // strides 1,1,1,2,2,3,4,5,7,9 packed one per nibble (no private array)
+  const uint64_t strides = 0x9754322111ull;
+  float tr = 1.0f, acc = 0.0f;
+  float dl = len3((float)dx_, (float)dy_, (float)dz_) * P->dx;
+  for (int st = 0; st < 10; st++) {
+    if (!inb(g, px, py, pz)) break;
+    uint32_t j = sidx(g, px, py, pz);
+    float4 so = solIn[j];
+    if (isSolidS(P, so)) {
+      acc += tr * pow4(so.w);
+      tr = 0.0f;
+      break;
+    }
+    float4 q = s0[j];
+    float T = q.x;
+    const int sd = (int)((strides >> (4 * st)) & 15u);
+    float len = (float)sd * dl;
+    float e = 1.0f - pexp(-kapP(P, T, q.z, q.w) * len);
+    acc += tr * e * pow4(T);
+    tr *= 1.0f - e;
+    px += dx_ * sd;
+    py += dy_ * sd;
+    pz += dz_ * sd;
+  }
+  acc += tr * pow4(P->t0);
+  return SIGMA * acc;
+}
+__device__ __forceinline__ void dirOf(int k, int &x, int &y, int &z) {
+  x = k == 0 ? 1 : (k == 1 ? -1 : 0);
+  y = k == 2 ? 1 : (k == 3 ? -1 : 0);
+  z = k == 4 ? 1 : (k == 5 ? -1 : 0);
+}
+// Hemispherical irradiance: normal ray plus 4 rays at 45 degrees (combust.wgsl:41-48).
+__device__ float faceIrr(const Params *__restrict__ P, const Geo &g, const float4 *__restrict__ solIn,
+                         const float4 *__restrict__ s0, int x, int y, int z, int k) {
+  int dx_, dy_, dz_;
+  dirOf(k, dx_, dy_, dz_);
+  int t1x = 0, t1y = 1, t1z = 0, t2x = 0, t2y = 0, t2z = 1;
+  if (dy_ != 0) { t1x = 1; t1y = 0; t1z = 0; }
+  if (dz_ != 0) { t2x = 1; t2y = 0; t2z = 0; }
+  float r0 = rayIrr(P, g, solIn, s0, x, y, z, dx_, dy_, dz_);
+  float r1 = rayIrr(P, g, solIn, s0, x, y, z, dx_ + t1x, dy_ + t1y, dz_ + t1z);
+  float r2 = rayIrr(P, g, solIn, s0, x, y, z, dx_ - t1x, dy_ - t1y, dz_ - t1z);
+  float r3 = rayIrr(P, g, solIn, s0, x, y, z, dx_ + t2x, dy_ + t2y, dz_ + t2z);
+  float r4 = rayIrr(P, g, solIn, s0, x, y, z, dx_ - t2x, dy_ - t2y, dz_ - t2z);
+  return 0.2f * ((((r0 + r1) + r2) + r3) + r4);
+}
+
+// Surface energy balance and pyrolysis of one solid cell (combust.wgsl:60-128).
+__device__ __noinline__ void solid_cell(const Params *__restrict__ P, const Geo &g, const float4 *__restrict__ solIn,
+                                        const float4 *__restrict__ s0, int x, int y, int z, float4 so, float4 &o,
+                                        float4 &po, uint32_t &fl, float &stPyro, float &stMass, float &stArea) {
+  fl = 1;
+  const Mat &m = P->mats[slotOf(so)];
+  uint32_t kind = kindOf(P, so);
+  float fuel = so.y, chr = so.z, Ts = so.w;
+  if (kind != 2u) {
+    float nF = 0.0f, Tg = 0.0f, Yo = 0.0f, qin = 0.0f, qcond = 0.0f;
+    for (int k = 0; k < 6; k++) {
+      int dx_, dy_, dz_;
+      dirOf(k, dx_, dy_, dz_);
+      int nx_ = x + dx_, ny_ = y + dy_, nz_ = z + dz_;
+      if (ny_ < 0) continue;
+      bool nin = inb(g, nx_, ny_, nz_);
+      if (nin) {
+        float4 sn = solIn[sidx(g, nx_, ny_, nz_)];
+        if (isSolidS(P, sn)) {
+          qcond += m.b[3] / P->dx * (sn.w - Ts);
+          continue;
+        }
+      }
+      nF += 1.0f;
+      if (nin) {
+        float4 q = s0[sidx(g, nx_, ny_, nz_)];
+        Tg += q.x;
+        Yo += q.z;
+        qin += faceIrr(P, g, solIn, s0, nx_, ny_, nz_, k);
+      } else {
+        Tg += P->t0;
+        Yo += P->yo2;
+        qin += SIGMA * pow4(P->t0);
+      }
+    }
+    float layer = kind == 1u ? 0.001f : fmin_(P->layer, P->dx);
+    float cap = m.a[0] * m.b[2] * layer;
+    if (nF > 0.0f) {
+      Tg /= nF;
+      Yo /= nF;
+      qin /= nF;
+      float qnet = m.b[1] * (qin - SIGMA * pow4(Ts)) + P->h_conv * (Tg - Ts) + qcond / nF;
+      if (P->ign_on > 0.5f) {
+        float dI = len3(((float)x + 0.5f) * P->dx - P->ign_x, ((float)y + 0.5f) * P->dx - P->ign_y,
+                        ((float)z + 0.5f) * P->dx - P->ign_z);
+        qnet += P->ign_flux * clampf_(1.0f - dI / (1.5f * P->ign_r), 0.0f, 1.0f);
+      }
+      float mv = 0.0f, mc = 0.0f;
+      if (kind == 1u) {
+        float Tb = m.a[2];
+        mv = smoothstep_(Tb - 60.0f, Tb, Ts) * fmax_(qnet, 0.0f) / m.a[1] * m.c[1];
+        Ts = fmin_(Ts + P->dt * fmax_(qnet - mv * m.a[1], fmin_(qnet, 0.0f)) / cap, Tb);
+        fuel -= mv * nF / P->dx * P->dt;
+      } else {
+        if (fuel > 0.0f) mv = smoothstep_(m.a[2] - 20.0f, m.a[2] + 30.0f, Ts) * fmax_(qnet, 0.0f) / m.a[1] * m.c[1];
+        float chi = kind == 0u ? m.b[0] : 0.0f;
+        float kin = P->char_a * pexp(-P->char_e / (RGAS * Ts)) * chr * layer;
+        float dif = P->h_mass * rhoOf(P, Tg) * Yo / 1.33f;
+        mc = fmin_(kin, dif);
+        Ts += P->dt * (qnet - mv * m.a[1] + 0.5f * mc * P->dh_char) / cap;
+        float virgin = fmin_(mv / (1.0f - chi) * nF / P->dx * P->dt, fuel);
+        fuel -= virgin;
+        chr += virgin * chi - mc * nF / P->dx * P->dt;
+      }
+      stPyro = mv * nF * P->dx * P->dx;
+      stArea = mv > 1e-4f ? nF * P->dx * P->dx : 0.0f;
+      po = make_float4(mv, mc, nF, 0.0f);
+    } else {
+      Ts += P->dt * qcond / (m.a[0] * m.b[2] * P->dx);
+    }
+    Ts = clampf_(Ts, 250.0f, 2000.0f);
+    fuel = fmax_(fuel, 0.0f);
+    chr = fmax_(chr, 0.0f);
+    o = make_float4(so.x, fuel, chr, Ts);
+    if (fuel + chr <= P->burnout) {
+      o = make_float4(0.0f, 0.0f, 0.0f, P->t0);
+      fl = 0;
+      po = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+  }
+  stMass = (o.y + o.z) * P->dx * P->dx * P->dx * (kind == 2u ? 0.0f : 1.0f);
+}
+
+template <bool STATS>
+__global__ __launch_bounds__(NT) void k_solid(const Params *__restrict__ P, Geo g, const float4 *__restrict__ solIn,
+                                               float4 *__restrict__ solOut, float4 *__restrict__ pyr,
+                                               uint8_t *__restrict__ flags, const float4 *__restrict__ s0,
+                                               double *__restrict__ part) {
+  uint32_t i = blockIdx.x * NT + threadIdx.x;
+  float stPyro = 0.0f, stMass = 0.0f, stArea = 0.0f;
+  if (i < g.n) {
+    float4 so = solIn[i];
+    float4 o = so, po = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    uint32_t fl = 0;
+    if (isSolidS(P, so)) {
+      int x, y, z;
+      decode_cell(g, i, x, y, z);
+      solid_cell(P, g, solIn, s0, x, y, z, so, o, po, fl, stPyro, stMass, stArea);
+    }
+    solOut[i] = o;
+    pyr[i] = po;
+    flags[i] = (uint8_t)fl;
+  }
+  if (STATS) {
+    __shared__ double sh[NT / 64];
+    double a = block_sum<NT>((double)stPyro, sh);
+    double b = block_sum<NT>((double)stMass, sh);
+    double c = block_sum<NT>((double)stArea, sh);
+    if (threadIdx.x == 0) {
+      part[blockIdx.x * 3 + 0] = a;
+      part[blockIdx.x * 3 + 1] = b;
+      part[blockIdx.x * 3 + 2] = c;
+    }
+  }
+}
+
+void launch_solid(const Fields &f, StatParts *st, hipStream_t s) {
+  int nb = nblocks(f.g.n);
+  if (st)
+    k_solid<true><<<nb, NT, 0, s>>>(f.P, f.g, f.sol_in, f.sol_out, f.pyr, f.flags, f.s0, st->solid);
+  else
+    k_solid<false><<<nb, NT, 0, s>>>(f.P, f.g, f.sol_in, f.sol_out, f.pyr, f.flags, f.s0, nullptr);
+}
+
+// ---------------------------------------------------------------- react
+__device__ __forceinline__ float spongeSideR(const Params *__restrict__ P, const Geo &g, int x, int z) {
+  int e = x < g.nx - 1 - x ? x : g.nx - 1 - x;
+  int e2 = z < g.nz - 1 - z ? z : g.nz - 1 - z;
+  if (e2 < e) e = e2;
+  return 1.0f - smoothstep_(0.0f, P->sponge, (float)e);
+}
+__device__ __forceinline__ float spongeTopR(const Params *__restrict__ P, const Geo &g, int y) {
+  float h = (float)g.ny;
+  return smoothstep_(h - 1.5f * P->sponge, h - 1.0f, (float)y);
+}
+
+// Blocks tile one y-layer each: grid = (ceil(nx*nz / NT), ny), so the
+// per-layer heat release (stats slots 16+y) is a sum of whole blocks.
+template <bool STATS>
+__global__ __launch_bounds__(NT) void k_react(const Params *__restrict__ P, Geo g, const float4 *__restrict__ s0in,
+                                               const float4 *__restrict__ s1in, float4 *__restrict__ s0out,
+                                               float4 *__restrict__ s1out, const uint8_t *__restrict__ flags,
+                                               const float4 *__restrict__ sol, const float4 *__restrict__ pyr,
+                                               const float4 *__restrict__ curl, double *__restrict__ part,
+                                               float *__restrict__ part_t) {
+  const int y = blockIdx.y;
+  const uint32_t i2 = blockIdx.x * NT + threadIdx.x;
+  float stH = 0.0f, stR = 0.0f, stS = 0.0f, stF = 0.0f, stM = 0.0f, stT = 0.0f, stL = 0.0f;
+  if (i2 < (uint32_t)g.nx * (uint32_t)g.nz) {
+    uint32_t zq = fdiv(i2, g.dnx);
+    const int x = (int)(i2 - zq * (uint32_t)g.nx), z = (int)zq;
+    const uint32_t i = sidx(g, x, y, z);
+    const float dx = P->dx, dt = P->dt;
+    if (flags[i] == 1) {
+      s0out[i] = make_float4(sol[i].w, 0.0f, P->yo2, 0.0f);
+      s1out[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    } else {
+      float4 a = s0in[i];
+      float4 b = s1in[i];
+      float nut = (P->cs * dx) * (P->cs * dx) * curl[i].w;
+      if (P->cs > 0.0f) {
+        float kd = fmin_(nut / 0.5f * dt / (dx * dx), 0.15f);
+        float lx = 0.0f, ly = 0.0f, lz = 0.0f, lw = 0.0f, lb = 0.0f;
+        for (int k = 0; k < 6; k++) {
+          int ddx, ddy, ddz;
+          dirOf(k, ddx, ddy, ddz);
+          int nx_ = x + ddx, ny_ = y + ddy, nz_ = z + ddz;
+          if (inb(g, nx_, ny_, nz_)) {
+            uint32_t j = sidx(g, nx_, ny_, nz_);
+            if (flags[j] != 1) {
+              float4 q = s0in[j];
+              lx += q.x - a.x; ly += q.y - a.y; lz += q.z - a.z; lw += q.w - a.w;
+              lb += s1in[j].x - b.x;
+            }
+          } else if (ny_ >= 0) {
+            lx += P->t0 - a.x; ly += 0.0f - a.y; lz += P->yo2 - a.z; lw += 0.0f - a.w;
+            lb -= b.x;
+          }
+        }
+        a.x += kd * lx; a.y += kd * ly; a.z += kd * lz; a.w += kd * lw;
+        b.x += kd * lb;
+      }
+      float T = a.x;
+      float yf = fmax_(a.y, 0.0f);
+      float yo = clampf_(a.z, 0.0f, P->yo2);
+      float ys = fmax_(a.w, 0.0f);
+      float ym = fmax_(b.x, 0.0f);
+      float S = 0.0f;
+      for (int k = 0; k < 6; k++) {
+        int ddx, ddy, ddz;
+        dirOf(k, ddx, ddy, ddz);
+        int nx_ = x + ddx, ny_ = y + ddy, nz_ = z + ddz;
+        if (!inb(g, nx_, ny_, nz_)) continue;
+        uint32_t j = sidx(g, nx_, ny_, nz_);
+        if (flags[j] != 1) continue;
+        float4 pr = pyr[j];
+        float Ts = sol[j].w;
+        float rho = rhoOf(P, T);
+        if (pr.x > 0.0f) {
+          float dm = pr.x * dt / (rho * dx);
+          float inv = 1.0f / (1.0f + dm);
+          yf = (yf + dm) * inv;
+          yo *= inv;
+          ys *= inv;
+          ym *= inv;
+          T = (T + dm * Ts) * inv;
+          S += pr.x / (rho * dx);
+        }
+        T += P->h_conv * (Ts - T) * dt / (rho * cpOf(T) * dx);
+        if (pr.y > 0.0f) {
+          yo = fmax_(yo - 1.33f * pr.y * dt / (rho * dx), 0.0f);
+          T += 0.5f * P->dh_char * pr.y * dt / (rho * cpOf(T) * dx);
+        }
+      }
+      if (P->ign_on > 0.5f) {
+        float dI = len3(((float)x + 0.5f) * dx - P->ign_x, ((float)y + 0.5f) * dx - P->ign_y,
+                        ((float)z + 0.5f) * dx - P->ign_z);
+        float w = clampf_(1.0f - dI / P->ign_r, 0.0f, 1.0f);
+        if (w > 0.0f) {
+          T = fmax_(T, mixf_(P->t0, P->ign_temp, smoothstep_(0.0f, 0.5f, w)));
+          yf += P->ign_fuel * w * dt;
+        }
+      }
+      float rho = rhoOf(P, T);
+      float tauU = 0.049f * dx * dx / (nut + 1e-6f);
+      float tauG = sqrtf(2.0f * dx / P->g);
+      float tau = fmax_(fmin_(tauU, tauG), P->tau_min);
+      float gate = smoothstep_(P->t_ign - 100.0f, P->t_ign + 100.0f, T);
+      float lim = fmin_(yf, yo / P->s_o2);
+      float dY = lim * (1.0f - pexp(-gate * dt / tau));
+      yf -= dY;
+      yo = fmax_(yo - P->s_o2 * dY, 0.0f);
+      T += dY * P->dhc / cpOf(T + 0.5f * dY * P->dhc / cpOf(T));
+      float q = rho * dY * P->dhc / dt;
+      float rich = smoothstep_(0.0f, 0.03f, yf);
+      ys += P->soot_form * dY * (0.25f + 0.75f * rich);
+      float oxg = smoothstep_(P->soot_ox_t, P->soot_ox_t + 300.0f, T) * (yo / P->yo2);
+      ys *= pexp(-P->soot_ox * oxg * dt);
+      float cold = 1.0f - smoothstep_(450.0f, 650.0f, T);
+      float cv = yf * fmin_(P->smoke_conv * cold * dt, 1.0f);
+      yf -= cv;
+      ym += cv;
+      rho = rhoOf(P, T);
+      float kap = kapP(P, T, yo, ys);
+      float tl = kap * P->rad_len;
+      float esc = tl > 1e-3f ? (1.0f - pexp(-tl)) / tl : 1.0f;
+      float prad = 4.0f * kap * esc * SIGMA * (pow4(T) - pow4(P->t0));
+      float dTr = fmin_(prad * dt / (rho * cpOf(T)), 0.5f * fmax_(T - P->t0, 0.0f));
+      T -= dTr;
+      float prr = dTr * rho * cpOf(T) / dt;
+      S += (q - prr) / (rho * cpOf(T) * T);
+      float sp = fmax_(spongeSideR(P, g, x, z), spongeTopR(P, g, y));
+      float dec = 1.0f - pexp(-dt * 4.0f * sp);
+      T = mixf_(T, P->t0, dec);
+      yf *= 1.0f - dec;
+      yo = mixf_(yo, P->yo2, dec);
+      ys *= 1.0f - dec;
+      ym *= 1.0f - dec;
+      T = clampf_(T, 250.0f, 2600.0f);
+      s0out[i] = make_float4(T, yf, yo, ys);
+      s1out[i] = make_float4(ym, q, S, 0.0f);
+      if (STATS) {
+        const float dV = dx * dx * dx;
+        rho = rhoOf(P, T);
+        stH = q * dV;
+        stR = prr * dV;
+        stS = rho * ys * dV;
+        stF = rho * yf * dV;
+        stM = rho * ym * dV;
+        stT = T;
+        stL = stH > 1.0f ? stH : 0.0f;
+      }
+    }
+  }
+  if (STATS) {
+    __shared__ double sh[NT / 64];
+    __shared__ float shf[NT / 64];
+    double vH = block_sum<NT>((double)stH, sh);
+    double vR = block_sum<NT>((double)stR, sh);
+    double vS = block_sum<NT>((double)stS, sh);
+    double vF = block_sum<NT>((double)stF, sh);
+    double vM = block_sum<NT>((double)stM, sh);
+    double vL = block_sum<NT>((double)stL, sh);
+    float vT = block_max<NT>(stT, shf);
+    if (threadIdx.x == 0) {
+      size_t b = (size_t)y * gridDim.x + blockIdx.x;
+      part[b * 6 + 0] = vH;
+      part[b * 6 + 1] = vR;
+      part[b * 6 + 2] = vS;
+      part[b * 6 + 3] = vF;
+      part[b * 6 + 4] = vM;
+      part[b * 6 + 5] = vL;
+      part_t[b] = vT;
+    }
+  }
+}
+
+void launch_react(const Fields &f, StatParts *st, hipStream_t s) {
+  dim3 grid((unsigned)((f.g.nx * f.g.nz + NT - 1) / NT), (unsigned)f.g.ny);
+  if (st)
+    k_react<true><<<grid, NT, 0, s>>>(f.P, f.g, f.s0b, f.s1b, f.s0, f.s1, f.flags, f.sol_out, f.pyr, f.curl,
+                                      st->react, st->react_t);
+  else
+    k_react<false><<<grid, NT, 0, s>>>(f.P, f.g, f.s0b, f.s1b, f.s0, f.s1, f.flags, f.sol_out, f.pyr, f.curl,
+                                       nullptr, nullptr);
+}
+
+}  // namespace pyro

@@ -1,0 +1,68 @@
+// kernels.hpp -- host-side launchers of the gfx950 kernels (one per reference
+// compute pass).  All launches are stream-ordered and capturable in a
+// hipGraph; none allocates.
+#pragma once
+#include "common.hpp"
+
+namespace pyro {
+
+struct Fields {
+  const Params *P;          // device copy of the Params blob
+  Geo g;
+  float *va[3], *vb[3];     // velocity ping-pong: va = vel[cur], vb = vel[1-cur]
+  float *hat[3];
+  float4 *s0, *s1, *s0b, *s1b, *h0, *h1, *sol_in, *sol_out, *pyr, *curl;
+  uint8_t *flags;           // 1 = solid (u8 copy of the reference u32 flags)
+  float *phi0, *b0;         // multigrid level 0
+};
+
+// Per-block FP64 partials of the step statistics (stats.wgsl slot map).
+struct StatParts {
+  double *solid;   // [blocks][3]: pyro, solid mass, area        (slots 2,7,9)
+  float *curl;     // [blocks]: max speed                         (slot 5)
+  double *react;   // [ny][gx][6]: H,R,S,F,M,layerH; Tmax below   (0,1,3,6,8,16+y)
+  float *react_t;  // [ny][gx]: Tmax                              (slot 4)
+  double *resid;   // [blocks][2]: sum r^2, sum b^2               (10,11)
+  double *out;     // [1040]
+  int n_solid, n_curl, react_gx, n_resid;
+};
+
+constexpr int NT = 256;  // threads per block of the element-wise kernels
+
+void launch_solid(const Fields &f, StatParts *st, hipStream_t s);
+void launch_curl(const Fields &f, StatParts *st, hipStream_t s);
+void launch_adv_vel1(const Fields &f, hipStream_t s);
+void launch_adv_vel2(const Fields &f, hipStream_t s);
+void launch_adv_s1(const Fields &f, hipStream_t s);
+void launch_adv_s2(const Fields &f, hipStream_t s);
+void launch_react(const Fields &f, StatParts *st, hipStream_t s);
+void launch_forces(const Fields &f, hipStream_t s);
+void launch_divergence(const Fields &f, hipStream_t s);
+void launch_project(const Fields &f, hipStream_t s);
+void launch_stats_finalize(const Geo &g, const StatParts &st, int with_resid, hipStream_t s);
+void launch_velocity_noise(const Fields &f, float amp, uint32_t seed, hipStream_t s);
+void launch_set_step(Params *P, float dt, float time, float ign_on, uint32_t frame, hipStream_t s);
+
+// Multigrid on one level hierarchy (mg.wgsl).
+struct MgLevel {
+  int d[3];
+  float h, hh;   // spacing and h*h (f32)
+  float *phi, *b;
+  uint8_t *f;
+  uint32_t n;
+};
+void launch_mg_smooth(const MgLevel &L, int color, float omega, hipStream_t s);
+void launch_mg_restrict(const MgLevel &F, const MgLevel &C, hipStream_t s);
+void launch_mg_prolong(const MgLevel &C, const MgLevel &F, hipStream_t s);
+// All `iters` red-black sweeps of the coarsest level in one launch.
+void launch_mg_coarse(const MgLevel &L, int iters, float omega, hipStream_t s);
+bool mg_coarse_fits_lds(const MgLevel &L);
+void launch_mg_resid(const MgLevel &L, StatParts *st, hipStream_t s);
+int mg_resid_blocks(const MgLevel &L);
+
+// Debug / test: apply device math to an array (0 exp, 1 sqrt, 2 div by in2).
+void launch_debug_math(int fn, const float *in, const float *in2, float *out, uint32_t n, hipStream_t s);
+
+inline int nblocks(uint32_t n) { return (int)((n + NT - 1) / NT); }
+
+}  // namespace pyro

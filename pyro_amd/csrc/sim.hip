@@ -1,0 +1,536 @@
+// sim.hip -- PyroSim: device state, step schedule, hipGraph replay and the
+// C ABI of include/pyro_hip.h.
+//
+// Mirrors the reference's Sim (src/sim.rs): buffers and MG hierarchy of
+// Sim::new (:108-232), the dispatch order of Sim::step (:246-423), the
+// V-cycle of Sim::vcycles (:451-491) and the stats readback of
+// Sim::read_stats (:510-530).  Each step is one hipGraph replay on the
+// handle's stream, preceded by a one-thread kernel that writes dt, time,
+// ign_on and frame into the device copy of Params (the reference's 576-byte
+// write_buffer, sim.rs:250-251).  Graphs are keyed by the ping-pong roles
+// (vel[cur], sol[sol_cur]) and by want_stats, so at most 8 exist.
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "pyro_hip.h"
+
+using namespace pyro;
+
+namespace {
+thread_local std::string g_err;
+int fail(int code, const std::string &m) {
+  g_err = m;
+  return code;
+}
+#define HIPCHK(x)                                                                          \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess) return fail(PYRO_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int MG_LEVELS = 4;     // sim.rs:13
+constexpr int COARSE_ITERS = 80;  // sim.rs:14
+}  // namespace
+
+struct PyroSim {
+  Params hp;
+  Params *dP = nullptr;
+  Geo g;
+  float *vel[2][3] = {};
+  float *hat[3] = {};
+  float4 *s0 = nullptr, *s1 = nullptr, *s0b = nullptr, *s1b = nullptr, *h0 = nullptr, *h1 = nullptr;
+  float4 *sol[2] = {}, *pyr = nullptr, *curl = nullptr;
+  uint8_t *flags = nullptr;
+  MgLevel lv[8];
+  int nl = 0;
+  StatParts st;
+  int cur = 0, sol_cur = 0;
+  uint32_t cycles = 4;
+  float ign_until = 0.0f, time = 0.0f, dt = 0.0f;
+  uint32_t frame = 0;
+  hipStream_t stream = nullptr;
+  bool use_graph = true;
+  hipGraphExec_t gexec[2][2][2] = {};
+  bool in_step = false, step_stats = false;
+  std::vector<void *> allocs;
+  int *d_flag = nullptr;
+};
+
+namespace {
+
+int dalloc(PyroSim *s, void **p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes < 16 ? 16 : bytes);
+  if (e != hipSuccess) return fail(PYRO_E_OOM, std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+  s->allocs.push_back(*p);
+  e = hipMemset(*p, 0, bytes < 16 ? 16 : bytes);
+  if (e != hipSuccess) return fail(PYRO_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+  return 0;
+}
+template <class T>
+int dalloc_t(PyroSim *s, T **p, size_t count) {
+  return dalloc(s, reinterpret_cast<void **>(p), count * sizeof(T));
+}
+
+Fields fields_of(PyroSim *s) {
+  Fields f;
+  f.P = s->dP;
+  f.g = s->g;
+  for (int k = 0; k < 3; k++) {
+    f.va[k] = s->vel[s->cur][k];
+    f.vb[k] = s->vel[1 - s->cur][k];
+    f.hat[k] = s->hat[k];
+  }
+  f.s0 = s->s0; f.s1 = s->s1; f.s0b = s->s0b; f.s1b = s->s1b; f.h0 = s->h0; f.h1 = s->h1;
+  f.sol_in = s->sol[s->sol_cur];
+  f.sol_out = s->sol[1 - s->sol_cur];
+  f.pyr = s->pyr; f.curl = s->curl; f.flags = s->flags;
+  f.phi0 = s->lv[0].phi; f.b0 = s->lv[0].b;
+  return f;
+}
+
+void enqueue_smooth(const MgLevel &L, int iters, float omega, hipStream_t st) {
+  for (int it = 0; it < iters; it++) {
+    launch_mg_smooth(L, 0, omega, st);
+    launch_mg_smooth(L, 1, omega, st);
+  }
+}
+
+// sim.rs:451-491
+void enqueue_vcycles(PyroSim *s, hipStream_t st) {
+  const int nl = s->nl;
+  for (uint32_t c = 0; c < s->cycles; c++) {
+    for (int l = 0; l < nl - 1; l++) {
+      enqueue_smooth(s->lv[l], 2, 1.0f, st);
+      launch_mg_restrict(s->lv[l], s->lv[l + 1], st);
+    }
+    launch_mg_coarse(s->lv[nl - 1], COARSE_ITERS, 1.6f, st);
+    for (int l = nl - 2; l >= 0; l--) {
+      launch_mg_prolong(s->lv[l + 1], s->lv[l], st);
+      enqueue_smooth(s->lv[l], 2, 1.0f, st);
+    }
+  }
+}
+
+void enqueue_stage(PyroSim *s, int stage, bool stats, hipStream_t st) {
+  Fields f = fields_of(s);
+  StatParts *sp = stats ? &s->st : nullptr;
+  switch (stage) {
+    case PYRO_STAGE_SOLID: launch_solid(f, sp, st); break;
+    case PYRO_STAGE_CURL: launch_curl(f, sp, st); break;
+    case PYRO_STAGE_ADV_VEL1: launch_adv_vel1(f, st); break;
+    case PYRO_STAGE_ADV_VEL2: launch_adv_vel2(f, st); break;
+    case PYRO_STAGE_ADV_S1: launch_adv_s1(f, st); break;
+    case PYRO_STAGE_ADV_S2: launch_adv_s2(f, st); break;
+    case PYRO_STAGE_REACT: launch_react(f, sp, st); break;
+    case PYRO_STAGE_FORCES: launch_forces(f, st); break;
+    case PYRO_STAGE_DIVERGENCE: launch_divergence(f, st); break;
+    case PYRO_STAGE_VCYCLES: enqueue_vcycles(s, st); break;
+    case PYRO_STAGE_RESID: if (stats) launch_mg_resid(s->lv[0], &s->st, st); break;
+    case PYRO_STAGE_PROJECT: launch_project(f, st); break;
+    default: break;
+  }
+}
+
+// sim.rs:394-414, plus the FP64 stats combination on stats steps.
+void enqueue_step_body(PyroSim *s, bool stats, hipStream_t st) {
+  for (int k = PYRO_STAGE_SOLID; k <= PYRO_STAGE_VCYCLES; k++) enqueue_stage(s, k, stats, st);
+  if (stats) enqueue_stage(s, PYRO_STAGE_RESID, true, st);
+  enqueue_stage(s, PYRO_STAGE_PROJECT, stats, st);
+  if (stats) launch_stats_finalize(s->g, s->st, 1, st);
+}
+
+int get_graph(PyroSim *s, bool stats, hipGraphExec_t *out) {
+  hipGraphExec_t &ge = s->gexec[s->cur][s->sol_cur][stats ? 1 : 0];
+  if (!ge) {
+    hipGraph_t graph;
+    HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_step_body(s, stats, s->stream);
+    hipError_t le = hipGetLastError();
+    HIPCHK(hipStreamEndCapture(s->stream, &graph));
+    if (le != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      return fail(PYRO_E_HIP, std::string("launch during capture: ") + hipGetErrorString(le));
+    }
+    hipError_t ie = hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+      ge = nullptr;
+      return fail(PYRO_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+    }
+  }
+  *out = ge;
+  return 0;
+}
+
+int sync_check(PyroSim *s) {
+  hipError_t e = hipStreamSynchronize(s->stream);
+  if (e != hipSuccess) return fail(PYRO_E_HIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+  e = hipGetLastError();
+  if (e != hipSuccess) return fail(PYRO_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  return 0;
+}
+
+struct FieldRef {
+  void *ptr = nullptr;
+  size_t dev_bytes = 0, host_bytes = 0;
+  bool u8flags = false;
+};
+FieldRef field_ref(PyroSim *s, int field) {
+  FieldRef r;
+  const size_t n4 = (size_t)s->g.n * 16, nv = (size_t)s->g.nv * 4;
+  auto v4 = [&](float4 *p) { r.ptr = p; r.dev_bytes = r.host_bytes = n4; };
+  auto vf = [&](float *p) { r.ptr = p; r.dev_bytes = r.host_bytes = nv; };
+  switch (field) {
+    case PYRO_FIELD_S0: v4(s->s0); return r;
+    case PYRO_FIELD_S1: v4(s->s1); return r;
+    case PYRO_FIELD_S0B: v4(s->s0b); return r;
+    case PYRO_FIELD_S1B: v4(s->s1b); return r;
+    case PYRO_FIELD_H0: v4(s->h0); return r;
+    case PYRO_FIELD_H1: v4(s->h1); return r;
+    case PYRO_FIELD_SOL: v4(s->sol[s->sol_cur]); return r;
+    case PYRO_FIELD_SOL_OUT: v4(s->sol[1 - s->sol_cur]); return r;
+    case PYRO_FIELD_PYR: v4(s->pyr); return r;
+    case PYRO_FIELD_CURL: v4(s->curl); return r;
+    case PYRO_FIELD_FLAGS:
+      r.ptr = s->flags; r.dev_bytes = s->g.n; r.host_bytes = (size_t)s->g.n * 4; r.u8flags = true;
+      return r;
+    case PYRO_FIELD_U: case PYRO_FIELD_V: case PYRO_FIELD_W: vf(s->vel[s->cur][field - PYRO_FIELD_U]); return r;
+    case PYRO_FIELD_UB: case PYRO_FIELD_VB: case PYRO_FIELD_WB: vf(s->vel[1 - s->cur][field - PYRO_FIELD_UB]); return r;
+    case PYRO_FIELD_HU: case PYRO_FIELD_HV: case PYRO_FIELD_HW: vf(s->hat[field - PYRO_FIELD_HU]); return r;
+    default: break;
+  }
+  if (field >= PYRO_FIELD_LEVEL_BASE) {
+    int l = (field - PYRO_FIELD_LEVEL_BASE) / 4, q = (field - PYRO_FIELD_LEVEL_BASE) % 4;
+    if (l < s->nl && q < 3) {
+      const MgLevel &L = s->lv[l];
+      if (q == 0) { r.ptr = L.phi; r.dev_bytes = r.host_bytes = (size_t)L.n * 4; }
+      else if (q == 1) { r.ptr = L.b; r.dev_bytes = r.host_bytes = (size_t)L.n * 4; }
+      else { r.ptr = L.f; r.dev_bytes = L.n; r.host_bytes = (size_t)L.n * 4; r.u8flags = true; }
+    }
+  }
+  return r;
+}
+
+__global__ void k_nonfinite(const float *p, size_t n, int *flag) {
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * 256;
+  int bad = 0;
+  for (; i < n; i += stride) {
+    float v = p[i];
+    bad |= !(v - v == 0.0f);
+  }
+  if (bad) flag[0] = 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *pyro_last_error(void) { return g_err.c_str(); }
+
+int pyro_sim_create(const void *params, size_t params_bytes, const float *sol_init, const float *s0_init,
+                    uint32_t pressure_cycles, float ign_until, uint32_t cflags, PyroSim **out) {
+  if (!params || !sol_init || !out) return fail(PYRO_E_INVALID, "pyro_sim_create: null argument");
+  if (params_bytes != PYRO_PARAMS_BYTES) return fail(PYRO_E_INVALID, "pyro_sim_create: Params blob must be 576 bytes");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(PYRO_E_NODEVICE, "no HIP device available");
+  Params hp;
+  std::memcpy(&hp, params, sizeof(Params));
+  if (hp.nx < 2 || hp.ny < 2 || hp.nz < 2) return fail(PYRO_E_INVALID, "grid dims must be >= 2");
+  uint64_t n64 = (uint64_t)hp.nx * hp.ny * hp.nz, nv64 = (uint64_t)(hp.nx + 1) * (hp.ny + 1) * (hp.nz + 1);
+  if (nv64 >= (1ull << 31)) return fail(PYRO_E_INVALID, "grid too large for 32-bit indexing");
+  PyroSim *s = new PyroSim();
+  s->hp = hp;
+  s->hp.dt = 0.005f;
+  Geo &g = s->g;
+  g.nx = (int)hp.nx; g.ny = (int)hp.ny; g.nz = (int)hp.nz;
+  g.vx = g.nx + 1; g.vy = g.ny + 1;
+  g.n = (uint32_t)n64; g.nv = (uint32_t)nv64;
+  g.dnx = make_fastdiv((uint32_t)g.nx); g.dny = make_fastdiv((uint32_t)g.ny);
+  g.dvx = make_fastdiv((uint32_t)g.vx); g.dvy = make_fastdiv((uint32_t)g.vy);
+  s->cycles = pressure_cycles;
+  s->ign_until = ign_until;
+  s->use_graph = !(cflags & PYRO_CREATE_NO_GRAPH);
+  auto bail = [&](int rc) { pyro_sim_destroy(s); return rc; };
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(PYRO_E_HIP, "hipStreamCreate failed"));
+  int rc = 0;
+  rc |= dalloc_t(s, &s->dP, 1);
+  for (int c = 0; c < 2 && !rc; c++)
+    for (int k = 0; k < 3 && !rc; k++) rc |= dalloc_t(s, &s->vel[c][k], g.nv);
+  for (int k = 0; k < 3 && !rc; k++) rc |= dalloc_t(s, &s->hat[k], g.nv);
+  float4 **f4[] = {&s->s0, &s->s1, &s->s0b, &s->s1b, &s->h0, &s->h1, &s->sol[0], &s->sol[1], &s->pyr, &s->curl};
+  for (auto p : f4)
+    if (!rc) rc |= dalloc_t(s, p, g.n);
+  if (!rc) rc |= dalloc_t(s, &s->flags, g.n);
+  if (!rc) rc |= dalloc_t(s, &s->d_flag, 1);
+  // MG hierarchy (sim.rs:142-168)
+  int d[3] = {g.nx, g.ny, g.nz};
+  float h = hp.dx;
+  for (int l = 0; l < MG_LEVELS && !rc; l++) {
+    MgLevel &L = s->lv[s->nl++];
+    L.d[0] = d[0]; L.d[1] = d[1]; L.d[2] = d[2];
+    L.h = h;
+    L.hh = h * h;
+    L.n = (uint32_t)d[0] * (uint32_t)d[1] * (uint32_t)d[2];
+    rc |= dalloc_t(s, &L.phi, L.n);
+    if (!rc) rc |= dalloc_t(s, &L.b, L.n);
+    if (l == 0) L.f = s->flags;
+    else if (!rc) rc |= dalloc_t(s, &L.f, L.n);
+    int mn = d[0] < d[1] ? d[0] : d[1];
+    mn = d[2] < mn ? d[2] : mn;
+    if (mn < 16) break;
+    for (int q = 0; q < 3; q++) d[q] = (d[q] + 1) / 2;
+    h *= 2.0f;
+  }
+  // stats partials
+  StatParts &st = s->st;
+  st.n_solid = nblocks(g.n);
+  st.n_curl = nblocks(g.n);
+  st.react_gx = (g.nx * g.nz + NT - 1) / NT;
+  st.n_resid = mg_resid_blocks(s->lv[0]);
+  if (!rc) rc |= dalloc_t(s, &st.solid, (size_t)st.n_solid * 3);
+  if (!rc) rc |= dalloc_t(s, &st.curl, (size_t)st.n_curl);
+  if (!rc) rc |= dalloc_t(s, &st.react, (size_t)g.ny * st.react_gx * 6);
+  if (!rc) rc |= dalloc_t(s, &st.react_t, (size_t)g.ny * st.react_gx);
+  if (!rc) rc |= dalloc_t(s, &st.resid, (size_t)st.n_resid * 2);
+  if (!rc) rc |= dalloc_t(s, &st.out, STATS_N);
+  if (rc) return bail(rc < 0 ? rc : PYRO_E_OOM);
+  // initial fields (sim.rs:133-140)
+  std::vector<float> amb;
+  const float *s0src = s0_init;
+  if (!s0src) {
+    amb.resize((size_t)g.n * 4);
+    for (size_t i = 0; i < g.n; i++) {
+      amb[4 * i] = hp.t0; amb[4 * i + 1] = 0.0f; amb[4 * i + 2] = hp.yo2; amb[4 * i + 3] = 0.0f;
+    }
+    s0src = amb.data();
+  }
+  if (hipMemcpy(s->dP, &s->hp, sizeof(Params), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(s->s0, s0src, (size_t)g.n * 16, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(s->sol[0], sol_init, (size_t)g.n * 16, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(s->sol[1], sol_init, (size_t)g.n * 16, hipMemcpyHostToDevice) != hipSuccess)
+    return bail(fail(PYRO_E_HIP, "initial upload failed"));
+  *out = s;
+  return 0;
+}
+
+int pyro_sim_destroy(PyroSim *s) {
+  if (!s) return 0;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (auto &a : s->gexec)
+    for (auto &b : a)
+      for (auto &c : b)
+        if (c) (void)hipGraphExecDestroy(c);
+  for (void *p : s->allocs) (void)hipFree(p);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return 0;
+}
+
+int pyro_sim_begin_step(PyroSim *s, float dt, int want_stats) {
+  if (!s) return fail(PYRO_E_INVALID, "null sim");
+  if (s->in_step) return fail(PYRO_E_STATE, "begin_step inside a step");
+  // sim.rs:247-251
+  float ign_on = s->time < s->ign_until ? 1.0f : 0.0f;
+  s->hp.dt = dt;
+  s->hp.time = s->time;
+  s->hp.ign_on = ign_on;
+  s->hp.frame = s->frame;
+  launch_set_step(s->dP, dt, s->time, ign_on, s->frame, s->stream);
+  s->dt = dt;
+  s->in_step = true;
+  s->step_stats = want_stats != 0;
+  return 0;
+}
+
+int pyro_sim_run_stage(PyroSim *s, int stage) {
+  if (!s) return fail(PYRO_E_INVALID, "null sim");
+  if (!s->in_step) return fail(PYRO_E_STATE, "run_stage outside begin_step/end_step");
+  if (stage < 0 || stage >= PYRO_STAGE_COUNT) return fail(PYRO_E_INVALID, "bad stage id");
+  enqueue_stage(s, stage, s->step_stats, s->stream);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(PYRO_E_HIP, std::string("launch: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int pyro_sim_end_step(PyroSim *s) {
+  if (!s) return fail(PYRO_E_INVALID, "null sim");
+  if (!s->in_step) return fail(PYRO_E_STATE, "end_step without begin_step");
+  if (s->step_stats) launch_stats_finalize(s->g, s->st, 1, s->stream);
+  // sim.rs:419-422
+  s->cur = 1 - s->cur;
+  s->sol_cur = 1 - s->sol_cur;
+  s->time = s->time + s->dt;
+  s->frame += 1;
+  s->in_step = false;
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(PYRO_E_HIP, std::string("launch: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int pyro_sim_step(PyroSim *s, float dt, int want_stats) {
+  if (!s) return fail(PYRO_E_INVALID, "null sim");
+  int rc = pyro_sim_begin_step(s, dt, want_stats);
+  if (rc) return rc;
+  bool stats = want_stats != 0;
+  if (s->use_graph) {
+    hipGraphExec_t ge;
+    rc = get_graph(s, stats, &ge);
+    if (rc) { s->in_step = false; return rc; }
+    hipError_t e = hipGraphLaunch(ge, s->stream);
+    if (e != hipSuccess) { s->in_step = false; return fail(PYRO_E_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(e)); }
+  } else {
+    enqueue_step_body(s, stats, s->stream);
+  }
+  // the finalize is part of the body; end the step without a second one
+  s->step_stats = false;
+  return pyro_sim_end_step(s);
+}
+
+int pyro_sim_read_stats(PyroSim *s, double *out, size_t n) {
+  if (!s || !out) return fail(PYRO_E_INVALID, "null argument");
+  if (n != PYRO_STATS_N) return fail(PYRO_E_INVALID, "stats buffer must hold 1040 doubles");
+  int rc = sync_check(s);
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(out, s->st.out, PYRO_STATS_N * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pyro_sim_read_field(PyroSim *s, int field, void *dst, size_t bytes) {
+  if (!s || !dst) return fail(PYRO_E_INVALID, "null argument");
+  FieldRef r = field_ref(s, field);
+  if (!r.ptr) return fail(PYRO_E_INVALID, "unknown field id " + std::to_string(field));
+  if (bytes != r.host_bytes) return fail(PYRO_E_INVALID, "field size mismatch: expected " + std::to_string(r.host_bytes));
+  int rc = sync_check(s);
+  if (rc) return rc;
+  if (r.u8flags) {
+    std::vector<uint8_t> tmp(r.dev_bytes);
+    HIPCHK(hipMemcpy(tmp.data(), r.ptr, r.dev_bytes, hipMemcpyDeviceToHost));
+    uint32_t *o = static_cast<uint32_t *>(dst);
+    for (size_t i = 0; i < r.dev_bytes; i++) o[i] = tmp[i];
+  } else {
+    HIPCHK(hipMemcpy(dst, r.ptr, bytes, hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+int pyro_sim_write_field(PyroSim *s, int field, const void *src, size_t bytes) {
+  if (!s || !src) return fail(PYRO_E_INVALID, "null argument");
+  FieldRef r = field_ref(s, field);
+  if (!r.ptr) return fail(PYRO_E_INVALID, "unknown field id " + std::to_string(field));
+  if (bytes != r.host_bytes) return fail(PYRO_E_INVALID, "field size mismatch: expected " + std::to_string(r.host_bytes));
+  int rc = sync_check(s);
+  if (rc) return rc;
+  if (r.u8flags) {
+    std::vector<uint8_t> tmp(r.dev_bytes);
+    const uint32_t *in = static_cast<const uint32_t *>(src);
+    for (size_t i = 0; i < r.dev_bytes; i++) tmp[i] = in[i] == 1u ? 1 : 0;
+    HIPCHK(hipMemcpy(r.ptr, tmp.data(), r.dev_bytes, hipMemcpyHostToDevice));
+  } else {
+    HIPCHK(hipMemcpy(r.ptr, src, bytes, hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
+int pyro_sim_info(const PyroSim *s, uint32_t dims[3], uint64_t *cells, double *time, uint32_t *frame,
+                  uint32_t *levels) {
+  if (!s) return fail(PYRO_E_INVALID, "null sim");
+  if (dims) { dims[0] = (uint32_t)s->g.nx; dims[1] = (uint32_t)s->g.ny; dims[2] = (uint32_t)s->g.nz; }
+  if (cells) *cells = s->g.n;
+  if (time) *time = s->time;
+  if (frame) *frame = s->frame;
+  if (levels) *levels = (uint32_t)s->nl;
+  return 0;
+}
+
+int pyro_sim_level_dims(const PyroSim *s, int level, uint32_t dims[3]) {
+  if (!s || !dims) return fail(PYRO_E_INVALID, "null argument");
+  if (level < 0 || level >= s->nl) return fail(PYRO_E_INVALID, "bad level");
+  for (int q = 0; q < 3; q++) dims[q] = (uint32_t)s->lv[level].d[q];
+  return 0;
+}
+
+int pyro_sim_sync(PyroSim *s) {
+  if (!s) return fail(PYRO_E_INVALID, "null sim");
+  return sync_check(s);
+}
+
+int pyro_sim_device_ptr(PyroSim *s, int field, void **dptr, size_t *bytes) {
+  if (!s || !dptr) return fail(PYRO_E_INVALID, "null argument");
+  FieldRef r = field_ref(s, field);
+  if (!r.ptr) return fail(PYRO_E_INVALID, "unknown field id");
+  *dptr = r.ptr;
+  if (bytes) *bytes = r.dev_bytes;
+  return 0;
+}
+
+int pyro_sim_time_stage(PyroSim *s, int stage, int reps, float *avg_ms) {
+  if (!s || !avg_ms || reps < 1) return fail(PYRO_E_INVALID, "bad argument");
+  if (stage < 0 || stage >= PYRO_STAGE_COUNT) return fail(PYRO_E_INVALID, "bad stage id");
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  HIPCHK(hipEventRecord(a, s->stream));
+  for (int r = 0; r < reps; r++) enqueue_stage(s, stage, false, s->stream);
+  HIPCHK(hipEventRecord(b, s->stream));
+  HIPCHK(hipEventSynchronize(b));
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *avg_ms = ms / (float)reps;
+  return sync_check(s);
+}
+
+int pyro_sim_add_velocity_noise(PyroSim *s, float amp, uint32_t seed) {
+  if (!s) return fail(PYRO_E_INVALID, "null sim");
+  Fields f = fields_of(s);
+  launch_velocity_noise(f, amp, seed, s->stream);
+  return sync_check(s);
+}
+
+int pyro_sim_check_finite(PyroSim *s) {
+  if (!s) return fail(PYRO_E_INVALID, "null sim");
+  HIPCHK(hipMemsetAsync(s->d_flag, 0, sizeof(int), s->stream));
+  auto scan = [&](const void *p, size_t nfloat) {
+    k_nonfinite<<<1024, 256, 0, s->stream>>>(static_cast<const float *>(p), nfloat, s->d_flag);
+  };
+  const size_t n4 = (size_t)s->g.n * 4;
+  scan(s->s0, n4);
+  scan(s->s1, n4);
+  scan(s->sol[s->sol_cur], n4);
+  for (int k = 0; k < 3; k++) scan(s->vel[s->cur][k], s->g.nv);
+  scan(s->lv[0].phi, s->g.n);
+  int flag = 0;
+  int rc = sync_check(s);
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(&flag, s->d_flag, sizeof(int), hipMemcpyDeviceToHost));
+  if (flag) return fail(PYRO_E_NONFINITE, "non-finite value in the simulation state");
+  return 0;
+}
+
+int pyro_debug_math(int fn, const float *in, const float *in2, float *out, size_t n) {
+  if (!in || !out || (fn == 2 && !in2) || fn < 0 || fn > 2) return fail(PYRO_E_INVALID, "bad argument");
+  if (n == 0) return 0;
+  float *di = nullptr, *di2 = nullptr, *dout = nullptr;
+  HIPCHK(hipMalloc(&di, n * 4));
+  HIPCHK(hipMalloc(&dout, n * 4));
+  HIPCHK(hipMemcpy(di, in, n * 4, hipMemcpyHostToDevice));
+  if (fn == 2) {
+    HIPCHK(hipMalloc(&di2, n * 4));
+    HIPCHK(hipMemcpy(di2, in2, n * 4, hipMemcpyHostToDevice));
+  }
+  launch_debug_math(fn, di, di2, dout, (uint32_t)n, nullptr);
+  HIPCHK(hipDeviceSynchronize());
+  HIPCHK(hipMemcpy(out, dout, n * 4, hipMemcpyDeviceToHost));
+  (void)hipFree(di);
+  (void)hipFree(dout);
+  if (di2) (void)hipFree(di2);
+  return 0;
+}
+
+}  // extern "C"

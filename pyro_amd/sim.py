@@ -1,0 +1,221 @@
+"""Sim: the per-timestep fire solver on the GPU, with the reference's API.
+
+Python face of ``libpyro_hip.so`` (include/pyro_hip.h).  Method names and
+meaning follow the reference's ``src/sim.rs``: ``Sim.new`` (:108),
+``step`` (:246), ``read_stats`` (:510), ``read_scalars`` (:494), ``cells``
+(:234), ``Stats.flame_height`` (:91-104).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native
+from .scene import Params, Scene
+
+STATS_N = 1040
+
+# Field ids (include/pyro_hip.h PyroField)
+FIELDS = {
+    "s0": 0, "s1": 1, "s0b": 2, "s1b": 3, "h0": 4, "h1": 5, "sol": 6, "sol_out": 7,
+    "pyr": 8, "flags": 9, "curl": 10, "u": 11, "v": 12, "w": 13, "ub": 14, "vb": 15,
+    "wb": 16, "hu": 17, "hv": 18, "hw": 19,
+}
+LEVEL_BASE = 32
+STAGES = [
+    "solid", "curl", "adv_vel1", "adv_vel2", "adv_s1", "adv_s2", "react", "forces",
+    "divergence", "vcycles", "resid", "project",
+]
+CREATE_NO_GRAPH = 1
+
+
+def level_field(level: int, what: str) -> int:
+    return LEVEL_BASE + 4 * level + {"phi": 0, "b": 1, "flags": 2}[what]
+
+
+@dataclass
+class Stats:
+    """Per-step statistics (sim.rs:73-87), FP64."""
+
+    hrr: float = 0.0
+    rad: float = 0.0
+    pyro: float = 0.0
+    soot: float = 0.0
+    max_t: float = 0.0
+    max_u: float = 0.0
+    fuel: float = 0.0
+    solid: float = 0.0
+    smoke: float = 0.0
+    area: float = 0.0
+    r2: float = 0.0
+    b2: float = 0.0
+    resid: float = 0.0
+    layers: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    raw: np.ndarray = field(default_factory=lambda: np.zeros(STATS_N))
+
+    @classmethod
+    def from_raw(cls, v: np.ndarray, ny: int) -> "Stats":
+        resid = float(np.sqrt(v[10] / v[11])) if v[11] > 0 else 0.0  # sim.rs:523-527
+        return cls(hrr=v[0], rad=v[1], pyro=v[2], soot=v[3], max_t=v[4], max_u=v[5], fuel=v[6],
+                   solid=v[7], smoke=v[8], area=v[9], r2=v[10], b2=v[11], resid=resid,
+                   layers=v[16:16 + min(ny, 1024)].copy(), raw=v.copy())
+
+    def flame_height(self, dx: float, frac: float = 0.95) -> float:
+        """Height where cumulative per-layer HRR reaches frac of the total (sim.rs:91-104)."""
+        tot = float(np.sum(self.layers))
+        if tot <= 0.0:
+            return 0.0
+        acc = 0.0
+        for j, v in enumerate(self.layers):
+            acc += v
+            if acc >= frac * tot:
+                return (j + 1.0) * dx
+        return len(self.layers) * dx
+
+
+class Sim:
+    """GPU simulation state (sim.rs:42-68) behind the C ABI."""
+
+    def __init__(self, params: Params, sol_init: np.ndarray, s0_init=None, pressure_cycles: int = 4,
+                 ign_until: float = 0.0, use_graph: bool = True):
+        lib = _native.hip()
+        self.p = params
+        self.dims = (params["nx"], params["ny"], params["nz"])
+        self.dx = params["dx"]
+        sol = np.ascontiguousarray(sol_init, dtype=np.float32)
+        n = self.dims[0] * self.dims[1] * self.dims[2]
+        if sol.size != 4 * n:
+            raise ValueError("sol_init must hold N x 4 floats")
+        s0p = None
+        if s0_init is not None:
+            s0 = np.ascontiguousarray(s0_init, dtype=np.float32)
+            if s0.size != 4 * n:
+                raise ValueError("s0_init must hold N x 4 floats")
+            s0p = s0.ctypes.data
+        h = C.c_void_p()
+        blob = (C.c_uint8 * 576).from_buffer_copy(params.bytes())
+        _native.check_hip(lib.pyro_sim_create(blob, 576, sol.ctypes.data, s0p, int(pressure_cycles),
+                                              float(ign_until), 0 if use_graph else CREATE_NO_GRAPH,
+                                              C.byref(h)))
+        self._h = h
+        self.cycles = int(pressure_cycles)
+        self.nv = (self.dims[0] + 1) * (self.dims[1] + 1) * (self.dims[2] + 1)
+
+    @classmethod
+    def new(cls, scene: Scene, use_graph: bool = True) -> "Sim":
+        """Sim::new(gpu, scene) (sim.rs:108-232)."""
+        p = scene.params()
+        sol = scene.voxelize()
+        return cls(p, sol, None, int(scene.get("sim.pressure_cycles")), scene.get("ignition.until"),
+                   use_graph=use_graph)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _native.hip().pyro_sim_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- step API -------------------------------------------------------
+    def step(self, dt: float, want_stats: bool = False) -> None:
+        _native.check_hip(_native.hip().pyro_sim_step(self._h, float(dt), int(bool(want_stats))))
+
+    def read_stats(self) -> Stats:
+        out = np.zeros(STATS_N, dtype=np.float64)
+        _native.check_hip(_native.hip().pyro_sim_read_stats(self._h, out.ctypes.data, STATS_N))
+        return Stats.from_raw(out, self.dims[1])
+
+    def read_scalars(self):
+        """(s0, s1) as (nz, ny, nx, 4) float32 arrays (sim.rs:494-508)."""
+        return self.read_field("s0"), self.read_field("s1")
+
+    def cells(self) -> int:
+        return self.dims[0] * self.dims[1] * self.dims[2]
+
+    @property
+    def time(self) -> float:
+        t = C.c_double()
+        _native.check_hip(_native.hip().pyro_sim_info(self._h, None, None, C.byref(t), None, None))
+        return t.value
+
+    @property
+    def levels(self) -> int:
+        n = C.c_uint32()
+        _native.check_hip(_native.hip().pyro_sim_info(self._h, None, None, None, None, C.byref(n)))
+        return n.value
+
+    def level_dims(self, level: int):
+        d = (C.c_uint32 * 3)()
+        _native.check_hip(_native.hip().pyro_sim_level_dims(self._h, level, d))
+        return (d[0], d[1], d[2])
+
+    # -- fields ---------------------------------------------------------
+    def _shape(self, fid: int):
+        nx, ny, nz = self.dims
+        if fid in (0, 1, 2, 3, 4, 5, 6, 7, 8, 10):
+            return (nz, ny, nx, 4), np.float32
+        if fid == 9:
+            return (nz, ny, nx), np.uint32
+        if 11 <= fid <= 19:
+            return (nz + 1, ny + 1, nx + 1), np.float32
+        l, q = divmod(fid - LEVEL_BASE, 4)
+        d = self.level_dims(l)
+        return (d[2], d[1], d[0]), (np.uint32 if q == 2 else np.float32)
+
+    def read_field(self, name) -> np.ndarray:
+        fid = FIELDS[name] if isinstance(name, str) else int(name)
+        shape, dt = self._shape(fid)
+        out = np.zeros(shape, dtype=dt)
+        _native.check_hip(_native.hip().pyro_sim_read_field(self._h, fid, out.ctypes.data, out.nbytes))
+        return out
+
+    def write_field(self, name, arr: np.ndarray) -> None:
+        fid = FIELDS[name] if isinstance(name, str) else int(name)
+        shape, dt = self._shape(fid)
+        a = np.ascontiguousarray(arr, dtype=dt)
+        if a.size != int(np.prod(shape)):
+            raise ValueError(f"field {name}: expected shape {shape}")
+        _native.check_hip(_native.hip().pyro_sim_write_field(self._h, fid, a.ctypes.data, a.nbytes))
+
+    # -- hooks ------------------------------------------------------------
+    def begin_step(self, dt: float, want_stats: bool = False) -> None:
+        _native.check_hip(_native.hip().pyro_sim_begin_step(self._h, float(dt), int(bool(want_stats))))
+
+    def run_stage(self, stage) -> None:
+        sid = STAGES.index(stage) if isinstance(stage, str) else int(stage)
+        _native.check_hip(_native.hip().pyro_sim_run_stage(self._h, sid))
+
+    def end_step(self) -> None:
+        _native.check_hip(_native.hip().pyro_sim_end_step(self._h))
+
+    def time_stage(self, stage, reps: int) -> float:
+        sid = STAGES.index(stage) if isinstance(stage, str) else int(stage)
+        ms = C.c_float()
+        _native.check_hip(_native.hip().pyro_sim_time_stage(self._h, sid, int(reps), C.byref(ms)))
+        return ms.value
+
+    def add_velocity_noise(self, amp: float, seed: int) -> None:
+        _native.check_hip(_native.hip().pyro_sim_add_velocity_noise(self._h, float(amp), int(seed)))
+
+    def check_finite(self) -> None:
+        _native.check_hip(_native.hip().pyro_sim_check_finite(self._h))
+
+    def sync(self) -> None:
+        _native.check_hip(_native.hip().pyro_sim_sync(self._h))
+
+
+def debug_math(fn: int, x: np.ndarray, y: np.ndarray | None = None) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.zeros_like(x)
+    yp = None
+    if y is not None:
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        yp = y.ctypes.data
+    _native.check_hip(_native.hip().pyro_debug_math(fn, x.ctypes.data, yp, out.ctypes.data, x.size))
+    return out
