@@ -107,6 +107,14 @@ int pyro_sim_end_step(PyroSim *sim);
 /* Time `reps` launches of one stage with HIP events on the sim stream
  * (average ms per launch; state is advanced reps times). */
 int pyro_sim_time_stage(PyroSim *sim, int stage, int reps, float *avg_ms);
+/* Kernel ids for pyro_sim_time_kernel. */
+#define PYRO_KERNEL_MG_SMOOTH 0   /* one red-black sweep (both colours) on `level` */
+#define PYRO_KERNEL_MG_RESTRICT 1 /* residual + restriction level -> level+1 */
+#define PYRO_KERNEL_MG_PROLONG 2  /* prolongation level+1 -> level */
+#define PYRO_KERNEL_MG_COARSE 3   /* all 80 SOR iterations of the coarsest level */
+/* Time `reps` launches of one multigrid kernel (HIP events on the sim
+ * stream); the pressure state is modified. */
+int pyro_sim_time_kernel(PyroSim *sim, int kernel, int level, int reps, float *avg_ms);
 /* C1 harness: vel[cur] += amp * U[-1,1] (PCG hash of component, face, seed). */
 int pyro_sim_add_velocity_noise(PyroSim *sim, float amp, uint32_t seed);
 /* Scan the state for NaN/Inf; returns PYRO_E_NONFINITE if any. */

@@ -54,30 +54,6 @@ static LvArgs lv_args(const MgLevel &L) {
   return a;
 }
 
-// One colour of the red-black sweep: cells with (x+y+z)&1 == color.
-__global__ __launch_bounds__(NT) void k_mg_smooth(LvArgs a, const float *__restrict__ b,
-                                                   const uint8_t *__restrict__ f, float *__restrict__ p, int color,
-                                                   float omega) {
-  uint32_t t = blockIdx.x * NT + threadIdx.x;
-  if (t >= a.nhalf) return;
-  uint32_t r = fdiv(t, a.dhx);
-  int gx = (int)(t - r * a.dhx.d);
-  uint32_t zq = fdiv(r, a.dny);
-  int y = (int)(r - zq * (uint32_t)a.d1), z = (int)zq;
-  int x = 2 * gx + ((y + z + color) & 1);
-  if (x >= a.d0) return;
-  uint32_t i = (uint32_t)x + (uint32_t)a.d0 * ((uint32_t)y + (uint32_t)a.d1 * (uint32_t)z);
-  if (f[i] == 1) return;
-  float sum, cnt;
-  nsum(a.d0, a.d1, a.d2, p, f, x, y, z, i, sum, cnt);
-  if (cnt < 0.5f) return;
-  p[i] = mixf_(p[i], (sum - a.hh * b[i]) / cnt, omega);
-}
-void launch_mg_smooth(const MgLevel &L, int color, float omega, hipStream_t s) {
-  LvArgs a = lv_args(L);
-  k_mg_smooth<<<nblocks(a.nhalf), NT, 0, s>>>(a, L.b, L.f, L.phi, color, omega);
-}
-
 // Fine residual at (x,y,z) (mg.wgsl:52-66).
 __device__ __forceinline__ float resid_at(const LvArgs &F, const float *__restrict__ p, const float *__restrict__ b,
                                           const uint8_t *__restrict__ f, int x, int y, int z) {
@@ -89,13 +65,12 @@ __device__ __forceinline__ float resid_at(const LvArgs &F, const float *__restri
   return b[i] - (sum - cnt * pcen) / F.hh;
 }
 
-// Residual + average over fluid children; coarse phi = 0; coarse cell solid
-// iff every child is solid (mg.wgsl:67-83).
+// Residual + average over fluid children, coarse phi = 0 (mg.wgsl:67-81).
+// The coarse flags (mg.wgsl:82) are produced once per step by the setup pass.
 __global__ __launch_bounds__(NT) void k_mg_restrict(LvArgs F, const float *__restrict__ pf,
                                                      const float *__restrict__ bf, const uint8_t *__restrict__ ff,
                                                      int c0, int c1, int c2, FastDiv dc0, FastDiv dc1,
-                                                     float *__restrict__ bc, float *__restrict__ pc,
-                                                     uint8_t *__restrict__ fc) {
+                                                     float *__restrict__ bc, float *__restrict__ pc) {
   uint32_t i = blockIdx.x * NT + threadIdx.x;
   if (i >= (uint32_t)c0 * (uint32_t)c1 * (uint32_t)c2) return;
   uint32_t r = fdiv(i, dc0);
@@ -114,13 +89,12 @@ __global__ __launch_bounds__(NT) void k_mg_restrict(LvArgs F, const float *__res
   }
   bc[i] = nfl > 0.0f ? acc / fmax_(nfl, 1.0f) : 0.0f;
   pc[i] = 0.0f;
-  fc[i] = nfl > 0.0f ? 0 : 1;
 }
 void launch_mg_restrict(const MgLevel &Fl, const MgLevel &C, hipStream_t s) {
   LvArgs F = lv_args(Fl);
   k_mg_restrict<<<nblocks(C.n), NT, 0, s>>>(F, Fl.phi, Fl.b, Fl.f, C.d[0], C.d[1], C.d[2],
                                             make_fastdiv((uint32_t)C.d[0]), make_fastdiv((uint32_t)C.d[1]), C.b,
-                                            C.phi, C.f);
+                                            C.phi);
 }
 
 // Masked, renormalised trilinear correction (mg.wgsl:97-120).
@@ -159,52 +133,295 @@ void launch_mg_prolong(const MgLevel &C, const MgLevel &F, hipStream_t s) {
                                            C.phi, C.f);
 }
 
-// ---------------------------------------------------------------- coarse
-constexpr int COARSE_NT = 1024;
-constexpr int COARSE_MAX = 16384;  // cells; phi + b (f32) + mask (u8) = 144 KB of LDS
+// ---------------------------------------------------------------- setup
+// Per-level neighbour codes, computed once per step (flags are fixed during
+// a step and restriction derives the same coarse flags in every cycle):
+//   bit 7: solid (never updated, phi is Neumann for its neighbours)
+//   bit 6: fluid cell without a counted neighbour (mg.wgsl:34 skips it)
+//   bits 0-2: cnt of mg.wgsl:25-33 (fluid neighbours + sides/top outside)
+constexpr uint8_t NC_SOLID = 0x80, NC_NONE = 0x40;
 
-__global__ __launch_bounds__(COARSE_NT) void k_mg_coarse_lds(LvArgs a, const float *__restrict__ b,
-                                                              const uint8_t *__restrict__ f, float *__restrict__ p,
-                                                              int iters, float omega) {
-  __shared__ float sp[COARSE_MAX];
-  __shared__ float sb[COARSE_MAX];
-  __shared__ uint8_t sf[COARSE_MAX];
-  for (uint32_t i = threadIdx.x; i < a.n; i += COARSE_NT) {
-    sp[i] = p[i];
-    sb[i] = b[i];
-    sf[i] = f[i];
+__global__ __launch_bounds__(NT) void k_mg_counts(LvArgs a, FastDiv d0, const uint8_t *__restrict__ f,
+                                                   uint8_t *__restrict__ nc) {
+  uint32_t i = blockIdx.x * NT + threadIdx.x;
+  if (i >= a.n) return;
+  if (f[i] == 1) { nc[i] = NC_SOLID; return; }
+  uint32_t r = fdiv(i, d0);
+  int x = (int)(i - r * (uint32_t)a.d0);
+  uint32_t zq = fdiv(r, a.dny);
+  int y = (int)(r - zq * (uint32_t)a.d1), z = (int)zq;
+  const uint32_t sy = (uint32_t)a.d0, sz = (uint32_t)a.d0 * (uint32_t)a.d1;
+  int cnt = 0;
+  cnt += (x + 1 < a.d0) ? (f[i + 1] != 1) : 1;
+  cnt += (x - 1 >= 0) ? (f[i - 1] != 1) : 1;
+  cnt += (y + 1 < a.d1) ? (f[i + sy] != 1) : 1;
+  cnt += (y - 1 >= 0) ? (f[i - sy] != 1) : 0;  // ground: not counted
+  cnt += (z + 1 < a.d2) ? (f[i + sz] != 1) : 1;
+  cnt += (z - 1 >= 0) ? (f[i - sz] != 1) : 1;
+  nc[i] = cnt == 0 ? NC_NONE : (uint8_t)cnt;
+}
+
+// Coarse cell is solid iff every child inside the fine box is solid (mg.wgsl:74-82).
+__global__ __launch_bounds__(NT) void k_mg_coarse_flags(LvArgs F, const uint8_t *__restrict__ ff, int c0, int c1,
+                                                         int c2, FastDiv dc0, FastDiv dc1, uint8_t *__restrict__ fc) {
+  uint32_t i = blockIdx.x * NT + threadIdx.x;
+  if (i >= (uint32_t)c0 * (uint32_t)c1 * (uint32_t)c2) return;
+  uint32_t r = fdiv(i, dc0);
+  int x = (int)(i - r * (uint32_t)c0);
+  uint32_t zq = fdiv(r, dc1);
+  int y = (int)(r - zq * (uint32_t)c1), z = (int)zq;
+  int nfl = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    int a = 2 * x + (k & 1), bb = 2 * y + ((k >> 1) & 1), c = 2 * z + ((k >> 2) & 1);
+    if (a >= F.d0 || bb >= F.d1 || c >= F.d2) continue;
+    nfl += ff[(uint32_t)a + (uint32_t)F.d0 * ((uint32_t)bb + (uint32_t)F.d1 * (uint32_t)c)] != 1;
+  }
+  fc[i] = nfl > 0 ? 0 : 1;
+}
+
+void launch_mg_setup(const MgLevel *L, int nl, hipStream_t s) {
+  for (int l = 0; l < nl; l++) {
+    if (l > 0) {
+      const MgLevel &F = L[l - 1], &C = L[l];
+      k_mg_coarse_flags<<<nblocks(C.n), NT, 0, s>>>(lv_args(F), F.f, C.d[0], C.d[1], C.d[2],
+                                                   make_fastdiv((uint32_t)C.d[0]), make_fastdiv((uint32_t)C.d[1]), C.f);
+    }
+    k_mg_counts<<<nblocks(L[l].n), NT, 0, s>>>(lv_args(L[l]), make_fastdiv((uint32_t)L[l].d[0]), L[l].f, L[l].nc);
+  }
+}
+
+// ---------------------------------------------------------------- sweep
+// One full red-black sweep (colour 0 then colour 1, sim.rs:444-448) of a
+// 32x8x4 tile per workgroup, staged in LDS: phi with a halo of 2, codes and
+// h^2 b with a halo of 1.  Colour-0 cells are updated on the tile plus a ring
+// of 1 (the values the tile's colour-1 cells read), then colour-1 cells on
+// the tile.  phi is read from `pin` and written to `pout` (ping-pong), so a
+// tile never sees a neighbour's already-swept values.
+// Solid and outside cells hold 0 in the LDS copy of phi, so the neighbour
+// sum needs no mask: a skipped neighbour adds +0.0, which cannot change the
+// running sum (it starts at +0.0 and never becomes -0.0).  The per-cell
+// arithmetic is that of mg.wgsl:25-35, so the result equals two colour passes.
+constexpr int SW_TX = 32, SW_TY = 8, SW_TZ = 4, SW_NT = 256;
+constexpr int SW_PX = SW_TX + 4, SW_PY = SW_TY + 4, SW_PZ = SW_TZ + 4;
+constexpr int SW_BX = SW_TX + 2, SW_BY = SW_TY + 2, SW_BZ = SW_TZ + 2;
+constexpr int SW_PN = SW_PX * SW_PY * SW_PZ, SW_BN = SW_BX * SW_BY * SW_BZ;
+
+__device__ __forceinline__ float sweep_update(const float *sp, int pi, uint8_t code, float hb, float omega) {
+  float sum = 0.0f;
+  sum += sp[pi + 1];
+  sum += sp[pi - 1];
+  sum += sp[pi + SW_PX];
+  sum += sp[pi - SW_PX];
+  sum += sp[pi + SW_PX * SW_PY];
+  sum += sp[pi - SW_PX * SW_PY];
+  const float cnt = (float)(code & 7u);
+  return mixf_(sp[pi], (sum - hb) / cnt, omega);
+}
+
+template <bool FINE>
+__global__ __launch_bounds__(SW_NT) void k_mg_sweep(LvArgs a, int ntx, int nty, const float *__restrict__ b,
+                                                     const uint8_t *__restrict__ f, const uint8_t *__restrict__ nc,
+                                                     const float *__restrict__ pin, float *__restrict__ pout,
+                                                     float omega) {
+  __shared__ float sp[SW_PN];
+  __shared__ float sb[SW_BN];
+  __shared__ uint8_t sc[SW_BN];
+  const int bid = blockIdx.x;
+  const int tx = bid % ntx, rr = bid / ntx, ty = rr % nty, tz = rr / nty;
+  const int x0 = tx * SW_TX - 2, y0 = ty * SW_TY - 2, z0 = tz * SW_TZ - 2;  // x0 is even
+  const uint32_t sy = (uint32_t)a.d0, sz = (uint32_t)a.d0 * (uint32_t)a.d1;
+  const bool inner = x0 >= 0 && y0 >= 0 && z0 >= 0 && x0 + SW_PX <= a.d0 && y0 + SW_PY <= a.d1 && z0 + SW_PZ <= a.d2;
+  const uint32_t gbase = (uint32_t)x0 + sy * (uint32_t)y0 + sz * (uint32_t)z0;  // wraps for edge tiles; unused there
+  if (inner) {
+    for (int i = threadIdx.x; i < SW_PN; i += SW_NT) {
+      int lx = i % SW_PX, t = i / SW_PX, ly = t % SW_PY, lz = t / SW_PY;
+      uint32_t g = gbase + (uint32_t)lx + sy * (uint32_t)ly + sz * (uint32_t)lz;
+      float pv = pin[g];
+      sp[i] = f[g] == 1 ? 0.0f : pv;
+    }
+    for (int i = threadIdx.x; i < SW_BN; i += SW_NT) {
+      int lx = i % SW_BX, t = i / SW_BX, ly = t % SW_BY, lz = t / SW_BY;
+      uint32_t g = gbase + (uint32_t)(lx + 1) + sy * (uint32_t)(ly + 1) + sz * (uint32_t)(lz + 1);
+      sb[i] = a.hh * b[g];
+      sc[i] = nc[g];
+    }
+  } else {
+    for (int i = threadIdx.x; i < SW_PN; i += SW_NT) {
+      int lx = i % SW_PX, t = i / SW_PX, ly = t % SW_PY, lz = t / SW_PY;
+      int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+      float pv = 0.0f;
+      if (gx >= 0 && gy >= 0 && gz >= 0 && gx < a.d0 && gy < a.d1 && gz < a.d2) {
+        uint32_t g = (uint32_t)gx + sy * (uint32_t)gy + sz * (uint32_t)gz;
+        pv = f[g] == 1 ? 0.0f : pin[g];
+      }
+      sp[i] = pv;
+    }
+    for (int i = threadIdx.x; i < SW_BN; i += SW_NT) {
+      int lx = i % SW_BX, t = i / SW_BX, ly = t % SW_BY, lz = t / SW_BY;
+      int gx = x0 + 1 + lx, gy = y0 + 1 + ly, gz = z0 + 1 + lz;
+      float bv = 0.0f;
+      uint8_t code = NC_NONE;
+      if (gx >= 0 && gy >= 0 && gz >= 0 && gx < a.d0 && gy < a.d1 && gz < a.d2) {
+        uint32_t g = (uint32_t)gx + sy * (uint32_t)gy + sz * (uint32_t)gz;
+        bv = a.hh * b[g];
+        code = nc[g];
+      }
+      sb[i] = bv;
+      sc[i] = code;
+    }
   }
   __syncthreads();
+  // colour 0 on tile + ring 1: one thread per x-pair, picks the cell of colour 0
+  constexpr int RP = SW_BX / 2, RN = RP * SW_BY * SW_BZ;
+  for (int i = threadIdx.x; i < RN; i += SW_NT) {
+    int px = i % RP, t = i / RP, ly = t % SW_BY, lz = t / SW_BY;
+    int lxb = 2 * px + ((1 + 2 * px + (y0 + ly + 1) + (z0 + lz + 1)) & 1);  // (gx+gy+gz) even
+    int bi = (lz * SW_BY + ly) * SW_BX + lxb;
+    uint8_t code = sc[bi];
+    if (code & (NC_SOLID | NC_NONE)) continue;
+    int pi = ((lz + 1) * SW_PY + (ly + 1)) * SW_PX + (lxb + 1);
+    sp[pi] = sweep_update(sp, pi, code, sb[bi], omega);
+  }
+  __syncthreads();
+  // colour 1 on the tile
+  constexpr int BP = SW_TX / 2, BN = BP * SW_TY * SW_TZ;
+  for (int i = threadIdx.x; i < BN; i += SW_NT) {
+    int px = i % BP, t = i / BP, ly = t % SW_TY, lz = t / SW_TY;
+    int lxb = 1 + 2 * px + ((2 + 2 * px + (y0 + ly + 2) + (z0 + lz + 2) + 1) & 1);  // (gx+gy+gz) odd
+    int bi = ((lz + 1) * SW_BY + (ly + 1)) * SW_BX + lxb;
+    uint8_t code = sc[bi];
+    if (code & (NC_SOLID | NC_NONE)) continue;
+    int pi = ((lz + 2) * SW_PY + (ly + 2)) * SW_PX + (lxb + 1);
+    sp[pi] = sweep_update(sp, pi, code, sb[bi], omega);
+  }
+  __syncthreads();
+  constexpr int TN = SW_TX * SW_TY * SW_TZ;
+  for (int i = threadIdx.x; i < TN; i += SW_NT) {
+    int lx = i % SW_TX, t = i / SW_TX, ly = t % SW_TY, lz = t / SW_TY;
+    int gx = x0 + 2 + lx, gy = y0 + 2 + ly, gz = z0 + 2 + lz;
+    if (gx < a.d0 && gy < a.d1 && gz < a.d2) {
+      uint32_t g = (uint32_t)gx + sy * (uint32_t)gy + sz * (uint32_t)gz;
+      uint8_t code = sc[((lz + 1) * SW_BY + (ly + 1)) * SW_BX + lx + 1];
+      pout[g] = (code & NC_SOLID) ? pin[g] : sp[((lz + 2) * SW_PY + (ly + 2)) * SW_PX + lx + 2];
+    }
+  }
+}
+
+void launch_mg_sweep(const MgLevel &L, const float *pin, float *pout, float omega, hipStream_t s, bool fine) {
+  LvArgs a = lv_args(L);
+  int ntx = (L.d[0] + SW_TX - 1) / SW_TX, nty = (L.d[1] + SW_TY - 1) / SW_TY, ntz = (L.d[2] + SW_TZ - 1) / SW_TZ;
+  int nb = ntx * nty * ntz;
+  if (fine)
+    k_mg_sweep<true><<<nb, SW_NT, 0, s>>>(a, ntx, nty, L.b, L.f, L.nc, pin, pout, omega);
+  else
+    k_mg_sweep<false><<<nb, SW_NT, 0, s>>>(a, ntx, nty, L.b, L.f, L.nc, pin, pout, omega);
+}
+
+// ---------------------------------------------------------------- coarse
+// All red-black SOR iterations of the coarsest level in one workgroup
+// (sim.rs:472-473: 80 iterations, omega = 1.6).  phi lives in a zero-padded
+// LDS grid (the ghost ring is the Dirichlet phi = 0 of mg.wgsl:29, solids are
+// zeroed as in the sweep); each thread owns up to CO_K cells per colour and
+// keeps their LDS index, count and h^2 b in registers.  Within one colour
+// pass no cell reads a cell of its own colour, so the loads of a chunk of
+// CO_C cells are issued together before any store.
+constexpr int CO_NT = 512, CO_K = 16, CO_C = 4;
+constexpr int CO_PAD_MAX = 36000;  // padded cells: 144 KB of LDS (< 65536: 16-bit index)
+
+__global__ __launch_bounds__(CO_NT) void k_mg_coarse(LvArgs a, const float *__restrict__ b,
+                                                      const uint8_t *__restrict__ f, const uint8_t *__restrict__ nc,
+                                                      float *__restrict__ p, int iters, float omega) {
+  __shared__ float sp[CO_PAD_MAX];
+  const int P0 = a.d0 + 2, P1 = a.d1 + 2, P2 = a.d2 + 2;
+  const int PN = P0 * P1 * P2;
+  const uint32_t sy = (uint32_t)a.d0, sz = (uint32_t)a.d0 * (uint32_t)a.d1;
+  for (int i = threadIdx.x; i < PN; i += CO_NT) {
+    int lx = i % P0, t = i / P0, ly = t % P1, lz = t / P1;
+    int gx = lx - 1, gy = ly - 1, gz = lz - 1;
+    float v = 0.0f;
+    if (gx >= 0 && gy >= 0 && gz >= 0 && gx < a.d0 && gy < a.d1 && gz < a.d2) {
+      uint32_t g = (uint32_t)gx + sy * (uint32_t)gy + sz * (uint32_t)gz;
+      v = f[g] == 1 ? 0.0f : p[g];
+    }
+    sp[i] = v;
+  }
+  // per owned cell: LDS index (bits 0-15) | cnt (16-18) | active (31)
+  uint32_t pk[2][CO_K];
+  float hb[2][CO_K];
+#pragma unroll
+  for (int c = 0; c < 2; c++) {
+#pragma unroll
+    for (int j = 0; j < CO_K; j++) {
+      pk[c][j] = 0;
+      hb[c][j] = 0.0f;
+      uint32_t t = threadIdx.x + (uint32_t)j * CO_NT;
+      if (t >= a.nhalf) continue;
+      uint32_t r = fdiv(t, a.dhx);
+      int gxh = (int)(t - r * a.dhx.d);
+      uint32_t zq = fdiv(r, a.dny);
+      int y = (int)(r - zq * (uint32_t)a.d1), z = (int)zq;
+      int x = 2 * gxh + ((y + z + c) & 1);
+      if (x >= a.d0) continue;
+      uint32_t g = (uint32_t)x + sy * (uint32_t)y + sz * (uint32_t)z;
+      uint8_t code = nc[g];
+      if (code & (NC_SOLID | NC_NONE)) continue;
+      uint32_t li = (uint32_t)(((z + 1) * P1 + (y + 1)) * P0 + (x + 1));
+      pk[c][j] = li | ((uint32_t)(code & 7u) << 16) | 0x80000000u;
+      hb[c][j] = a.hh * b[g];
+    }
+  }
+  __syncthreads();
+  const int o1 = 1, o2 = P0, o3 = P0 * P1;
   for (int it = 0; it < iters; it++) {
-    for (int color = 0; color < 2; color++) {
-      for (uint32_t t = threadIdx.x; t < a.nhalf; t += COARSE_NT) {
-        uint32_t r = fdiv(t, a.dhx);
-        int gx = (int)(t - r * a.dhx.d);
-        uint32_t zq = fdiv(r, a.dny);
-        int y = (int)(r - zq * (uint32_t)a.d1), z = (int)zq;
-        int x = 2 * gx + ((y + z + color) & 1);
-        if (x >= a.d0) continue;
-        uint32_t i = (uint32_t)x + (uint32_t)a.d0 * ((uint32_t)y + (uint32_t)a.d1 * (uint32_t)z);
-        if (sf[i] == 1) continue;
-        float sum, cnt;
-        nsum(a.d0, a.d1, a.d2, sp, sf, x, y, z, i, sum, cnt);
-        if (cnt < 0.5f) continue;
-        sp[i] = mixf_(sp[i], (sum - a.hh * sb[i]) / cnt, omega);
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+#pragma unroll
+      for (int j0 = 0; j0 < CO_K; j0 += CO_C) {
+        float nb[CO_C][7];
+#pragma unroll
+        for (int q = 0; q < CO_C; q++) {
+          const int i = (int)(pk[c][j0 + q] & 0xffffu);  // inactive cells read ghost cell 0
+          nb[q][0] = sp[i + o1]; nb[q][1] = sp[i - o1]; nb[q][2] = sp[i + o2];
+          nb[q][3] = sp[i - o2]; nb[q][4] = sp[i + o3]; nb[q][5] = sp[i - o3];
+          nb[q][6] = sp[i];
+        }
+#pragma unroll
+        for (int q = 0; q < CO_C; q++) {
+          const uint32_t w = pk[c][j0 + q];
+          float sum = 0.0f;
+          sum += nb[q][0];
+          sum += nb[q][1];
+          sum += nb[q][2];
+          sum += nb[q][3];
+          sum += nb[q][4];
+          sum += nb[q][5];
+          const float cnt = (float)((w >> 16) & 7u);
+          const float nv = mixf_(nb[q][6], (sum - hb[c][j0 + q]) / cnt, omega);
+          if (w & 0x80000000u) sp[(int)(w & 0xffffu)] = nv;
+        }
       }
       __syncthreads();
     }
   }
-  for (uint32_t i = threadIdx.x; i < a.n; i += COARSE_NT) p[i] = sp[i];
+  for (int i = threadIdx.x; i < PN; i += CO_NT) {
+    int lx = i % P0, t = i / P0, ly = t % P1, lz = t / P1;
+    int gx = lx - 1, gy = ly - 1, gz = lz - 1;
+    if (gx >= 0 && gy >= 0 && gz >= 0 && gx < a.d0 && gy < a.d1 && gz < a.d2) {
+      uint32_t g = (uint32_t)gx + sy * (uint32_t)gy + sz * (uint32_t)gz;
+      if (f[g] != 1) p[g] = sp[i];
+    }
+  }
 }
 
 // Same schedule for a coarsest level larger than LDS: one workgroup, phi in
 // global memory (L2-resident), a workgroup barrier between colour passes.
-__global__ __launch_bounds__(COARSE_NT) void k_mg_coarse_global(LvArgs a, const float *__restrict__ b,
-                                                                 const uint8_t *__restrict__ f, float *p, int iters,
-                                                                 float omega) {
+__global__ __launch_bounds__(1024) void k_mg_coarse_global(LvArgs a, const float *__restrict__ b,
+                                                            const uint8_t *__restrict__ f, float *p, int iters,
+                                                            float omega) {
   for (int it = 0; it < iters; it++) {
     for (int color = 0; color < 2; color++) {
-      for (uint32_t t = threadIdx.x; t < a.nhalf; t += COARSE_NT) {
+      for (uint32_t t = threadIdx.x; t < a.nhalf; t += 1024) {
         uint32_t r = fdiv(t, a.dhx);
         int gx = (int)(t - r * a.dhx.d);
         uint32_t zq = fdiv(r, a.dny);
@@ -223,14 +440,18 @@ __global__ __launch_bounds__(COARSE_NT) void k_mg_coarse_global(LvArgs a, const 
   }
 }
 
-bool mg_coarse_fits_lds(const MgLevel &L) { return L.n <= (uint32_t)COARSE_MAX; }
+bool mg_coarse_fits_lds(const MgLevel &L) {
+  uint32_t pad = (uint32_t)(L.d[0] + 2) * (uint32_t)(L.d[1] + 2) * (uint32_t)(L.d[2] + 2);
+  uint32_t half = (uint32_t)((L.d[0] + 1) / 2) * (uint32_t)L.d[1] * (uint32_t)L.d[2];
+  return pad <= (uint32_t)CO_PAD_MAX && half <= (uint32_t)(CO_NT * CO_K);
+}
 
 void launch_mg_coarse(const MgLevel &L, int iters, float omega, hipStream_t s) {
   LvArgs a = lv_args(L);
   if (mg_coarse_fits_lds(L))
-    k_mg_coarse_lds<<<1, COARSE_NT, 0, s>>>(a, L.b, L.f, L.phi, iters, omega);
+    k_mg_coarse<<<1, CO_NT, 0, s>>>(a, L.b, L.f, L.nc, L.phi, iters, omega);
   else
-    k_mg_coarse_global<<<1, COARSE_NT, 0, s>>>(a, L.b, L.f, L.phi, iters, omega);
+    k_mg_coarse_global<<<1, 1024, 0, s>>>(a, L.b, L.f, L.phi, iters, omega);
 }
 
 // ---------------------------------------------------------------- resid

@@ -48,10 +48,15 @@ struct MgLevel {
   int d[3];
   float h, hh;   // spacing and h*h (f32)
   float *phi, *b;
-  uint8_t *f;
+  float *tmp;    // second phi buffer of the ping-pong sweeps
+  uint8_t *f;    // 1 = solid
+  uint8_t *nc;   // neighbour codes (k_mg.hip setup pass)
   uint32_t n;
 };
-void launch_mg_smooth(const MgLevel &L, int color, float omega, hipStream_t s);
+// Coarse flags and neighbour codes of every level (once per step).
+void launch_mg_setup(const MgLevel *L, int nl, hipStream_t s);
+// One red-black sweep (colour 0 then 1) reading phi from pin, writing pout.
+void launch_mg_sweep(const MgLevel &L, const float *pin, float *pout, float omega, hipStream_t s, bool fine);
 void launch_mg_restrict(const MgLevel &F, const MgLevel &C, hipStream_t s);
 void launch_mg_prolong(const MgLevel &C, const MgLevel &F, hipStream_t s);
 // All `iters` red-black sweeps of the coarsest level in one launch.

@@ -90,25 +90,28 @@ Fields fields_of(PyroSim *s) {
   return f;
 }
 
-void enqueue_smooth(const MgLevel &L, int iters, float omega, hipStream_t st) {
+// `iters` red-black sweeps (sim.rs:425-449); an even count leaves phi in
+// L.phi, the odd sweeps land in L.tmp.
+void enqueue_smooth(const MgLevel &L, int iters, float omega, hipStream_t st, bool fine) {
   for (int it = 0; it < iters; it++) {
-    launch_mg_smooth(L, 0, omega, st);
-    launch_mg_smooth(L, 1, omega, st);
+    if (it & 1) launch_mg_sweep(L, L.tmp, L.phi, omega, st, fine);
+    else launch_mg_sweep(L, L.phi, L.tmp, omega, st, fine);
   }
 }
 
 // sim.rs:451-491
 void enqueue_vcycles(PyroSim *s, hipStream_t st) {
   const int nl = s->nl;
+  launch_mg_setup(s->lv, nl, st);
   for (uint32_t c = 0; c < s->cycles; c++) {
     for (int l = 0; l < nl - 1; l++) {
-      enqueue_smooth(s->lv[l], 2, 1.0f, st);
+      enqueue_smooth(s->lv[l], 2, 1.0f, st, l == 0);
       launch_mg_restrict(s->lv[l], s->lv[l + 1], st);
     }
     launch_mg_coarse(s->lv[nl - 1], COARSE_ITERS, 1.6f, st);
     for (int l = nl - 2; l >= 0; l--) {
       launch_mg_prolong(s->lv[l + 1], s->lv[l], st);
-      enqueue_smooth(s->lv[l], 2, 1.0f, st);
+      enqueue_smooth(s->lv[l], 2, 1.0f, st, l == 0);
     }
   }
 }
@@ -278,6 +281,8 @@ int pyro_sim_create(const void *params, size_t params_bytes, const float *sol_in
     L.n = (uint32_t)d[0] * (uint32_t)d[1] * (uint32_t)d[2];
     rc |= dalloc_t(s, &L.phi, L.n);
     if (!rc) rc |= dalloc_t(s, &L.b, L.n);
+    if (!rc) rc |= dalloc_t(s, &L.tmp, L.n);
+    if (!rc) rc |= dalloc_t(s, &L.nc, L.n);
     if (l == 0) L.f = s->flags;
     else if (!rc) rc |= dalloc_t(s, &L.f, L.n);
     int mn = d[0] < d[1] ? d[0] : d[1];
@@ -476,6 +481,40 @@ int pyro_sim_time_stage(PyroSim *s, int stage, int reps, float *avg_ms) {
   HIPCHK(hipEventCreate(&b));
   HIPCHK(hipEventRecord(a, s->stream));
   for (int r = 0; r < reps; r++) enqueue_stage(s, stage, false, s->stream);
+  HIPCHK(hipEventRecord(b, s->stream));
+  HIPCHK(hipEventSynchronize(b));
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *avg_ms = ms / (float)reps;
+  return sync_check(s);
+}
+
+int pyro_sim_time_kernel(PyroSim *s, int kernel, int level, int reps, float *avg_ms) {
+  if (!s || !avg_ms || reps < 1) return fail(PYRO_E_INVALID, "bad argument");
+  if (level < 0 || level >= s->nl) return fail(PYRO_E_INVALID, "bad level");
+  if ((kernel == PYRO_KERNEL_MG_RESTRICT || kernel == PYRO_KERNEL_MG_PROLONG) && level >= s->nl - 1)
+    return fail(PYRO_E_INVALID, "restrict/prolong need a coarser level");
+  if (kernel < 0 || kernel > PYRO_KERNEL_MG_COARSE) return fail(PYRO_E_INVALID, "bad kernel id");
+  if (kernel == PYRO_KERNEL_MG_SMOOTH && (reps & 1)) reps++;  // even: phi ends in its own buffer
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  HIPCHK(hipEventRecord(a, s->stream));
+  for (int r = 0; r < reps; r++) {
+    switch (kernel) {
+      case PYRO_KERNEL_MG_SMOOTH: {
+        const MgLevel &L = s->lv[level];
+        if (r & 1) launch_mg_sweep(L, L.tmp, L.phi, 1.0f, s->stream, level == 0);
+        else launch_mg_sweep(L, L.phi, L.tmp, 1.0f, s->stream, level == 0);
+        break;
+      }
+      case PYRO_KERNEL_MG_RESTRICT: launch_mg_restrict(s->lv[level], s->lv[level + 1], s->stream); break;
+      case PYRO_KERNEL_MG_PROLONG: launch_mg_prolong(s->lv[level + 1], s->lv[level], s->stream); break;
+      default: launch_mg_coarse(s->lv[s->nl - 1], COARSE_ITERS, 1.6f, s->stream); break;
+    }
+  }
   HIPCHK(hipEventRecord(b, s->stream));
   HIPCHK(hipEventSynchronize(b));
   float ms = 0.0f;

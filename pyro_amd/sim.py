@@ -200,6 +200,12 @@ class Sim:
         _native.check_hip(_native.hip().pyro_sim_time_stage(self._h, sid, int(reps), C.byref(ms)))
         return ms.value
 
+    def time_kernel(self, kernel: str, level: int, reps: int) -> float:
+        kid = {"mg_smooth": 0, "mg_restrict": 1, "mg_prolong": 2, "mg_coarse": 3}[kernel]
+        ms = C.c_float()
+        _native.check_hip(_native.hip().pyro_sim_time_kernel(self._h, kid, int(level), int(reps), C.byref(ms)))
+        return ms.value
+
     def add_velocity_noise(self, amp: float, seed: int) -> None:
         _native.check_hip(_native.hip().pyro_sim_add_velocity_noise(self._h, float(amp), int(seed)))
 
