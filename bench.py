@@ -9,11 +9,15 @@ plus U[-1e-3, 1e-3] m/s velocity noise (seed 0).  ``value`` = cells x steps /
 wall seconds, inputs resident in HBM, timed between two device syncs.
 
 Extra fields:
-  roofline      -- the fine-level red-black smoother (``k_mg_smooth<true>``),
-                   the kernel with the most launches and bytes per step:
-                   algorithmic bytes = 12 B per level-0 cell per colour pass,
-                   divided by its average launch time (HIP events on the sim
-                   stream).
+  roofline      -- the level-0 red-black sweep (``k_mg_sweep<true, true>``),
+                   the kernel with the most launches and bytes per step
+                   (16 per step): algorithmic bytes = 16 B per level-0 cell
+                   per sweep (read phi, b, flags and write phi once, in the
+                   reference's f32/u32 layout), divided by its average launch
+                   time (HIP events on the sim stream).  ``traffic`` is the
+                   HBM bytes per launch measured by rocprofv3 FETCH_SIZE +
+                   WRITE_SIZE (profiles/mg_sweep_l0_pool_2cm_traffic.json,
+                   scaled by the copy-kernel calibration), when present.
   cpu_baseline  -- the CPU restatement in oracle/ (kind "port") timed on this
                    host on a bounded sample of the same workload.
   stage_ms      -- per-stage time of one step (HIP events, direct launches).
@@ -31,7 +35,8 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-SMOOTH_BYTES_PER_CELL = 12.0  # phi 4 + flags 4 + b/2 2 + written phi/2 2 (reference layout)
+SWEEP_BYTES_PER_CELL = 16.0  # read phi 4 + b 4 + flags 4, write phi 4 (reference layout)
+TRAFFIC_FILE = ROOT / "profiles" / "mg_sweep_l0_pool_2cm_traffic.json"
 
 
 def log(msg: str) -> None:
@@ -86,11 +91,16 @@ def gpu_run(args):
     sim.time_kernel("mg_smooth", 0, 10)
     smooth_ms = sim.time_kernel("mg_smooth", 0, reps)
     coarse_ms = sim.time_kernel("mg_coarse", 0, 20)
-    algo_bytes = SMOOTH_BYTES_PER_CELL * cells
+    algo_bytes = SWEEP_BYTES_PER_CELL * cells
     achieved = algo_bytes / (smooth_ms * 1e-3) / 1e9
-    log(f"[bench] k_mg_smooth<true> avg {smooth_ms * 1e3:.1f} us -> {achieved:.0f} GB/s algorithmic")
+    log(f"[bench] k_mg_sweep<true,true> avg {smooth_ms * 1e3:.1f} us -> {achieved:.0f} GB/s algorithmic")
+    # achievable streaming bandwidth on this device (plain copy of 2 x 256 MiB)
+    copy_ms = pa.sim.debug_copy(64 << 20, 20)
+    copy_gbs = 2 * (64 << 20) * 4 / (copy_ms * 1e-3) / 1e9
+    log(f"[bench] plain copy: {copy_gbs:.0f} GB/s")
     return dict(cells=cells, wall=wall, stats=st, stage_ms=stage_ms, smooth_ms=smooth_ms,
-                coarse_ms=coarse_ms, achieved=achieved, algo_bytes=algo_bytes, dims=(nx, ny, nz))
+                coarse_ms=coarse_ms, achieved=achieved, algo_bytes=algo_bytes, dims=(nx, ny, nz),
+                copy_gbs=copy_gbs)
 
 
 def cpu_baseline(args, dims):
@@ -138,6 +148,12 @@ def main():
     cpu = None if args.no_cpu else cpu_baseline(args, r["dims"])
     nx, ny, nz = r["dims"]
     value = r["cells"] * args.steps / r["wall"]
+    traffic = None
+    if TRAFFIC_FILE.exists():
+        try:
+            traffic = json.loads(TRAFFIC_FILE.read_text()).get("hbm_bytes_per_launch")
+        except (ValueError, OSError):
+            traffic = None
     out = {
         "metric": "cell-updates/s at 8.2M-cell step; time to 1e-4 pressure residual, learned vs plain MG",
         "value": value,
@@ -161,18 +177,19 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_mg_smooth<true> (level-0 red-black colour pass)",
+            "kernel": "k_mg_sweep<true,true> (level-0 red-black sweep, both colours)",
             "achieved": r["achieved"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": r["achieved"] / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
             "algo_bytes_per_launch": r["algo_bytes"],
             "avg_launch_us": r["smooth_ms"] * 1e3,
         },
         "cpu_baseline": cpu,
         "stage_ms": r["stage_ms"],
         "coarse_solve_ms": r["coarse_ms"],
+        "copy_kernel_GBs": r["copy_gbs"],
         "last_stats": {"hrr_W": r["stats"].hrr, "max_T": r["stats"].max_t, "resid": r["stats"].resid},
     }
     print(json.dumps(out), flush=True)

@@ -120,6 +120,9 @@ int pyro_sim_add_velocity_noise(PyroSim *sim, float amp, uint32_t seed);
 /* Scan the state for NaN/Inf; returns PYRO_E_NONFINITE if any. */
 int pyro_sim_check_finite(PyroSim *sim);
 
+/* Calibration: `reps` launches of a plain copy kernel over n_floats floats
+ * (reads and writes 4 * n_floats bytes each); average ms per launch. */
+int pyro_debug_copy(size_t n_floats, int reps, float *avg_ms);
 /* Test hook: apply the device's exp (fn 0), sqrt (1) or division in/in2 (2). */
 int pyro_debug_math(int fn, const float *in, const float *in2, float *out, size_t n);
 

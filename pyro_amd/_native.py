@@ -67,6 +67,7 @@ def hip() -> C.CDLL:
         lib.pyro_sim_add_velocity_noise.argtypes = [P, f32, u32]
         lib.pyro_sim_check_finite.argtypes = [P]
         lib.pyro_debug_math.argtypes = [i32, P, P, P, sz]
+        lib.pyro_debug_copy.argtypes = [sz, i32, C.POINTER(f32)]
         _hip = lib
     return _hip
 

@@ -135,6 +135,15 @@ __device__ __forceinline__ void decode_face(const Geo &g, uint32_t i, int &x, in
   z = (int)q;
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (XCD = id % 8), each
+// XCD with its own L2.  Remap so that every XCD gets a contiguous range of
+// tiles: neighbouring tiles then share halos in the same L2.
+__device__ __forceinline__ int xcd_tile(int bid, int nb) {
+  const int q = nb >> 3;
+  if (bid >= (q << 3)) return bid;
+  return (bid & 7) * q + (bid >> 3);
+}
+
 // Deterministic block reductions (fixed pairing order) for FP64 stats.
 template <int NT>
 __device__ __forceinline__ double block_sum(double v, double *sh) {

@@ -82,7 +82,7 @@ __device__ __forceinline__ void trace(const Geo &g, const VelPtrs &V, float k, f
 __global__ __launch_bounds__(NT) void k_adv_vel1(const Params *__restrict__ P, Geo g, VelPtrs V,
                                                   float *__restrict__ hu, float *__restrict__ hv,
                                                   float *__restrict__ hw) {
-  uint32_t i = blockIdx.x * NT + threadIdx.x;
+  uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   if (i >= g.nv) return;
   int x, y, z;
   decode_face(g, i, x, y, z);
@@ -116,7 +116,7 @@ __global__ __launch_bounds__(NT) void k_adv_vel2(const Params *__restrict__ P, G
                                                   const float *__restrict__ hw, float *__restrict__ ou,
                                                   float *__restrict__ ov, float *__restrict__ ow,
                                                   const uint8_t *__restrict__ fl) {
-  uint32_t i = blockIdx.x * NT + threadIdx.x;
+  uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   if (i >= g.nv) return;
   int x, y, z;
   decode_face(g, i, x, y, z);
@@ -206,7 +206,7 @@ __device__ __forceinline__ float outside(const Geo &g, float x, float y, float z
 __global__ __launch_bounds__(NT) void k_adv_s1(const Params *__restrict__ P, Geo g, VelPtrs V,
                                                 const float4 *__restrict__ s0, const float4 *__restrict__ s1,
                                                 float4 *__restrict__ h0, float4 *__restrict__ h1) {
-  uint32_t i = blockIdx.x * NT + threadIdx.x;
+  uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   if (i >= g.n) return;
   int x, y, z;
   decode_cell(g, i, x, y, z);
@@ -234,7 +234,7 @@ __global__ __launch_bounds__(NT) void k_adv_s2(const Params *__restrict__ P, Geo
                                                 const float4 *__restrict__ s0, const float4 *__restrict__ s1,
                                                 const float4 *__restrict__ h0, const float4 *__restrict__ h1,
                                                 float4 *__restrict__ o0, float4 *__restrict__ o1) {
-  uint32_t i = blockIdx.x * NT + threadIdx.x;
+  uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   if (i >= g.n) return;
   int x, y, z;
   decode_cell(g, i, x, y, z);

@@ -169,7 +169,7 @@ __global__ __launch_bounds__(NT) void k_solid(const Params *__restrict__ P, Geo 
                                                float4 *__restrict__ solOut, float4 *__restrict__ pyr,
                                                uint8_t *__restrict__ flags, const float4 *__restrict__ s0,
                                                double *__restrict__ part) {
-  uint32_t i = blockIdx.x * NT + threadIdx.x;
+  uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   float stPyro = 0.0f, stMass = 0.0f, stArea = 0.0f;
   if (i < g.n) {
     float4 so = solIn[i];
@@ -226,8 +226,9 @@ __global__ __launch_bounds__(NT) void k_react(const Params *__restrict__ P, Geo 
                                                const float4 *__restrict__ sol, const float4 *__restrict__ pyr,
                                                const float4 *__restrict__ curl, double *__restrict__ part,
                                                float *__restrict__ part_t) {
-  const int y = blockIdx.y;
-  const uint32_t i2 = blockIdx.x * NT + threadIdx.x;
+  const int lb = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const int y = lb / (int)gridDim.x, bxl = lb - y * (int)gridDim.x;
+  const uint32_t i2 = (uint32_t)bxl * NT + threadIdx.x;
   float stH = 0.0f, stR = 0.0f, stS = 0.0f, stF = 0.0f, stM = 0.0f, stT = 0.0f, stL = 0.0f;
   if (i2 < (uint32_t)g.nx * (uint32_t)g.nz) {
     uint32_t zq = fdiv(i2, g.dnx);
@@ -366,7 +367,7 @@ __global__ __launch_bounds__(NT) void k_react(const Params *__restrict__ P, Geo 
     double vL = block_sum<NT>((double)stL, sh);
     float vT = block_max<NT>(stT, shf);
     if (threadIdx.x == 0) {
-      size_t b = (size_t)y * gridDim.x + blockIdx.x;
+      size_t b = (size_t)y * gridDim.x + bxl;
       part[b * 6 + 0] = vH;
       part[b * 6 + 1] = vR;
       part[b * 6 + 2] = vS;

@@ -26,7 +26,7 @@ template <bool STATS>
 __global__ __launch_bounds__(NT) void k_curl(const Params *__restrict__ P, Geo g, const float *__restrict__ U,
                                               const float *__restrict__ V, const float *__restrict__ W,
                                               float4 *__restrict__ curl, float *__restrict__ part) {
-  uint32_t i = blockIdx.x * NT + threadIdx.x;
+  uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   float spd = 0.0f;
   if (i < g.n) {
     int x, y, z;
@@ -115,7 +115,7 @@ __global__ __launch_bounds__(NT) void k_forces(const Params *__restrict__ P, Geo
                                                 float *__restrict__ V, float *__restrict__ W,
                                                 const float4 *__restrict__ s0, const float4 *__restrict__ s1,
                                                 const float4 *__restrict__ curl, const uint8_t *__restrict__ fl) {
-  uint32_t i = blockIdx.x * NT + threadIdx.x;
+  uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   if (i >= g.nv) return;
   int x, y, z;
   decode_face(g, i, x, y, z);
@@ -178,7 +178,7 @@ __global__ __launch_bounds__(NT) void k_divergence(const Params *__restrict__ P,
                                                     const float *__restrict__ U, const float *__restrict__ V,
                                                     const float *__restrict__ W, const float4 *__restrict__ s1,
                                                     const uint8_t *__restrict__ fl, float *__restrict__ b) {
-  uint32_t i = blockIdx.x * NT + threadIdx.x;
+  uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   if (i >= g.n) return;
   if (fl[i] == 1) { b[i] = 0.0f; return; }
   int x, y, z;
@@ -196,7 +196,7 @@ void launch_divergence(const Fields &f, hipStream_t s) {
 __global__ __launch_bounds__(NT) void k_project(const Params *__restrict__ P, Geo g, float *__restrict__ U,
                                                  float *__restrict__ V, float *__restrict__ W,
                                                  const float *__restrict__ phi, const uint8_t *__restrict__ fl) {
-  uint32_t i = blockIdx.x * NT + threadIdx.x;
+  uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   if (i >= g.nv) return;
   int x, y, z;
   decode_face(g, i, x, y, z);
@@ -238,7 +238,7 @@ void launch_set_step(Params *P, float dt, float time, float ign_on, uint32_t fra
 
 // Harness noise: U[-1,1] from the PCG hash of (component, face index, seed).
 __global__ __launch_bounds__(NT) void k_noise(Geo g, float *U, float *V, float *W, float amp, uint32_t seed) {
-  uint32_t i = blockIdx.x * NT + threadIdx.x;
+  uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   if (i >= g.nv) return;
   int x, y, z;
   decode_face(g, i, x, y, z);
@@ -331,6 +331,17 @@ __global__ __launch_bounds__(1024) void k_stats_finalize(Geo g, StatParts st, in
 }
 void launch_stats_finalize(const Geo &g, const StatParts &st, int with_resid, hipStream_t s) {
   k_stats_finalize<<<1, 1024, 0, s>>>(g, st, with_resid);
+}
+
+// ---------------------------------------------------------------- copy
+// Plain streaming copy (16 B per lane): the calibration kernel for HBM byte
+// counters (FETCH_SIZE / WRITE_SIZE) and achievable bandwidth.
+__global__ __launch_bounds__(NT) void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, uint32_t n4) {
+  uint32_t i = blockIdx.x * NT + threadIdx.x;
+  if (i < n4) dst[i] = src[i];
+}
+void launch_copy(const float4 *src, float4 *dst, uint32_t n4, hipStream_t s) {
+  k_copy<<<nblocks(n4), NT, 0, s>>>(src, dst, n4);
 }
 
 // ---------------------------------------------------------------- debug math

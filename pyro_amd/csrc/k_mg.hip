@@ -71,7 +71,7 @@ __global__ __launch_bounds__(NT) void k_mg_restrict(LvArgs F, const float *__res
                                                      const float *__restrict__ bf, const uint8_t *__restrict__ ff,
                                                      int c0, int c1, int c2, FastDiv dc0, FastDiv dc1,
                                                      float *__restrict__ bc, float *__restrict__ pc) {
-  uint32_t i = blockIdx.x * NT + threadIdx.x;
+  uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   if (i >= (uint32_t)c0 * (uint32_t)c1 * (uint32_t)c2) return;
   uint32_t r = fdiv(i, dc0);
   int x = (int)(i - r * (uint32_t)c0);
@@ -102,7 +102,7 @@ __global__ __launch_bounds__(NT) void k_mg_prolong(int f0, int f1, int f2, FastD
                                                     const uint8_t *__restrict__ ff, float *__restrict__ pf, int c0,
                                                     int c1, int c2, const float *__restrict__ pc,
                                                     const uint8_t *__restrict__ fc) {
-  uint32_t i = blockIdx.x * NT + threadIdx.x;
+  uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   if (i >= (uint32_t)f0 * (uint32_t)f1 * (uint32_t)f2) return;
   if (ff[i] == 1) return;
   uint32_t r = fdiv(i, df0);
@@ -143,7 +143,7 @@ constexpr uint8_t NC_SOLID = 0x80, NC_NONE = 0x40;
 
 __global__ __launch_bounds__(NT) void k_mg_counts(LvArgs a, FastDiv d0, const uint8_t *__restrict__ f,
                                                    uint8_t *__restrict__ nc) {
-  uint32_t i = blockIdx.x * NT + threadIdx.x;
+  uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   if (i >= a.n) return;
   if (f[i] == 1) { nc[i] = NC_SOLID; return; }
   uint32_t r = fdiv(i, d0);
@@ -164,7 +164,7 @@ __global__ __launch_bounds__(NT) void k_mg_counts(LvArgs a, FastDiv d0, const ui
 // Coarse cell is solid iff every child inside the fine box is solid (mg.wgsl:74-82).
 __global__ __launch_bounds__(NT) void k_mg_coarse_flags(LvArgs F, const uint8_t *__restrict__ ff, int c0, int c1,
                                                          int c2, FastDiv dc0, FastDiv dc1, uint8_t *__restrict__ fc) {
-  uint32_t i = blockIdx.x * NT + threadIdx.x;
+  uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   if (i >= (uint32_t)c0 * (uint32_t)c1 * (uint32_t)c2) return;
   uint32_t r = fdiv(i, dc0);
   int x = (int)(i - r * (uint32_t)c0);
@@ -193,19 +193,24 @@ void launch_mg_setup(const MgLevel *L, int nl, hipStream_t s) {
 
 // ---------------------------------------------------------------- sweep
 // One full red-black sweep (colour 0 then colour 1, sim.rs:444-448) of a
-// 32x8x4 tile per workgroup, staged in LDS: phi with a halo of 2, codes and
-// h^2 b with a halo of 1.  Colour-0 cells are updated on the tile plus a ring
-// of 1 (the values the tile's colour-1 cells read), then colour-1 cells on
-// the tile.  phi is read from `pin` and written to `pout` (ping-pong), so a
-// tile never sees a neighbour's already-swept values.
+// 32x8x4 tile per workgroup, staged in LDS.  The LDS footprint is x in
+// [x0-4, x0+36) (16-byte aligned rows of 10 float4) by y, z with a halo of 2:
+// phi and neighbour codes on the whole footprint, h^2 b on the ring-1 box.
+// Colour-0 cells are updated on the tile plus a ring of 1 (the values the
+// tile's colour-1 cells read), then colour-1 cells on the tile.  phi is read
+// from `pin` and written to `pout` (ping-pong), so a tile never sees a
+// neighbour's already-swept values.
 // Solid and outside cells hold 0 in the LDS copy of phi, so the neighbour
 // sum needs no mask: a skipped neighbour adds +0.0, which cannot change the
 // running sum (it starts at +0.0 and never becomes -0.0).  The per-cell
 // arithmetic is that of mg.wgsl:25-35, so the result equals two colour passes.
+// VEC (level x-size a multiple of 4): whole float4/uchar4 rows; otherwise
+// one element at a time.
 constexpr int SW_TX = 32, SW_TY = 8, SW_TZ = 4, SW_NT = 256;
-constexpr int SW_PX = SW_TX + 4, SW_PY = SW_TY + 4, SW_PZ = SW_TZ + 4;
-constexpr int SW_BX = SW_TX + 2, SW_BY = SW_TY + 2, SW_BZ = SW_TZ + 2;
-constexpr int SW_PN = SW_PX * SW_PY * SW_PZ, SW_BN = SW_BX * SW_BY * SW_BZ;
+constexpr int SW_PX = SW_TX + 8, SW_PY = SW_TY + 4, SW_PZ = SW_TZ + 4;  // footprint
+constexpr int SW_BY = SW_TY + 2, SW_BZ = SW_TZ + 2;                      // ring-1 rows for b
+constexpr int SW_PN = SW_PX * SW_PY * SW_PZ, SW_BN = SW_PX * SW_BY * SW_BZ;
+constexpr int SW_Q = SW_PX / 4;  // float4 per footprint row
 
 __device__ __forceinline__ float sweep_update(const float *sp, int pi, uint8_t code, float hb, float omega) {
   float sum = 0.0f;
@@ -219,91 +224,120 @@ __device__ __forceinline__ float sweep_update(const float *sp, int pi, uint8_t c
   return mixf_(sp[pi], (sum - hb) / cnt, omega);
 }
 
-template <bool FINE>
+template <bool FINE, bool VEC>
 __global__ __launch_bounds__(SW_NT) void k_mg_sweep(LvArgs a, int ntx, int nty, const float *__restrict__ b,
-                                                     const uint8_t *__restrict__ f, const uint8_t *__restrict__ nc,
-                                                     const float *__restrict__ pin, float *__restrict__ pout,
-                                                     float omega) {
-  __shared__ float sp[SW_PN];
-  __shared__ float sb[SW_BN];
-  __shared__ uint8_t sc[SW_BN];
-  const int bid = blockIdx.x;
+                                                     const uint8_t *__restrict__ nc, const float *__restrict__ pin,
+                                                     float *__restrict__ pout, float omega) {
+  __shared__ __align__(16) float sp[SW_PN];
+  __shared__ __align__(16) float sb[SW_BN];
+  __shared__ __align__(16) uint8_t sc[SW_PN];
+  const int bid = xcd_tile(blockIdx.x, gridDim.x);
   const int tx = bid % ntx, rr = bid / ntx, ty = rr % nty, tz = rr / nty;
-  const int x0 = tx * SW_TX - 2, y0 = ty * SW_TY - 2, z0 = tz * SW_TZ - 2;  // x0 is even
-  const uint32_t sy = (uint32_t)a.d0, sz = (uint32_t)a.d0 * (uint32_t)a.d1;
-  const bool inner = x0 >= 0 && y0 >= 0 && z0 >= 0 && x0 + SW_PX <= a.d0 && y0 + SW_PY <= a.d1 && z0 + SW_PZ <= a.d2;
-  const uint32_t gbase = (uint32_t)x0 + sy * (uint32_t)y0 + sz * (uint32_t)z0;  // wraps for edge tiles; unused there
-  if (inner) {
-    for (int i = threadIdx.x; i < SW_PN; i += SW_NT) {
-      int lx = i % SW_PX, t = i / SW_PX, ly = t % SW_PY, lz = t / SW_PY;
-      uint32_t g = gbase + (uint32_t)lx + sy * (uint32_t)ly + sz * (uint32_t)lz;
-      float pv = pin[g];
-      sp[i] = f[g] == 1 ? 0.0f : pv;
+  const int x0 = tx * SW_TX - 4, y0 = ty * SW_TY - 2, z0 = tz * SW_TZ - 2;  // footprint origin, x0 % 4 == 0
+  const int sy = a.d0, sz = a.d0 * a.d1;
+  if (VEC) {
+    for (int s = threadIdx.x; s < SW_Q * SW_PY * SW_PZ; s += SW_NT) {
+      const int q = s % SW_Q, row = s / SW_Q, ly = row % SW_PY, lz = row / SW_PY;
+      const int gx = x0 + 4 * q, gy = y0 + ly, gz = z0 + lz;
+      float4 pv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      uint32_t cv = 0x40404040u;  // outside: never updated
+      if (gx >= 0 && gx < a.d0 && gy >= 0 && gy < a.d1 && gz >= 0 && gz < a.d2) {
+        const int g = gx + sy * gy + sz * gz;
+        pv = *reinterpret_cast<const float4 *>(pin + g);
+        cv = *reinterpret_cast<const uint32_t *>(nc + g);
+        if (cv & 0x80808080u) {
+          if (cv & 0x80u) pv.x = 0.0f;
+          if (cv & 0x8000u) pv.y = 0.0f;
+          if (cv & 0x800000u) pv.z = 0.0f;
+          if (cv & 0x80000000u) pv.w = 0.0f;
+        }
+      }
+      *reinterpret_cast<float4 *>(sp + 4 * s) = pv;
+      *reinterpret_cast<uint32_t *>(sc + 4 * s) = cv;
     }
-    for (int i = threadIdx.x; i < SW_BN; i += SW_NT) {
-      int lx = i % SW_BX, t = i / SW_BX, ly = t % SW_BY, lz = t / SW_BY;
-      uint32_t g = gbase + (uint32_t)(lx + 1) + sy * (uint32_t)(ly + 1) + sz * (uint32_t)(lz + 1);
-      sb[i] = a.hh * b[g];
-      sc[i] = nc[g];
+    for (int s = threadIdx.x; s < SW_Q * SW_BY * SW_BZ; s += SW_NT) {
+      const int q = s % SW_Q, row = s / SW_Q, ly = row % SW_BY, lz = row / SW_BY;
+      const int gx = x0 + 4 * q, gy = y0 + 1 + ly, gz = z0 + 1 + lz;
+      float4 bv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (gx >= 0 && gx < a.d0 && gy >= 0 && gy < a.d1 && gz >= 0 && gz < a.d2) {
+        bv = *reinterpret_cast<const float4 *>(b + gx + sy * gy + sz * gz);
+        bv.x = a.hh * bv.x; bv.y = a.hh * bv.y; bv.z = a.hh * bv.z; bv.w = a.hh * bv.w;
+      }
+      *reinterpret_cast<float4 *>(sb + 4 * s) = bv;
     }
   } else {
     for (int i = threadIdx.x; i < SW_PN; i += SW_NT) {
-      int lx = i % SW_PX, t = i / SW_PX, ly = t % SW_PY, lz = t / SW_PY;
-      int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+      const int lx = i % SW_PX, row = i / SW_PX, ly = row % SW_PY, lz = row / SW_PY;
+      const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
       float pv = 0.0f;
-      if (gx >= 0 && gy >= 0 && gz >= 0 && gx < a.d0 && gy < a.d1 && gz < a.d2) {
-        uint32_t g = (uint32_t)gx + sy * (uint32_t)gy + sz * (uint32_t)gz;
-        pv = f[g] == 1 ? 0.0f : pin[g];
+      uint8_t cv = NC_NONE;
+      if (gx >= 0 && gx < a.d0 && gy >= 0 && gy < a.d1 && gz >= 0 && gz < a.d2) {
+        const int g = gx + sy * gy + sz * gz;
+        cv = nc[g];
+        pv = (cv & NC_SOLID) ? 0.0f : pin[g];
       }
       sp[i] = pv;
+      sc[i] = cv;
     }
     for (int i = threadIdx.x; i < SW_BN; i += SW_NT) {
-      int lx = i % SW_BX, t = i / SW_BX, ly = t % SW_BY, lz = t / SW_BY;
-      int gx = x0 + 1 + lx, gy = y0 + 1 + ly, gz = z0 + 1 + lz;
+      const int lx = i % SW_PX, row = i / SW_PX, ly = row % SW_BY, lz = row / SW_BY;
+      const int gx = x0 + lx, gy = y0 + 1 + ly, gz = z0 + 1 + lz;
       float bv = 0.0f;
-      uint8_t code = NC_NONE;
-      if (gx >= 0 && gy >= 0 && gz >= 0 && gx < a.d0 && gy < a.d1 && gz < a.d2) {
-        uint32_t g = (uint32_t)gx + sy * (uint32_t)gy + sz * (uint32_t)gz;
-        bv = a.hh * b[g];
-        code = nc[g];
-      }
+      if (gx >= 0 && gx < a.d0 && gy >= 0 && gy < a.d1 && gz >= 0 && gz < a.d2) bv = a.hh * b[gx + sy * gy + sz * gz];
       sb[i] = bv;
-      sc[i] = code;
     }
   }
   __syncthreads();
-  // colour 0 on tile + ring 1: one thread per x-pair, picks the cell of colour 0
-  constexpr int RP = SW_BX / 2, RN = RP * SW_BY * SW_BZ;
+  // colour 0 on the tile + ring 1 (x in [3, 37)): one thread per x-pair
+  constexpr int RP = (SW_TX + 2) / 2, RN = RP * SW_BY * SW_BZ;
   for (int i = threadIdx.x; i < RN; i += SW_NT) {
-    int px = i % RP, t = i / RP, ly = t % SW_BY, lz = t / SW_BY;
-    int lxb = 2 * px + ((1 + 2 * px + (y0 + ly + 1) + (z0 + lz + 1)) & 1);  // (gx+gy+gz) even
-    int bi = (lz * SW_BY + ly) * SW_BX + lxb;
-    uint8_t code = sc[bi];
+    const int px = i % RP, row = i / RP, ly = row % SW_BY, lz = row / SW_BY;
+    const int lx = 3 + 2 * px + ((3 + 2 * px + y0 + ly + 1 + z0 + lz + 1) & 1);  // (gx+gy+gz) even
+    const int pi = ((lz + 1) * SW_PY + (ly + 1)) * SW_PX + lx;
+    const uint8_t code = sc[pi];
     if (code & (NC_SOLID | NC_NONE)) continue;
-    int pi = ((lz + 1) * SW_PY + (ly + 1)) * SW_PX + (lxb + 1);
-    sp[pi] = sweep_update(sp, pi, code, sb[bi], omega);
+    sp[pi] = sweep_update(sp, pi, code, sb[(lz * SW_BY + ly) * SW_PX + lx], omega);
   }
   __syncthreads();
-  // colour 1 on the tile
+  // colour 1 on the tile (x in [4, 36))
   constexpr int BP = SW_TX / 2, BN = BP * SW_TY * SW_TZ;
   for (int i = threadIdx.x; i < BN; i += SW_NT) {
-    int px = i % BP, t = i / BP, ly = t % SW_TY, lz = t / SW_TY;
-    int lxb = 1 + 2 * px + ((2 + 2 * px + (y0 + ly + 2) + (z0 + lz + 2) + 1) & 1);  // (gx+gy+gz) odd
-    int bi = ((lz + 1) * SW_BY + (ly + 1)) * SW_BX + lxb;
-    uint8_t code = sc[bi];
+    const int px = i % BP, row = i / BP, ly = row % SW_TY, lz = row / SW_TY;
+    const int lx = 4 + 2 * px + ((4 + 2 * px + y0 + ly + 2 + z0 + lz + 2 + 1) & 1);  // (gx+gy+gz) odd
+    const int pi = ((lz + 2) * SW_PY + (ly + 2)) * SW_PX + lx;
+    const uint8_t code = sc[pi];
     if (code & (NC_SOLID | NC_NONE)) continue;
-    int pi = ((lz + 2) * SW_PY + (ly + 2)) * SW_PX + (lxb + 1);
-    sp[pi] = sweep_update(sp, pi, code, sb[bi], omega);
+    sp[pi] = sweep_update(sp, pi, code, sb[((lz + 1) * SW_BY + (ly + 1)) * SW_PX + lx], omega);
   }
   __syncthreads();
-  constexpr int TN = SW_TX * SW_TY * SW_TZ;
-  for (int i = threadIdx.x; i < TN; i += SW_NT) {
-    int lx = i % SW_TX, t = i / SW_TX, ly = t % SW_TY, lz = t / SW_TY;
-    int gx = x0 + 2 + lx, gy = y0 + 2 + ly, gz = z0 + 2 + lz;
-    if (gx < a.d0 && gy < a.d1 && gz < a.d2) {
-      uint32_t g = (uint32_t)gx + sy * (uint32_t)gy + sz * (uint32_t)gz;
-      uint8_t code = sc[((lz + 1) * SW_BY + (ly + 1)) * SW_BX + lx + 1];
-      pout[g] = (code & NC_SOLID) ? pin[g] : sp[((lz + 2) * SW_PY + (ly + 2)) * SW_PX + lx + 2];
+  if (VEC) {
+    constexpr int WN = (SW_TX / 4) * SW_TY * SW_TZ;
+    for (int s = threadIdx.x; s < WN; s += SW_NT) {
+      const int q = s % (SW_TX / 4), row = s / (SW_TX / 4), ly = row % SW_TY, lz = row / SW_TY;
+      const int gx = x0 + 4 + 4 * q, gy = y0 + 2 + ly, gz = z0 + 2 + lz;
+      if (gx >= a.d0 || gy >= a.d1 || gz >= a.d2) continue;
+      const int g = gx + sy * gy + sz * gz;
+      const int pi = ((lz + 2) * SW_PY + (ly + 2)) * SW_PX + 4 + 4 * q;
+      float4 v = *reinterpret_cast<const float4 *>(sp + pi);
+      const uint32_t cv = *reinterpret_cast<const uint32_t *>(sc + pi);
+      if (cv & 0x80808080u) {  // solid cells keep their phi (zeroed only in LDS)
+        const float4 o = *reinterpret_cast<const float4 *>(pin + g);
+        if (cv & 0x80u) v.x = o.x;
+        if (cv & 0x8000u) v.y = o.y;
+        if (cv & 0x800000u) v.z = o.z;
+        if (cv & 0x80000000u) v.w = o.w;
+      }
+      *reinterpret_cast<float4 *>(pout + g) = v;
+    }
+  } else {
+    constexpr int TN = SW_TX * SW_TY * SW_TZ;
+    for (int i = threadIdx.x; i < TN; i += SW_NT) {
+      const int lx = i % SW_TX, row = i / SW_TX, ly = row % SW_TY, lz = row / SW_TY;
+      const int gx = x0 + 4 + lx, gy = y0 + 2 + ly, gz = z0 + 2 + lz;
+      if (gx >= a.d0 || gy >= a.d1 || gz >= a.d2) continue;
+      const int g = gx + sy * gy + sz * gz;
+      const int pi = ((lz + 2) * SW_PY + (ly + 2)) * SW_PX + 4 + lx;
+      pout[g] = (sc[pi] & NC_SOLID) ? pin[g] : sp[pi];
     }
   }
 }
@@ -312,10 +346,13 @@ void launch_mg_sweep(const MgLevel &L, const float *pin, float *pout, float omeg
   LvArgs a = lv_args(L);
   int ntx = (L.d[0] + SW_TX - 1) / SW_TX, nty = (L.d[1] + SW_TY - 1) / SW_TY, ntz = (L.d[2] + SW_TZ - 1) / SW_TZ;
   int nb = ntx * nty * ntz;
-  if (fine)
-    k_mg_sweep<true><<<nb, SW_NT, 0, s>>>(a, ntx, nty, L.b, L.f, L.nc, pin, pout, omega);
+  bool vec = (L.d[0] % 4) == 0;
+  if (fine && vec)
+    k_mg_sweep<true, true><<<nb, SW_NT, 0, s>>>(a, ntx, nty, L.b, L.nc, pin, pout, omega);
+  else if (vec)
+    k_mg_sweep<false, true><<<nb, SW_NT, 0, s>>>(a, ntx, nty, L.b, L.nc, pin, pout, omega);
   else
-    k_mg_sweep<false><<<nb, SW_NT, 0, s>>>(a, ntx, nty, L.b, L.f, L.nc, pin, pout, omega);
+    k_mg_sweep<false, false><<<nb, SW_NT, 0, s>>>(a, ntx, nty, L.b, L.nc, pin, pout, omega);
 }
 
 // ---------------------------------------------------------------- coarse
@@ -459,7 +496,7 @@ void launch_mg_coarse(const MgLevel &L, int iters, float omega, hipStream_t s) {
 __global__ __launch_bounds__(NT) void k_mg_resid(LvArgs a, FastDiv d0, const float *__restrict__ p,
                                                   const float *__restrict__ b, const uint8_t *__restrict__ f,
                                                   double *__restrict__ part) {
-  uint32_t i = blockIdx.x * NT + threadIdx.x;
+  uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   double r2 = 0.0, b2 = 0.0;
   if (i < a.n && f[i] != 1) {
     uint32_t r = fdiv(i, d0);

@@ -65,6 +65,7 @@ bool mg_coarse_fits_lds(const MgLevel &L);
 void launch_mg_resid(const MgLevel &L, StatParts *st, hipStream_t s);
 int mg_resid_blocks(const MgLevel &L);
 
+void launch_copy(const float4 *src, float4 *dst, uint32_t n4, hipStream_t s);
 // Debug / test: apply device math to an array (0 exp, 1 sqrt, 2 div by in2).
 void launch_debug_math(int fn, const float *in, const float *in2, float *out, uint32_t n, hipStream_t s);
 

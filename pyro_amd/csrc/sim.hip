@@ -552,6 +552,32 @@ int pyro_sim_check_finite(PyroSim *s) {
   return 0;
 }
 
+int pyro_debug_copy(size_t n_floats, int reps, float *avg_ms) {
+  if (n_floats == 0 || n_floats % 4 || n_floats / 4 >= (1ull << 31) || reps < 1 || !avg_ms)
+    return fail(PYRO_E_INVALID, "bad argument");
+  float *a = nullptr, *b = nullptr;
+  HIPCHK(hipMalloc(&a, n_floats * 4));
+  HIPCHK(hipMalloc(&b, n_floats * 4));
+  HIPCHK(hipMemset(a, 0, n_floats * 4));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  launch_copy(reinterpret_cast<const float4 *>(a), reinterpret_cast<float4 *>(b), (uint32_t)(n_floats / 4), nullptr);
+  HIPCHK(hipEventRecord(e0, nullptr));
+  for (int r = 0; r < reps; r++)
+    launch_copy(reinterpret_cast<const float4 *>(a), reinterpret_cast<float4 *>(b), (uint32_t)(n_floats / 4), nullptr);
+  HIPCHK(hipEventRecord(e1, nullptr));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  *avg_ms = ms / (float)reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return 0;
+}
+
 int pyro_debug_math(int fn, const float *in, const float *in2, float *out, size_t n) {
   if (!in || !out || (fn == 2 && !in2) || fn < 0 || fn > 2) return fail(PYRO_E_INVALID, "bad argument");
   if (n == 0) return 0;

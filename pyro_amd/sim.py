@@ -225,3 +225,10 @@ def debug_math(fn: int, x: np.ndarray, y: np.ndarray | None = None) -> np.ndarra
         yp = y.ctypes.data
     _native.check_hip(_native.hip().pyro_debug_math(fn, x.ctypes.data, yp, out.ctypes.data, x.size))
     return out
+
+
+def debug_copy(n_floats: int, reps: int) -> float:
+    """Average ms of a plain device copy of n_floats floats (calibration)."""
+    ms = C.c_float()
+    _native.check_hip(_native.hip().pyro_debug_copy(int(n_floats), int(reps), C.byref(ms)))
+    return ms.value
