@@ -87,7 +87,9 @@ int pyro_sim_step(PyroSim *sim, float dt, int want_stats);
 /* Sim::read_stats (sim.rs:510-530): the 1040 slots of the last stats step
  * in FP64 (slot map of stats.wgsl:2-3; 10 = sum r^2, 11 = sum b^2). */
 int pyro_sim_read_stats(PyroSim *sim, double *out, size_t n);
-/* Sim::read_scalars (sim.rs:494-508) and every other field, blocking. */
+/* Sim::read_scalars (sim.rs:494-508) and every other field, blocking.
+ * Writing SOL (or SOL_OUT) sets both SOL buffers and recomputes the solid
+ * flags, as Sim::new does from one voxelized array (sim.rs:136-140). */
 int pyro_sim_read_field(PyroSim *sim, int field, void *dst, size_t bytes);
 int pyro_sim_write_field(PyroSim *sim, int field, const void *src, size_t bytes);
 /* Sim::cells (sim.rs:234), Sim::time / dims / cycles / levels. */
@@ -123,6 +125,37 @@ int pyro_sim_check_finite(PyroSim *sim);
 /* Calibration: `reps` launches of a plain copy kernel over n_floats floats
  * (reads and writes 4 * n_floats bytes each); average ms per launch. */
 int pyro_debug_copy(size_t n_floats, int reps, float *avg_ms);
+/* ------------------------------------------------------------------------
+ * Stand-alone pressure solve on one grid (no reference counterpart: the
+ * reference only runs a fixed number of V-cycles per step, sim.rs:405).
+ * Operator: level 0 of mg.wgsl (ground Neumann, phi = 0 outside the sides and
+ * top, Neumann at solids); FP64 right-hand side, solution and residuals.
+ * ------------------------------------------------------------------------ */
+typedef struct PyroPoisson PyroPoisson;
+enum PyroSolver {
+  PYRO_SOLVER_VCYCLE = 0, /* the reference V-cycles (sim.rs:451-491) on FP32 */
+  PYRO_SOLVER_REFINE = 1, /* FP64-residual iterative refinement, FP32 V-cycle corrections */
+  PYRO_SOLVER_MGPCG = 2   /* flexible CG, FP64 vectors/dots, FP32 V-cycle preconditioner */
+};
+typedef struct {
+  int iterations; /* outer iterations */
+  int vcycles;    /* V-cycles applied */
+  double rel_resid; /* sqrt(sum r^2 / sum b^2) over fluid cells, FP64 */
+  float ms;       /* device time of the solve (HIP events) */
+} PyroSolveInfo;
+const char *pyro_poisson_last_error(void);
+/* flags: N x u32, 1 = solid (Neumann). */
+int pyro_poisson_create(uint32_t nx, uint32_t ny, uint32_t nz, float h, const uint32_t *flags,
+                        PyroPoisson **out);
+int pyro_poisson_destroy(PyroPoisson *p);
+int pyro_poisson_levels(const PyroPoisson *p, int *ref_levels, int *pc_levels);
+/* Set the right-hand side b and/or the initial guess phi (either may be NULL). */
+int pyro_poisson_set(PyroPoisson *p, const double *b, const double *phi);
+int pyro_poisson_get(PyroPoisson *p, double *phi);
+int pyro_poisson_residual(PyroPoisson *p, double *rel);
+/* Iterate until the relative residual is <= tol or max_iters outer iterations. */
+int pyro_poisson_solve(PyroPoisson *p, int method, double tol, int max_iters, PyroSolveInfo *info);
+
 /* Test hook: apply the device's exp (fn 0), sqrt (1) or division in/in2 (2). */
 int pyro_debug_math(int fn, const float *in, const float *in2, float *out, size_t n);
 

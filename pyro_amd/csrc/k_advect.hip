@@ -77,6 +77,17 @@ __device__ __forceinline__ void trace(const Geo &g, const VelPtrs &V, float k, f
     ox = x + k * u2x; oy = y + k * u2y; oz = z + k * u2z;
   }
 }
+// traceBack and traceFwd from one point share their first sample u1 = velAt(x).
+__device__ __forceinline__ void trace2(const Geo &g, const VelPtrs &V, float k, float x, float y, float z, float &bx,
+                                       float &by, float &bz, float &fx, float &fy, float &fz) {
+  float u1x, u1y, u1z, u2x, u2y, u2z;
+  velAt(g, V, x, y, z, u1x, u1y, u1z);
+  const float hk = 0.5f * k;
+  velAt(g, V, x - hk * u1x, y - hk * u1y, z - hk * u1z, u2x, u2y, u2z);
+  bx = x - k * u2x; by = y - k * u2y; bz = z - k * u2z;
+  velAt(g, V, x + hk * u1x, y + hk * u1y, z + hk * u1z, u2x, u2y, u2z);
+  fx = x + k * u2x; fy = y + k * u2y; fz = z + k * u2z;
+}
 
 // ---------------------------------------------------------------- adv_vel1
 __global__ __launch_bounds__(NT) void k_adv_vel1(const Params *__restrict__ P, Geo g, VelPtrs V,
@@ -136,8 +147,7 @@ __global__ __launch_bounds__(NT) void k_adv_vel2(const Params *__restrict__ P, G
     const float *Hc = c == 0 ? hu : (c == 1 ? hv : hw);
     float px = (float)x + offx, py = (float)y + offy, pz = (float)z + offz;
     float bx, by, bz, fx, fy, fz;
-    trace<-1>(g, V, k, px, py, pz, bx, by, bz);
-    trace<1>(g, V, k, px, py, pz, fx, fy, fz);
+    trace2(g, V, k, px, py, pz, bx, by, bz, fx, fy, fz);
     bx = bx - offx; by = by - offy; bz = bz - offz;
     fx = fx - offx; fy = fy - offy; fz = fz - offz;
     float lo, hi;
@@ -241,8 +251,7 @@ __global__ __launch_bounds__(NT) void k_adv_s2(const Params *__restrict__ P, Geo
   const float k = P->dt / P->dx;
   float px = (float)x + 0.5f, py = (float)y + 0.5f, pz = (float)z + 0.5f;
   float bx, by, bz, fx, fy, fz;
-  trace<-1>(g, V, k, px, py, pz, bx, by, bz);
-  trace<1>(g, V, k, px, py, pz, fx, fy, fz);
+  trace2(g, V, k, px, py, pz, bx, by, bz, fx, fy, fz);
   bx = bx - 0.5f; by = by - 0.5f; bz = bz - 0.5f;
   fx = fx - 0.5f; fy = fy - 0.5f; fz = fz - 0.5f;
   float4 r0 = h0[i], r1 = h1[i];

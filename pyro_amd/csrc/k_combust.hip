@@ -82,7 +82,7 @@ __device__ float faceIrr(const Params *__restrict__ P, const Geo &g, const float
 }
 
 // Surface energy balance and pyrolysis of one solid cell (combust.wgsl:60-128).
-__device__ __noinline__ void solid_cell(const Params *__restrict__ P, const Geo &g, const float4 *__restrict__ solIn,
+__device__ __forceinline__ void solid_cell(const Params *__restrict__ P, const Geo &g, const float4 *__restrict__ solIn,
                                         const float4 *__restrict__ s0, int x, int y, int z, float4 so, float4 &o,
                                         float4 &po, uint32_t &fl, float &stPyro, float &stMass, float &stArea) {
   fl = 1;
@@ -164,6 +164,17 @@ __device__ __noinline__ void solid_cell(const Params *__restrict__ P, const Geo 
   stMass = (o.y + o.z) * P->dx * P->dx * P->dx * (kind == 2u ? 0.0f : 1.0f);
 }
 
+// Sparse form of the per-cell pass of combust.wgsl:50-132.  Two facts hold at
+// the start of every step (set up by launch_init_flags and kept by this
+// kernel): flags[i] == 1 exactly when sol_in[i] is solid, and a cell that is
+// not solid holds the same SOL value in both ping-pong buffers (solids only
+// ever disappear).  So a cell with flag 0 needs no work: the reference would
+// copy SOL unchanged, write pyr = 0 and flag 0, all already in place.  A cell
+// that burns out gets flag FLAG_BURNT (read as "not solid" by every
+// consumer, which test == 1); on the next step its SOL value is copied into
+// the other buffer and the flag becomes 0.
+constexpr uint8_t FLAG_BURNT = 2;
+
 template <bool STATS>
 __global__ __launch_bounds__(NT) void k_solid(const Params *__restrict__ P, Geo g, const float4 *__restrict__ solIn,
                                                float4 *__restrict__ solOut, float4 *__restrict__ pyr,
@@ -172,17 +183,22 @@ __global__ __launch_bounds__(NT) void k_solid(const Params *__restrict__ P, Geo 
   uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   float stPyro = 0.0f, stMass = 0.0f, stArea = 0.0f;
   if (i < g.n) {
-    float4 so = solIn[i];
-    float4 o = so, po = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    uint32_t fl = 0;
-    if (isSolidS(P, so)) {
+    const uint8_t fprev = flags[i];
+    if (fprev == FLAG_BURNT) {
+      solOut[i] = solIn[i];
+      pyr[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      flags[i] = 0;
+    } else if (fprev == 1) {
+      float4 so = solIn[i];
+      float4 o = so, po = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      uint32_t fl = 0;
       int x, y, z;
       decode_cell(g, i, x, y, z);
       solid_cell(P, g, solIn, s0, x, y, z, so, o, po, fl, stPyro, stMass, stArea);
+      solOut[i] = o;
+      pyr[i] = po;
+      flags[i] = fl ? 1 : FLAG_BURNT;
     }
-    solOut[i] = o;
-    pyr[i] = po;
-    flags[i] = (uint8_t)fl;
   }
   if (STATS) {
     __shared__ double sh[NT / 64];
@@ -203,6 +219,16 @@ void launch_solid(const Fields &f, StatParts *st, hipStream_t s) {
     k_solid<true><<<nb, NT, 0, s>>>(f.P, f.g, f.sol_in, f.sol_out, f.pyr, f.flags, f.s0, st->solid);
   else
     k_solid<false><<<nb, NT, 0, s>>>(f.P, f.g, f.sol_in, f.sol_out, f.pyr, f.flags, f.s0, nullptr);
+}
+
+// flags = isSolidS(sol) (combust.wgsl:11-14), establishing the invariant above.
+__global__ __launch_bounds__(NT) void k_init_flags(const Params *__restrict__ P, Geo g,
+                                                    const float4 *__restrict__ sol, uint8_t *__restrict__ flags) {
+  uint32_t i = blockIdx.x * NT + threadIdx.x;
+  if (i < g.n) flags[i] = isSolidS(P, sol[i]) ? 1 : 0;
+}
+void launch_init_flags(const Params *P, const Geo &g, const float4 *sol, uint8_t *flags, hipStream_t s) {
+  k_init_flags<<<nblocks(g.n), NT, 0, s>>>(P, g, sol, flags);
 }
 
 // ---------------------------------------------------------------- react

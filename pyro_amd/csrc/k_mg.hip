@@ -14,31 +14,31 @@
 
 namespace pyro {
 
-// Neighbour sum and count of cell (x,y,z) (mg.wgsl:25-33 / :56-64).
+// Neighbour sum and count of cell (x,y,z) (mg.wgsl:25-33 / :56-64): fluid
+// neighbours add phi and count 1, neighbours outside the box on a side or the
+// top count wdir (1 in mg.wgsl), the ground and solids are skipped.  The
+// count is formed as fluid + wdir * outside, which for wdir = 1 is the same
+// small integer as the reference's running sum of 1.0.
 __device__ __forceinline__ void nsum(const int d0, const int d1, const int d2, const float *__restrict__ p,
-                                     const uint8_t *__restrict__ f, int x, int y, int z, uint32_t i, float &sum,
-                                     float &cnt) {
+                                     const uint8_t *__restrict__ f, int x, int y, int z, uint32_t i, float wdir,
+                                     float &sum, float &cnt) {
   const uint32_t sy = (uint32_t)d0, sz = (uint32_t)d0 * (uint32_t)d1;
   sum = 0.0f;
-  cnt = 0.0f;
-  // +x
-  if (x + 1 < d0) { if (f[i + 1] != 1) { sum += p[i + 1]; cnt += 1.0f; } } else cnt += 1.0f;
-  // -x
-  if (x - 1 >= 0) { if (f[i - 1] != 1) { sum += p[i - 1]; cnt += 1.0f; } } else cnt += 1.0f;
-  // +y
-  if (y + 1 < d1) { if (f[i + sy] != 1) { sum += p[i + sy]; cnt += 1.0f; } } else cnt += 1.0f;
-  // -y (ground: skipped)
-  if (y - 1 >= 0) { if (f[i - sy] != 1) { sum += p[i - sy]; cnt += 1.0f; } }
-  // +z
-  if (z + 1 < d2) { if (f[i + sz] != 1) { sum += p[i + sz]; cnt += 1.0f; } } else cnt += 1.0f;
-  // -z
-  if (z - 1 >= 0) { if (f[i - sz] != 1) { sum += p[i - sz]; cnt += 1.0f; } } else cnt += 1.0f;
+  int fl = 0, out = 0;
+  if (x + 1 < d0) { if (f[i + 1] != 1) { sum += p[i + 1]; fl++; } } else out++;
+  if (x - 1 >= 0) { if (f[i - 1] != 1) { sum += p[i - 1]; fl++; } } else out++;
+  if (y + 1 < d1) { if (f[i + sy] != 1) { sum += p[i + sy]; fl++; } } else out++;
+  if (y - 1 >= 0) { if (f[i - sy] != 1) { sum += p[i - sy]; fl++; } }  // ground: skipped
+  if (z + 1 < d2) { if (f[i + sz] != 1) { sum += p[i + sz]; fl++; } } else out++;
+  if (z - 1 >= 0) { if (f[i - sz] != 1) { sum += p[i - sz]; fl++; } } else out++;
+  cnt = (float)fl + wdir * (float)out;
 }
 
 struct LvArgs {
   int d0, d1, d2;
   uint32_t n;
   float hh;
+  float wdir;        // weight of a neighbour outside the box (1 = mg.wgsl)
   FastDiv dhx, dny;  // ceil(d0/2), d1
   uint32_t nhalf;    // ceil(d0/2)*d1*d2
 };
@@ -47,6 +47,7 @@ static LvArgs lv_args(const MgLevel &L) {
   a.d0 = L.d[0]; a.d1 = L.d[1]; a.d2 = L.d[2];
   a.n = L.n;
   a.hh = L.hh;
+  a.wdir = L.wdir;
   uint32_t hx = (uint32_t)(L.d[0] + 1) / 2;
   a.dhx = make_fastdiv(hx);
   a.dny = make_fastdiv((uint32_t)L.d[1]);
@@ -61,7 +62,7 @@ __device__ __forceinline__ float resid_at(const LvArgs &F, const float *__restri
   if (f[i] == 1) return 0.0f;
   float pcen = p[i];
   float sum, cnt;
-  nsum(F.d0, F.d1, F.d2, p, f, x, y, z, i, sum, cnt);
+  nsum(F.d0, F.d1, F.d2, p, f, x, y, z, i, F.wdir, sum, cnt);
   return b[i] - (sum - cnt * pcen) / F.hh;
 }
 
@@ -138,8 +139,13 @@ void launch_mg_prolong(const MgLevel &C, const MgLevel &F, hipStream_t s) {
 // a step and restriction derives the same coarse flags in every cycle):
 //   bit 7: solid (never updated, phi is Neumann for its neighbours)
 //   bit 6: fluid cell without a counted neighbour (mg.wgsl:34 skips it)
-//   bits 0-2: cnt of mg.wgsl:25-33 (fluid neighbours + sides/top outside)
+//   bits 0-2: fluid neighbours; bits 3-5: neighbours outside the box on a side
+//   or the top.  cnt of mg.wgsl:25-33 = fluid + wdir * outside; with
+//   wdir = 1 it is the same small integer, exactly.
 constexpr uint8_t NC_SOLID = 0x80, NC_NONE = 0x40;
+__device__ __forceinline__ float nc_count(uint32_t code, float wdir) {
+  return (float)(code & 7u) + wdir * (float)((code >> 3) & 7u);
+}
 
 __global__ __launch_bounds__(NT) void k_mg_counts(LvArgs a, FastDiv d0, const uint8_t *__restrict__ f,
                                                    uint8_t *__restrict__ nc) {
@@ -151,14 +157,14 @@ __global__ __launch_bounds__(NT) void k_mg_counts(LvArgs a, FastDiv d0, const ui
   uint32_t zq = fdiv(r, a.dny);
   int y = (int)(r - zq * (uint32_t)a.d1), z = (int)zq;
   const uint32_t sy = (uint32_t)a.d0, sz = (uint32_t)a.d0 * (uint32_t)a.d1;
-  int cnt = 0;
-  cnt += (x + 1 < a.d0) ? (f[i + 1] != 1) : 1;
-  cnt += (x - 1 >= 0) ? (f[i - 1] != 1) : 1;
-  cnt += (y + 1 < a.d1) ? (f[i + sy] != 1) : 1;
-  cnt += (y - 1 >= 0) ? (f[i - sy] != 1) : 0;  // ground: not counted
-  cnt += (z + 1 < a.d2) ? (f[i + sz] != 1) : 1;
-  cnt += (z - 1 >= 0) ? (f[i - sz] != 1) : 1;
-  nc[i] = cnt == 0 ? NC_NONE : (uint8_t)cnt;
+  int fl = 0, out = 0;
+  if (x + 1 < a.d0) fl += f[i + 1] != 1; else out++;
+  if (x - 1 >= 0) fl += f[i - 1] != 1; else out++;
+  if (y + 1 < a.d1) fl += f[i + sy] != 1; else out++;
+  if (y - 1 >= 0) fl += f[i - sy] != 1;  // ground: not counted
+  if (z + 1 < a.d2) fl += f[i + sz] != 1; else out++;
+  if (z - 1 >= 0) fl += f[i - sz] != 1; else out++;
+  nc[i] = (fl + out) == 0 ? NC_NONE : (uint8_t)(fl | (out << 3));
 }
 
 // Coarse cell is solid iff every child inside the fine box is solid (mg.wgsl:74-82).
@@ -212,7 +218,8 @@ constexpr int SW_BY = SW_TY + 2, SW_BZ = SW_TZ + 2;                      // ring
 constexpr int SW_PN = SW_PX * SW_PY * SW_PZ, SW_BN = SW_PX * SW_BY * SW_BZ;
 constexpr int SW_Q = SW_PX / 4;  // float4 per footprint row
 
-__device__ __forceinline__ float sweep_update(const float *sp, int pi, uint8_t code, float hb, float omega) {
+__device__ __forceinline__ float sweep_update(const float *sp, int pi, uint8_t code, float hb, float omega,
+                                              float wdir) {
   float sum = 0.0f;
   sum += sp[pi + 1];
   sum += sp[pi - 1];
@@ -220,7 +227,7 @@ __device__ __forceinline__ float sweep_update(const float *sp, int pi, uint8_t c
   sum += sp[pi - SW_PX];
   sum += sp[pi + SW_PX * SW_PY];
   sum += sp[pi - SW_PX * SW_PY];
-  const float cnt = (float)(code & 7u);
+  const float cnt = nc_count(code, wdir);
   return mixf_(sp[pi], (sum - hb) / cnt, omega);
 }
 
@@ -296,7 +303,7 @@ __global__ __launch_bounds__(SW_NT) void k_mg_sweep(LvArgs a, int ntx, int nty, 
     const int pi = ((lz + 1) * SW_PY + (ly + 1)) * SW_PX + lx;
     const uint8_t code = sc[pi];
     if (code & (NC_SOLID | NC_NONE)) continue;
-    sp[pi] = sweep_update(sp, pi, code, sb[(lz * SW_BY + ly) * SW_PX + lx], omega);
+    sp[pi] = sweep_update(sp, pi, code, sb[(lz * SW_BY + ly) * SW_PX + lx], omega, a.wdir);
   }
   __syncthreads();
   // colour 1 on the tile (x in [4, 36))
@@ -307,7 +314,7 @@ __global__ __launch_bounds__(SW_NT) void k_mg_sweep(LvArgs a, int ntx, int nty, 
     const int pi = ((lz + 2) * SW_PY + (ly + 2)) * SW_PX + lx;
     const uint8_t code = sc[pi];
     if (code & (NC_SOLID | NC_NONE)) continue;
-    sp[pi] = sweep_update(sp, pi, code, sb[((lz + 1) * SW_BY + (ly + 1)) * SW_PX + lx], omega);
+    sp[pi] = sweep_update(sp, pi, code, sb[((lz + 1) * SW_BY + (ly + 1)) * SW_PX + lx], omega, a.wdir);
   }
   __syncthreads();
   if (VEC) {
@@ -383,7 +390,7 @@ __global__ __launch_bounds__(CO_NT) void k_mg_coarse(LvArgs a, const float *__re
     }
     sp[i] = v;
   }
-  // per owned cell: LDS index (bits 0-15) | cnt (16-18) | active (31)
+  // per owned cell: LDS index (bits 0-15) | fluid (16-18) | outside (19-21) | active (31)
   uint32_t pk[2][CO_K];
   float hb[2][CO_K];
 #pragma unroll
@@ -404,7 +411,7 @@ __global__ __launch_bounds__(CO_NT) void k_mg_coarse(LvArgs a, const float *__re
       uint8_t code = nc[g];
       if (code & (NC_SOLID | NC_NONE)) continue;
       uint32_t li = (uint32_t)(((z + 1) * P1 + (y + 1)) * P0 + (x + 1));
-      pk[c][j] = li | ((uint32_t)(code & 7u) << 16) | 0x80000000u;
+      pk[c][j] = li | ((uint32_t)(code & 63u) << 16) | 0x80000000u;
       hb[c][j] = a.hh * b[g];
     }
   }
@@ -433,7 +440,7 @@ __global__ __launch_bounds__(CO_NT) void k_mg_coarse(LvArgs a, const float *__re
           sum += nb[q][3];
           sum += nb[q][4];
           sum += nb[q][5];
-          const float cnt = (float)((w >> 16) & 7u);
+          const float cnt = nc_count(w >> 16, a.wdir);
           const float nv = mixf_(nb[q][6], (sum - hb[c][j0 + q]) / cnt, omega);
           if (w & 0x80000000u) sp[(int)(w & 0xffffu)] = nv;
         }
@@ -468,7 +475,7 @@ __global__ __launch_bounds__(1024) void k_mg_coarse_global(LvArgs a, const float
         uint32_t i = (uint32_t)x + (uint32_t)a.d0 * ((uint32_t)y + (uint32_t)a.d1 * (uint32_t)z);
         if (f[i] == 1) continue;
         float sum, cnt;
-        nsum(a.d0, a.d1, a.d2, p, f, x, y, z, i, sum, cnt);
+        nsum(a.d0, a.d1, a.d2, p, f, x, y, z, i, a.wdir, sum, cnt);
         if (cnt < 0.5f) continue;
         p[i] = mixf_(p[i], (sum - a.hh * b[i]) / cnt, omega);
       }
@@ -520,6 +527,23 @@ int mg_resid_blocks(const MgLevel &L) { return nblocks(L.n); }
 void launch_mg_resid(const MgLevel &L, StatParts *st, hipStream_t s) {
   LvArgs a = lv_args(L);
   k_mg_resid<<<nblocks(L.n), NT, 0, s>>>(a, make_fastdiv((uint32_t)L.d[0]), L.phi, L.b, L.f, st->resid);
+}
+
+// ---------------------------------------------------------------- V-cycle
+void enqueue_vcycle(const MgLevel *lv, int nl, int coarse_iters, hipStream_t s) {
+  auto sweeps = [&](const MgLevel &L, bool fine) {
+    launch_mg_sweep(L, L.phi, L.tmp, 1.0f, s, fine);
+    launch_mg_sweep(L, L.tmp, L.phi, 1.0f, s, fine);
+  };
+  for (int l = 0; l < nl - 1; l++) {
+    sweeps(lv[l], l == 0);
+    launch_mg_restrict(lv[l], lv[l + 1], s);
+  }
+  launch_mg_coarse(lv[nl - 1], coarse_iters, 1.6f, s);
+  for (int l = nl - 2; l >= 0; l--) {
+    launch_mg_prolong(lv[l + 1], lv[l], s);
+    sweeps(lv[l], l == 0);
+  }
 }
 
 }  // namespace pyro

@@ -1,0 +1,169 @@
+// k_solve.hip -- FP64 vector kernels of the pressure solvers (iterative
+// refinement and flexible MGPCG).
+//
+// The operator is the level-0 operator of mg.wgsl:25-35, evaluated in FP64:
+//   (A x)_i = (sum of fluid-neighbour x - cnt_i x_i) / h^2   on fluid cells,
+// with cnt from the level's neighbour codes (sides/top outside count 1, the
+// ground and solids are skipped).  Solid cells carry 0.  Dot products are
+// reduced per block in a fixed order and then by one block, so every run is
+// bit-reproducible.
+#include "kernels.hpp"
+
+namespace pyro {
+
+struct Op64 {
+  int d0, d1, d2;
+  uint32_t n;
+  double ih2;  // 1 / h^2
+  FastDiv dd0, dd1;
+};
+
+Op64 op64_of(const MgLevel &L) {
+  Op64 o;
+  o.d0 = L.d[0]; o.d1 = L.d[1]; o.d2 = L.d[2];
+  o.n = L.n;
+  o.ih2 = 1.0 / ((double)L.h * (double)L.h);
+  o.dd0 = make_fastdiv((uint32_t)L.d[0]);
+  o.dd1 = make_fastdiv((uint32_t)L.d[1]);
+  return o;
+}
+
+__device__ __forceinline__ double apply_at(const Op64 &o, const uint8_t *__restrict__ f, const uint8_t *__restrict__ nc,
+                                           const double *__restrict__ x, uint32_t i) {
+  const uint8_t code = nc[i];
+  if (code & 0x80) return 0.0;
+  uint32_t r = fdiv(i, o.dd0);
+  const int xx = (int)(i - r * (uint32_t)o.d0);
+  const uint32_t zq = fdiv(r, o.dd1);
+  const int yy = (int)(r - zq * (uint32_t)o.d1), zz = (int)zq;
+  const uint32_t sy = (uint32_t)o.d0, sz = (uint32_t)o.d0 * (uint32_t)o.d1;
+  double sum = 0.0;
+  if (xx + 1 < o.d0 && f[i + 1] != 1) sum += x[i + 1];
+  if (xx - 1 >= 0 && f[i - 1] != 1) sum += x[i - 1];
+  if (yy + 1 < o.d1 && f[i + sy] != 1) sum += x[i + sy];
+  if (yy - 1 >= 0 && f[i - sy] != 1) sum += x[i - sy];
+  if (zz + 1 < o.d2 && f[i + sz] != 1) sum += x[i + sz];
+  if (zz - 1 >= 0 && f[i - sz] != 1) sum += x[i - sz];
+  const double cnt = (double)((code & 7u) + ((code >> 3) & 7u));
+  return (sum - cnt * x[i]) * o.ih2;
+}
+
+// r = b - A x  (fluid cells; 0 on solids)
+__global__ __launch_bounds__(NT) void k_resid64(Op64 o, const uint8_t *__restrict__ f, const uint8_t *__restrict__ nc,
+                                                 const double *__restrict__ x, const double *__restrict__ b,
+                                                 double *__restrict__ r) {
+  uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
+  if (i >= o.n) return;
+  r[i] = (nc[i] & 0x80) ? 0.0 : b[i] - apply_at(o, f, nc, x, i);
+}
+// y = A x
+__global__ __launch_bounds__(NT) void k_apply64(Op64 o, const uint8_t *__restrict__ f, const uint8_t *__restrict__ nc,
+                                                 const double *__restrict__ x, double *__restrict__ y) {
+  uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
+  if (i >= o.n) return;
+  y[i] = apply_at(o, f, nc, x, i);
+}
+
+void launch_resid64(const MgLevel &L, const double *x, const double *b, double *r, hipStream_t s) {
+  k_resid64<<<nblocks(L.n), NT, 0, s>>>(op64_of(L), L.f, L.nc, x, b, r);
+}
+void launch_apply64(const MgLevel &L, const double *x, double *y, hipStream_t s) {
+  k_apply64<<<nblocks(L.n), NT, 0, s>>>(op64_of(L), L.f, L.nc, x, y);
+}
+
+// Up to 4 dot products at once: out[k] = sum_i a_k[i] * b_k[i].
+struct DotArgs {
+  const double *a[4];
+  const double *b[4];
+  int k;
+};
+constexpr int DOT_BLOCKS = 1024;
+template <int K>
+__global__ __launch_bounds__(NT) void k_dot_partial(DotArgs d, uint32_t n, double *__restrict__ part) {
+  double acc[K];
+#pragma unroll
+  for (int q = 0; q < K; q++) acc[q] = 0.0;
+  for (uint32_t i = blockIdx.x * NT + threadIdx.x; i < n; i += DOT_BLOCKS * NT) {
+#pragma unroll
+    for (int q = 0; q < K; q++) acc[q] += d.a[q][i] * d.b[q][i];
+  }
+  __shared__ double sh[NT / 64];
+#pragma unroll
+  for (int q = 0; q < K; q++) {
+    double v = block_sum<NT>(acc[q], sh);
+    if (threadIdx.x == 0) part[q * DOT_BLOCKS + blockIdx.x] = v;
+  }
+}
+__global__ __launch_bounds__(1024) void k_dot_final(const double *__restrict__ part, int k, double *__restrict__ out) {
+  __shared__ double sh[1024];
+  for (int q = 0; q < k; q++) {
+    sh[threadIdx.x] = part[q * DOT_BLOCKS + threadIdx.x];
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[q] = sh[0];
+    __syncthreads();
+  }
+}
+void launch_dots(const double *const *a, const double *const *b, int k, uint32_t n, double *part, double *out,
+                 hipStream_t s) {
+  DotArgs d;
+  for (int q = 0; q < 4; q++) {
+    d.a[q] = q < k ? a[q] : nullptr;
+    d.b[q] = q < k ? b[q] : nullptr;
+  }
+  d.k = k;
+  switch (k) {
+    case 1: k_dot_partial<1><<<DOT_BLOCKS, NT, 0, s>>>(d, n, part); break;
+    case 2: k_dot_partial<2><<<DOT_BLOCKS, NT, 0, s>>>(d, n, part); break;
+    case 3: k_dot_partial<3><<<DOT_BLOCKS, NT, 0, s>>>(d, n, part); break;
+    default: k_dot_partial<4><<<DOT_BLOCKS, NT, 0, s>>>(d, n, part); break;
+  }
+  k_dot_final<<<1, 1024, 0, s>>>(part, k, out);
+}
+
+// y += alpha * x
+__global__ __launch_bounds__(NT) void k_axpy64(uint32_t n, double alpha, const double *__restrict__ x,
+                                                double *__restrict__ y) {
+  uint32_t i = blockIdx.x * NT + threadIdx.x;
+  if (i < n) y[i] += alpha * x[i];
+}
+void launch_axpy64(uint32_t n, double alpha, const double *x, double *y, hipStream_t s) {
+  k_axpy64<<<nblocks(n), NT, 0, s>>>(n, alpha, x, y);
+}
+// p = z + beta * p
+__global__ __launch_bounds__(NT) void k_xpby64(uint32_t n, const double *__restrict__ z, double beta,
+                                                double *__restrict__ p) {
+  uint32_t i = blockIdx.x * NT + threadIdx.x;
+  if (i < n) p[i] = z[i] + beta * p[i];
+}
+void launch_xpby64(uint32_t n, const double *z, double beta, double *p, hipStream_t s) {
+  k_xpby64<<<nblocks(n), NT, 0, s>>>(n, z, beta, p);
+}
+__global__ __launch_bounds__(NT) void k_to_f32(uint32_t n, const double *__restrict__ a, float *__restrict__ b) {
+  uint32_t i = blockIdx.x * NT + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+__global__ __launch_bounds__(NT) void k_to_f64(uint32_t n, const float *__restrict__ a, double *__restrict__ b) {
+  uint32_t i = blockIdx.x * NT + threadIdx.x;
+  if (i < n) b[i] = (double)a[i];
+}
+__global__ __launch_bounds__(NT) void k_add_f32_to_f64(uint32_t n, const float *__restrict__ a, double *__restrict__ b) {
+  uint32_t i = blockIdx.x * NT + threadIdx.x;
+  if (i < n) b[i] += (double)a[i];
+}
+void launch_to_f32(uint32_t n, const double *a, float *b, hipStream_t s) { k_to_f32<<<nblocks(n), NT, 0, s>>>(n, a, b); }
+void launch_to_f64(uint32_t n, const float *a, double *b, hipStream_t s) { k_to_f64<<<nblocks(n), NT, 0, s>>>(n, a, b); }
+void launch_add_f32_to_f64(uint32_t n, const float *a, double *b, hipStream_t s) {
+  k_add_f32_to_f64<<<nblocks(n), NT, 0, s>>>(n, a, b);
+}
+// zero fill (a kernel rather than a memset node, so it is graph-safe)
+__global__ __launch_bounds__(NT) void k_zero32(uint32_t n, float *__restrict__ a) {
+  uint32_t i = blockIdx.x * NT + threadIdx.x;
+  if (i < n) a[i] = 0.0f;
+}
+void launch_zero32(uint32_t n, float *a, hipStream_t s) { k_zero32<<<nblocks(n), NT, 0, s>>>(n, a); }
+
+}  // namespace pyro

@@ -31,6 +31,8 @@ constexpr int NT = 256;  // threads per block of the element-wise kernels
 
 void launch_solid(const Fields &f, StatParts *st, hipStream_t s);
 void launch_curl(const Fields &f, StatParts *st, hipStream_t s);
+// flags[i] = 1 where sol[i] is solid (at creation and after SOL is written).
+void launch_init_flags(const Params *P, const Geo &g, const float4 *sol, uint8_t *flags, hipStream_t s);
 void launch_adv_vel1(const Fields &f, hipStream_t s);
 void launch_adv_vel2(const Fields &f, hipStream_t s);
 void launch_adv_s1(const Fields &f, hipStream_t s);
@@ -47,6 +49,7 @@ void launch_set_step(Params *P, float dt, float time, float ign_on, uint32_t fra
 struct MgLevel {
   int d[3];
   float h, hh;   // spacing and h*h (f32)
+  float wdir = 1.0f;  // weight of a neighbour outside the box (1 = mg.wgsl)
   float *phi, *b;
   float *tmp;    // second phi buffer of the ping-pong sweeps
   uint8_t *f;    // 1 = solid
@@ -66,6 +69,23 @@ void launch_mg_resid(const MgLevel &L, StatParts *st, hipStream_t s);
 int mg_resid_blocks(const MgLevel &L);
 
 void launch_copy(const float4 *src, float4 *dst, uint32_t n4, hipStream_t s);
+// V-cycle of sim.rs:451-491 over an arbitrary hierarchy (2 pre- and 2
+// post-sweeps per level, `coarse_iters` SOR iterations at omega = 1.6).
+void enqueue_vcycle(const MgLevel *lv, int nl, int coarse_iters, hipStream_t s);
+
+// FP64 solver kernels (k_solve.hip).
+void launch_resid64(const MgLevel &L, const double *x, const double *b, double *r, hipStream_t s);
+void launch_apply64(const MgLevel &L, const double *x, double *y, hipStream_t s);
+void launch_dots(const double *const *a, const double *const *b, int k, uint32_t n, double *part, double *out,
+                 hipStream_t s);
+constexpr int DOT_PART = 4 * 1024;  // doubles of scratch for launch_dots
+void launch_axpy64(uint32_t n, double alpha, const double *x, double *y, hipStream_t s);
+void launch_xpby64(uint32_t n, const double *z, double beta, double *p, hipStream_t s);
+void launch_to_f32(uint32_t n, const double *a, float *b, hipStream_t s);
+void launch_to_f64(uint32_t n, const float *a, double *b, hipStream_t s);
+void launch_add_f32_to_f64(uint32_t n, const float *a, double *b, hipStream_t s);
+void launch_zero32(uint32_t n, float *a, hipStream_t s);
+
 // Debug / test: apply device math to an array (0 exp, 1 sqrt, 2 div by in2).
 void launch_debug_math(int fn, const float *in, const float *in2, float *out, uint32_t n, hipStream_t s);
 

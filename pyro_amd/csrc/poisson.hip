@@ -1,0 +1,296 @@
+// poisson.hip -- stand-alone pressure solves on one grid (C ABI
+// pyro_poisson_*): the reference V-cycles, FP64-residual iterative
+// refinement around FP32 V-cycles, and flexible MGPCG with FP64 vectors and
+// dot products.
+//
+// Operator: level 0 of mg.wgsl (ground Neumann, phi = 0 outside the sides
+// and top, solids Neumann).  Two hierarchies share the buffers:
+//  - `ref`: the reference's (sim.rs:142-168: halve until 4 levels or a
+//    dimension below 16), outside weight 1 -- the plain method, bit-equal to
+//    Sim::vcycles;
+//  - `pc`: the preconditioner's, halved down to a few hundred cells, with the
+//    outside weight of level l set to 2*2^l/(2^l+1) so that the Dirichlet
+//    point phi = 0 sits at the same distance beyond the wall on every level
+//    (on level 0 it is half a fine cell beyond the wall).  The reference
+//    weight 1 moves that point outward on each coarser level, which is what
+//    slows its V-cycle to ~0.55 per cycle at 32^3 and stalls it at 64^3.
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "pyro_hip.h"
+
+using namespace pyro;
+
+namespace {
+thread_local std::string g_perr;
+int pfail(int code, const std::string &m) {
+  g_perr = m;
+  return code;
+}
+#define PCHK(x)                                                                                 \
+  do {                                                                                          \
+    hipError_t e_ = (x);                                                                        \
+    if (e_ != hipSuccess) return pfail(PYRO_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+}  // namespace
+
+struct PyroPoisson {
+  int d[3] = {0, 0, 0};
+  float h = 0.0f;
+  uint32_t n = 0;
+  MgLevel pc[12];
+  int nl_pc = 0;
+  MgLevel ref[4];
+  int nl_ref = 0;
+  double *x = nullptr, *b = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *z = nullptr;
+  double *part = nullptr, *dots = nullptr;
+  hipStream_t stream = nullptr;
+  std::vector<void *> allocs;
+  int coarse_iters_pc = 80;
+};
+
+namespace {
+template <class T>
+int palloc(PyroPoisson *P, T **ptr, size_t count) {
+  void *v = nullptr;
+  size_t bytes = count * sizeof(T);
+  if (hipMalloc(&v, bytes < 16 ? 16 : bytes) != hipSuccess) return pfail(PYRO_E_OOM, "hipMalloc failed");
+  P->allocs.push_back(v);
+  if (hipMemset(v, 0, bytes < 16 ? 16 : bytes) != hipSuccess) return pfail(PYRO_E_HIP, "hipMemset failed");
+  *ptr = static_cast<T *>(v);
+  return 0;
+}
+
+int sync(PyroPoisson *P) {
+  PCHK(hipStreamSynchronize(P->stream));
+  PCHK(hipGetLastError());
+  return 0;
+}
+
+// dots[0..k) of pairs (a_i, b_i), read back to the host
+int dots(PyroPoisson *P, std::initializer_list<std::pair<const double *, const double *>> pairs, double *out) {
+  const double *a[4], *b[4];
+  int k = 0;
+  for (auto &pr : pairs) { a[k] = pr.first; b[k] = pr.second; k++; }
+  launch_dots(a, b, k, P->n, P->part, P->dots, P->stream);
+  PCHK(hipMemcpyAsync(out, P->dots, k * sizeof(double), hipMemcpyDeviceToHost, P->stream));
+  return sync(P);
+}
+
+// z = M(r): one preconditioner V-cycle on FP32 copies, from a zero guess.
+void precond(PyroPoisson *P, const double *r, double *z) {
+  MgLevel &L0 = P->pc[0];
+  launch_to_f32(P->n, r, L0.b, P->stream);
+  launch_zero32(P->n, L0.phi, P->stream);
+  enqueue_vcycle(P->pc, P->nl_pc, P->coarse_iters_pc, P->stream);
+  launch_to_f64(P->n, L0.phi, z, P->stream);
+}
+
+int rel_resid(PyroPoisson *P, double *rel, double *bb_out) {
+  launch_resid64(P->pc[0], P->x, P->b, P->r, P->stream);
+  double d[2];
+  int rc = dots(P, {{P->r, P->r}, {P->b, P->b}}, d);
+  if (rc) return rc;
+  *rel = d[1] > 0.0 ? std::sqrt(d[0] / d[1]) : std::sqrt(d[0]);
+  if (bb_out) *bb_out = d[1];
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+const char *pyro_poisson_last_error(void) { return g_perr.c_str(); }
+
+int pyro_poisson_create(uint32_t nx, uint32_t ny, uint32_t nz, float h, const uint32_t *flags, PyroPoisson **out) {
+  if (!flags || !out || nx < 2 || ny < 2 || nz < 2 || !(h > 0.0f)) return pfail(PYRO_E_INVALID, "bad argument");
+  if ((uint64_t)nx * ny * nz >= (1ull << 31)) return pfail(PYRO_E_INVALID, "grid too large");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return pfail(PYRO_E_NODEVICE, "no HIP device available");
+  PyroPoisson *P = new PyroPoisson();
+  auto bail = [&](int rc) { pyro_poisson_destroy(P); return rc; };
+  P->d[0] = (int)nx; P->d[1] = (int)ny; P->d[2] = (int)nz;
+  P->h = h;
+  P->n = nx * ny * nz;
+  if (hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(pfail(PYRO_E_HIP, "hipStreamCreate failed"));
+  int d[3] = {(int)nx, (int)ny, (int)nz};
+  float hl = h;
+  int rc = 0;
+  bool ref_open = true;
+  for (int l = 0; l < 12 && !rc; l++) {
+    MgLevel &L = P->pc[P->nl_pc++];
+    L.d[0] = d[0]; L.d[1] = d[1]; L.d[2] = d[2];
+    L.h = hl;
+    L.hh = hl * hl;
+    L.n = (uint32_t)d[0] * (uint32_t)d[1] * (uint32_t)d[2];
+    L.wdir = l == 0 ? 1.0f : (float)(2.0 * std::ldexp(1.0, l) / (std::ldexp(1.0, l) + 1.0));
+    rc |= palloc(P, &L.phi, L.n);
+    if (!rc) rc |= palloc(P, &L.b, L.n);
+    if (!rc) rc |= palloc(P, &L.tmp, L.n);
+    if (!rc) rc |= palloc(P, &L.f, L.n);
+    if (!rc) rc |= palloc(P, &L.nc, L.n);
+    // reference hierarchy: the first levels of the same halving, a level is
+    // kept and the halving stops after 4 levels or once min(dims) < 16
+    // (sim.rs:146-168).  It never outlasts `pc`, which stops only below
+    // 512 cells or min(dims) < 4.
+    int mn = d[0] < d[1] ? d[0] : d[1];
+    mn = d[2] < mn ? d[2] : mn;
+    if (ref_open) {
+      P->ref[P->nl_ref] = L;
+      P->ref[P->nl_ref].wdir = 1.0f;
+      P->nl_ref++;
+      ref_open = P->nl_ref < 4 && mn >= 16;
+    }
+    if (L.n <= 512 || mn < 4) break;
+    for (int qd = 0; qd < 3; qd++) d[qd] = (d[qd] + 1) / 2;
+    hl *= 2.0f;
+  }
+  if (!rc) rc |= palloc(P, &P->x, P->n);
+  if (!rc) rc |= palloc(P, &P->b, P->n);
+  if (!rc) rc |= palloc(P, &P->r, P->n);
+  if (!rc) rc |= palloc(P, &P->p, P->n);
+  if (!rc) rc |= palloc(P, &P->q, P->n);
+  if (!rc) rc |= palloc(P, &P->z, P->n);
+  if (!rc) rc |= palloc(P, &P->part, DOT_PART);
+  if (!rc) rc |= palloc(P, &P->dots, 4);
+  if (rc) return bail(rc < 0 ? rc : PYRO_E_OOM);
+  std::vector<uint8_t> f8(P->n);
+  for (uint32_t i = 0; i < P->n; i++) f8[i] = flags[i] == 1u ? 1 : 0;
+  if (hipMemcpy(P->pc[0].f, f8.data(), P->n, hipMemcpyHostToDevice) != hipSuccess)
+    return bail(pfail(PYRO_E_HIP, "upload failed"));
+  launch_mg_setup(P->pc, P->nl_pc, P->stream);
+  if (sync(P)) return bail(PYRO_E_HIP);
+  *out = P;
+  return 0;
+}
+
+int pyro_poisson_destroy(PyroPoisson *P) {
+  if (!P) return 0;
+  if (P->stream) (void)hipStreamSynchronize(P->stream);
+  for (void *v : P->allocs) (void)hipFree(v);
+  if (P->stream) (void)hipStreamDestroy(P->stream);
+  delete P;
+  return 0;
+}
+
+int pyro_poisson_levels(const PyroPoisson *P, int *ref_levels, int *pc_levels) {
+  if (!P) return pfail(PYRO_E_INVALID, "null handle");
+  if (ref_levels) *ref_levels = P->nl_ref;
+  if (pc_levels) *pc_levels = P->nl_pc;
+  return 0;
+}
+
+int pyro_poisson_set(PyroPoisson *P, const double *b, const double *phi) {
+  if (!P) return pfail(PYRO_E_INVALID, "null handle");
+  if (b) PCHK(hipMemcpy(P->b, b, P->n * sizeof(double), hipMemcpyHostToDevice));
+  if (phi) PCHK(hipMemcpy(P->x, phi, P->n * sizeof(double), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int pyro_poisson_get(PyroPoisson *P, double *phi) {
+  if (!P || !phi) return pfail(PYRO_E_INVALID, "null argument");
+  int rc = sync(P);
+  if (rc) return rc;
+  PCHK(hipMemcpy(phi, P->x, P->n * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pyro_poisson_residual(PyroPoisson *P, double *rel) {
+  if (!P || !rel) return pfail(PYRO_E_INVALID, "null argument");
+  return rel_resid(P, rel, nullptr);
+}
+
+int pyro_poisson_solve(PyroPoisson *P, int method, double tol, int max_iters, PyroSolveInfo *info) {
+  if (!P || max_iters < 1) return pfail(PYRO_E_INVALID, "bad argument");
+  if (method < PYRO_SOLVER_VCYCLE || method > PYRO_SOLVER_MGPCG) return pfail(PYRO_E_INVALID, "unknown method");
+  hipEvent_t e0, e1;
+  PCHK(hipEventCreate(&e0));
+  PCHK(hipEventCreate(&e1));
+  PCHK(hipEventRecord(e0, P->stream));
+  int it = 0, cycles = 0, rc = 0;
+  double rel = 0.0;
+  const uint32_t n = P->n;
+  if (method == PYRO_SOLVER_VCYCLE) {
+    // the reference's V-cycles on the FP32 level-0 state, FP64 residual after each
+    MgLevel &L0 = P->ref[0];
+    launch_to_f32(n, P->x, L0.phi, P->stream);
+    launch_to_f32(n, P->b, L0.b, P->stream);
+    rc = rel_resid(P, &rel, nullptr);
+    while (!rc && rel > tol && it < max_iters) {
+      enqueue_vcycle(P->ref, P->nl_ref, 80, P->stream);
+      launch_to_f64(n, L0.phi, P->x, P->stream);
+      cycles++;
+      it++;
+      rc = rel_resid(P, &rel, nullptr);
+    }
+  } else if (method == PYRO_SOLVER_REFINE) {
+    // x += V(b - A x) with the residual in FP64
+    rc = rel_resid(P, &rel, nullptr);
+    while (!rc && rel > tol && it < max_iters) {
+      MgLevel &L0 = P->pc[0];
+      launch_to_f32(n, P->r, L0.b, P->stream);
+      launch_zero32(n, L0.phi, P->stream);
+      enqueue_vcycle(P->pc, P->nl_pc, P->coarse_iters_pc, P->stream);
+      launch_add_f32_to_f64(n, L0.phi, P->x, P->stream);
+      cycles++;
+      it++;
+      rc = rel_resid(P, &rel, nullptr);
+    }
+  } else {
+    // flexible CG (Notay's FCG(1)): p_{k+1} = z_{k+1} - (z_{k+1}.q_k / p_k.q_k) p_k
+    rc = rel_resid(P, &rel, nullptr);  // r = b - A x
+    if (!rc && rel > tol) {
+      precond(P, P->r, P->z);
+      cycles++;
+      PCHK(hipMemcpyAsync(P->p, P->z, n * sizeof(double), hipMemcpyDeviceToDevice, P->stream));
+      double bb;
+      {
+        double d1[1];
+        rc = dots(P, {{P->b, P->b}}, d1);
+        bb = d1[0];
+      }
+      while (!rc && it < max_iters) {
+        launch_apply64(P->pc[0], P->p, P->q, P->stream);
+        double d2[2];
+        rc = dots(P, {{P->p, P->q}, {P->p, P->r}}, d2);
+        if (rc) break;
+        if (d2[0] == 0.0) break;
+        const double alpha = d2[1] / d2[0];
+        launch_axpy64(n, alpha, P->p, P->x, P->stream);
+        launch_axpy64(n, -alpha, P->q, P->r, P->stream);
+        it++;
+        double d3[1];
+        rc = dots(P, {{P->r, P->r}}, d3);
+        if (rc) break;
+        rel = bb > 0.0 ? std::sqrt(d3[0] / bb) : std::sqrt(d3[0]);
+        if (rel <= tol) break;
+        precond(P, P->r, P->z);
+        cycles++;
+        double d4[1];
+        rc = dots(P, {{P->z, P->q}}, d4);
+        if (rc) break;
+        launch_xpby64(n, P->z, -d4[0] / d2[0], P->p, P->stream);
+      }
+      if (!rc) rc = rel_resid(P, &rel, nullptr);  // true residual of the final x
+    }
+  }
+  PCHK(hipEventRecord(e1, P->stream));
+  PCHK(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  PCHK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  if (info) {
+    info->iterations = it;
+    info->vcycles = cycles;
+    info->rel_resid = rel;
+    info->ms = ms;
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -90,30 +90,10 @@ Fields fields_of(PyroSim *s) {
   return f;
 }
 
-// `iters` red-black sweeps (sim.rs:425-449); an even count leaves phi in
-// L.phi, the odd sweeps land in L.tmp.
-void enqueue_smooth(const MgLevel &L, int iters, float omega, hipStream_t st, bool fine) {
-  for (int it = 0; it < iters; it++) {
-    if (it & 1) launch_mg_sweep(L, L.tmp, L.phi, omega, st, fine);
-    else launch_mg_sweep(L, L.phi, L.tmp, omega, st, fine);
-  }
-}
-
 // sim.rs:451-491
 void enqueue_vcycles(PyroSim *s, hipStream_t st) {
-  const int nl = s->nl;
-  launch_mg_setup(s->lv, nl, st);
-  for (uint32_t c = 0; c < s->cycles; c++) {
-    for (int l = 0; l < nl - 1; l++) {
-      enqueue_smooth(s->lv[l], 2, 1.0f, st, l == 0);
-      launch_mg_restrict(s->lv[l], s->lv[l + 1], st);
-    }
-    launch_mg_coarse(s->lv[nl - 1], COARSE_ITERS, 1.6f, st);
-    for (int l = nl - 2; l >= 0; l--) {
-      launch_mg_prolong(s->lv[l + 1], s->lv[l], st);
-      enqueue_smooth(s->lv[l], 2, 1.0f, st, l == 0);
-    }
-  }
+  launch_mg_setup(s->lv, s->nl, st);
+  for (uint32_t c = 0; c < s->cycles; c++) enqueue_vcycle(s->lv, s->nl, COARSE_ITERS, st);
 }
 
 void enqueue_stage(PyroSim *s, int stage, bool stats, hipStream_t st) {
@@ -319,6 +299,8 @@ int pyro_sim_create(const void *params, size_t params_bytes, const float *sol_in
       hipMemcpy(s->sol[0], sol_init, (size_t)g.n * 16, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(s->sol[1], sol_init, (size_t)g.n * 16, hipMemcpyHostToDevice) != hipSuccess)
     return bail(fail(PYRO_E_HIP, "initial upload failed"));
+  launch_init_flags(s->dP, s->g, s->sol[0], s->flags, s->stream);
+  if (sync_check(s)) return bail(PYRO_E_HIP);
   *out = s;
   return 0;
 }
@@ -416,7 +398,7 @@ int pyro_sim_read_field(PyroSim *s, int field, void *dst, size_t bytes) {
     std::vector<uint8_t> tmp(r.dev_bytes);
     HIPCHK(hipMemcpy(tmp.data(), r.ptr, r.dev_bytes, hipMemcpyDeviceToHost));
     uint32_t *o = static_cast<uint32_t *>(dst);
-    for (size_t i = 0; i < r.dev_bytes; i++) o[i] = tmp[i];
+    for (size_t i = 0; i < r.dev_bytes; i++) o[i] = tmp[i] == 1 ? 1u : 0u;
   } else {
     HIPCHK(hipMemcpy(dst, r.ptr, bytes, hipMemcpyDeviceToHost));
   }
@@ -437,6 +419,13 @@ int pyro_sim_write_field(PyroSim *s, int field, const void *src, size_t bytes) {
     HIPCHK(hipMemcpy(r.ptr, tmp.data(), r.dev_bytes, hipMemcpyHostToDevice));
   } else {
     HIPCHK(hipMemcpy(r.ptr, src, bytes, hipMemcpyHostToDevice));
+  }
+  if (field == PYRO_FIELD_SOL || field == PYRO_FIELD_SOL_OUT) {
+    // a new SOL state: both ping-pong buffers and the solid flags follow it
+    void *other = field == PYRO_FIELD_SOL ? (void *)s->sol[1 - s->sol_cur] : (void *)s->sol[s->sol_cur];
+    HIPCHK(hipMemcpy(other, src, bytes, hipMemcpyHostToDevice));
+    launch_init_flags(s->dP, s->g, static_cast<const float4 *>(r.ptr), s->flags, s->stream);
+    return sync_check(s);
   }
   return 0;
 }

@@ -1,0 +1,166 @@
+"""Stand-alone pressure solves on the GPU (include/pyro_hip.h, pyro_poisson_*).
+
+Methods: ``vcycle`` (the reference V-cycles, sim.rs:451-491), ``refine``
+(FP64-residual iterative refinement around FP32 V-cycles) and ``mgpcg``
+(flexible CG with FP64 vectors and dot products, preconditioned by an FP32
+V-cycle with level-consistent Dirichlet weights).
+
+``make_c3_problem`` builds the standalone pressure benchmark case: a cubic
+grid with seeded random axis-aligned solid boxes (~5 % of the cells) and a
+right-hand side of 32 Gaussian blobs (sigma 4-16 cells, amplitude
+U[0.1, 5] 1/s), zero on solids and made zero-mean on every fluid component
+that has no open (Dirichlet) face, where the Neumann problem would otherwise
+have no solution.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native
+
+METHODS = {"vcycle": 0, "refine": 1, "mgpcg": 2}
+
+
+class _Info(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("vcycles", C.c_int), ("rel_resid", C.c_double), ("ms", C.c_float)]
+
+
+@dataclass
+class SolveInfo:
+    iterations: int
+    vcycles: int
+    rel_resid: float
+    ms: float
+
+
+def _lib():
+    lib = _native.hip()
+    if not hasattr(lib, "_poisson_ready"):
+        P = C.c_void_p
+        lib.pyro_poisson_last_error.restype = C.c_char_p
+        lib.pyro_poisson_create.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, P, C.POINTER(P)]
+        lib.pyro_poisson_destroy.argtypes = [P]
+        lib.pyro_poisson_levels.argtypes = [P, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        lib.pyro_poisson_set.argtypes = [P, P, P]
+        lib.pyro_poisson_get.argtypes = [P, P]
+        lib.pyro_poisson_residual.argtypes = [P, C.POINTER(C.c_double)]
+        lib.pyro_poisson_solve.argtypes = [P, C.c_int, C.c_double, C.c_int, C.POINTER(_Info)]
+        lib._poisson_ready = True
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise _native.PyroError(rc, _lib().pyro_poisson_last_error().decode(errors="replace"))
+
+
+class Poisson:
+    """A pressure problem on an (nz, ny, nx) grid with solid flags (1 = solid)."""
+
+    def __init__(self, flags: np.ndarray, h: float):
+        f = np.ascontiguousarray(flags, dtype=np.uint32)
+        nz, ny, nx = f.shape
+        self.shape = f.shape
+        self.h = float(h)
+        h_ = C.c_void_p()
+        _check(_lib().pyro_poisson_create(nx, ny, nz, float(h), f.ctypes.data, C.byref(h_)))
+        self._h = h_
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib().pyro_poisson_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def levels(self):
+        a, b = C.c_int(), C.c_int()
+        _check(_lib().pyro_poisson_levels(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def set(self, b: np.ndarray | None = None, phi: np.ndarray | None = None) -> None:
+        bp = pp = None
+        if b is not None:
+            bb = np.ascontiguousarray(b, dtype=np.float64)
+            assert bb.shape == self.shape
+            bp = bb.ctypes.data
+        if phi is not None:
+            ph = np.ascontiguousarray(phi, dtype=np.float64)
+            assert ph.shape == self.shape
+            pp = ph.ctypes.data
+        _check(_lib().pyro_poisson_set(self._h, bp, pp))
+
+    def get(self) -> np.ndarray:
+        out = np.zeros(self.shape, np.float64)
+        _check(_lib().pyro_poisson_get(self._h, out.ctypes.data))
+        return out
+
+    def residual(self) -> float:
+        r = C.c_double()
+        _check(_lib().pyro_poisson_residual(self._h, C.byref(r)))
+        return r.value
+
+    def solve(self, method: str = "mgpcg", tol: float = 1e-6, max_iters: int = 200) -> SolveInfo:
+        info = _Info()
+        _check(_lib().pyro_poisson_solve(self._h, METHODS[method], float(tol), int(max_iters), C.byref(info)))
+        return SolveInfo(info.iterations, info.vcycles, info.rel_resid, info.ms)
+
+
+def fluid_components(flags: np.ndarray):
+    """Face-connected fluid components and whether each touches an open face.
+
+    Open (Dirichlet, phi = 0) faces are the sides and the top of the box
+    (mg.wgsl:29); the ground (y = 0 face) and solids are Neumann.
+    Returns (labels, n, open_mask[n+1]).
+    """
+    from scipy import ndimage
+
+    fluid = np.asarray(flags) == 0
+    lab, n = ndimage.label(fluid)
+    touch = np.zeros(n + 1, bool)
+    for sl in (lab[0, :, :], lab[-1, :, :], lab[:, -1, :], lab[:, :, 0], lab[:, :, -1]):
+        touch[np.unique(sl)] = True
+    touch[0] = False
+    return lab, n, touch
+
+
+def make_c3_problem(n: int = 256, seed: int = 0, h: float = 1.0 / 256, solid_frac: float = 0.05,
+                    blobs: int = 32):
+    """(flags, b) of the standalone pressure case (see the module docstring)."""
+    rng = np.random.default_rng(seed)
+    nz = ny = nx = n
+    flags = np.zeros((nz, ny, nx), np.uint32)
+    target = solid_frac * flags.size
+    while flags.sum() < target:
+        sz = rng.integers(max(2, n // 64), max(3, n // 10), size=3)
+        lo = [rng.integers(0, n - s) for s in sz]
+        flags[lo[0]:lo[0] + sz[0], lo[1]:lo[1] + sz[1], lo[2]:lo[2] + sz[2]] = 1
+    b = np.zeros((nz, ny, nx), np.float64)
+    zz = np.arange(nz)[:, None, None]
+    yy = np.arange(ny)[None, :, None]
+    xx = np.arange(nx)[None, None, :]
+    for _ in range(blobs):
+        c = rng.uniform(0, n, 3)
+        sig = rng.uniform(4.0, 16.0) * n / 256.0
+        amp = rng.uniform(0.1, 5.0) * rng.choice([-1.0, 1.0])
+        gz = np.exp(-((zz - c[0]) ** 2) / (2 * sig * sig))
+        gy = np.exp(-((yy - c[1]) ** 2) / (2 * sig * sig))
+        gx = np.exp(-((xx - c[2]) ** 2) / (2 * sig * sig))
+        b += amp * gz * gy * gx
+    b[flags == 1] = 0.0
+    lab, ncomp, open_ = fluid_components(flags)
+    if ncomp:
+        sums = np.bincount(lab.ravel(), weights=b.ravel(), minlength=ncomp + 1)
+        cnts = np.bincount(lab.ravel(), minlength=ncomp + 1)
+        mean = np.where(open_ | (cnts == 0), 0.0, sums / np.maximum(cnts, 1))
+        mean[0] = 0.0
+        b -= mean[lab]
+    return flags, b
