@@ -21,6 +21,13 @@ Extra fields:
   cpu_baseline  -- the CPU restatement in oracle/ (kind "port") timed on this
                    host on a bounded sample of the same workload.
   stage_ms      -- per-stage time of one step (HIP events, direct launches).
+  pressure      -- time to a relative residual of 1e-4 (sqrt of sum r^2 over
+                   sum b^2 on fluid cells, FP64) on the next step's pressure
+                   problem of this run, warm-started as the step would be,
+                   by the reference V-cycles and by MGPCG; and the standalone
+                   256^3 case (random solid boxes, 32 Gaussian sources):
+                   V-cycles to 1e-4, FP64-residual refinement and MGPCG to
+                   1e-6, with the error against an FP64 solution.
 """
 from __future__ import annotations
 
@@ -30,6 +37,8 @@ import os
 import sys
 import time
 from pathlib import Path
+
+import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
@@ -52,8 +61,6 @@ def make_pool(cell: float):
 
 
 def gpu_run(args):
-    import numpy as np  # noqa: F401
-
     import pyro_amd as pa
 
     scene = make_pool(args.cell)
@@ -98,9 +105,86 @@ def gpu_run(args):
     copy_ms = pa.sim.debug_copy(64 << 20, 20)
     copy_gbs = 2 * (64 << 20) * 4 / (copy_ms * 1e-3) / 1e9
     log(f"[bench] plain copy: {copy_gbs:.0f} GB/s")
+    frame = None if args.no_pressure else pressure_frame(sim, args, p["dx"])
     return dict(cells=cells, wall=wall, stats=st, stage_ms=stage_ms, smooth_ms=smooth_ms,
                 coarse_ms=coarse_ms, achieved=achieved, algo_bytes=algo_bytes, dims=(nx, ny, nz),
-                copy_gbs=copy_gbs)
+                copy_gbs=copy_gbs, frame=frame)
+
+
+def _info(i):
+    return {"ms": round(i.ms, 3), "vcycles": i.vcycles, "iterations": i.iterations, "rel_resid": i.rel_resid}
+
+
+def pressure_frame(sim, args, dx):
+    """Time to a 1e-4 relative residual on the pressure problem of the next
+    step of the benchmark run: its right-hand side, solid flags and the warm
+    start the step itself would use (phi of the previous step)."""
+    import pyro_amd as pa
+    from pyro_amd.poisson import Poisson
+
+    sim.begin_step(args.dt, False)
+    for stg in pa.STAGES[: pa.STAGES.index("divergence") + 1]:
+        sim.run_stage(stg)
+    b = sim.read_field(pa.sim.level_field(0, "b")).astype(np.float64)
+    phi = sim.read_field(pa.sim.level_field(0, "phi")).astype(np.float64)
+    flags = sim.read_field(pa.sim.level_field(0, "flags"))
+    for stg in pa.STAGES[pa.STAGES.index("divergence") + 1:]:
+        sim.run_stage(stg)
+    sim.end_step()
+    P = Poisson(flags, dx)
+    out = {"tol": 1e-4, "start": "warm (phi of the previous step)"}
+    for m, cap in (("vcycle", 400), ("mgpcg", 100)):
+        P.set(b=b, phi=phi)
+        P.solve(m, 1e-4, cap)  # first call: allocation / code-object warm-up
+        P.set(b=b, phi=phi)
+        out[m] = _info(P.solve(m, 1e-4, cap))
+        log(f"[bench] frame pressure {m}: {out[m]}")
+    P.set(b=b, phi=phi)
+    out["initial_rel_resid"] = P.residual()
+    P.close()
+    return out
+
+
+def pressure_c3(args):
+    """The standalone 256^3 pressure case: plain V-cycles to 1e-4, FP32
+    V-cycles in FP64-residual refinement and MGPCG to 1e-6; error of each
+    against an FP64 solution (MGPCG to 1e-12 on the GPU, and the oracle's
+    FP64 Jacobi-PCG on the host with --c3-cpu)."""
+    from pyro_amd.poisson import Poisson, make_c3_problem
+
+    n = args.c3_n
+    t0 = time.perf_counter()
+    flags, b = make_c3_problem(n, seed=0, h=1.0 / n)
+    log(f"[bench] c3 problem {n}^3 built in {time.perf_counter() - t0:.1f} s")
+    fluid = flags == 0
+    P = Poisson(flags, 1.0 / n)
+    zero = np.zeros_like(b)
+    P.set(b=b, phi=zero)
+    P.solve("mgpcg", 1e-12, 400)
+    phi64 = P.get()
+    out = {"grid": [n, n, n], "solid_frac": float(flags.mean()), "levels_ref_pc": list(P.levels)}
+    for m, tol, cap in (("vcycle", 1e-4, 400), ("refine", 1e-4, 200), ("mgpcg", 1e-4, 200),
+                         ("refine", 1e-6, 200), ("mgpcg", 1e-6, 200)):
+        P.set(b=b, phi=zero)
+        info = P.solve(m, tol, cap)
+        d = _info(info)
+        d["err_vs_fp64"] = float(np.linalg.norm((P.get() - phi64)[fluid]) / np.linalg.norm(phi64[fluid]))
+        out[f"{m}_{tol:g}"] = d
+        log(f"[bench] c3 {m} to {tol:g}: {d}")
+    if args.c3_cpu:
+        import oracle
+
+        oracle.set_threads(min(16, os.cpu_count() or 1))
+        t0 = time.perf_counter()
+        ref, its, rel = oracle.poisson_cg64(flags, b, 1.0 / n, tol=1e-10, max_iter=20000)
+        P.set(b=b, phi=zero)
+        P.solve("refine", 1e-6, 200)
+        err = float(np.linalg.norm((P.get() - ref)[fluid]) / np.linalg.norm(ref[fluid]))
+        out["cpu_fp64_oracle"] = {"iterations": its, "rel_resid": rel, "seconds": time.perf_counter() - t0,
+                                  "refine_1e-6_err": err}
+        log(f"[bench] c3 vs CPU FP64 oracle: {out['cpu_fp64_oracle']}")
+    P.close()
+    return out
 
 
 def cpu_baseline(args, dims):
@@ -137,6 +221,9 @@ def main():
     ap.add_argument("--dt", type=float, default=0.002)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-pressure", action="store_true", help="skip the pressure-solver legs")
+    ap.add_argument("--c3-n", type=int, default=256)
+    ap.add_argument("--c3-cpu", action="store_true", help="also solve C3 with the FP64 CPU oracle (~1 min)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -145,6 +232,7 @@ def main():
     if int(os.environ.get("RANK", "0")) != 0:
         sys.exit(0)
     r = gpu_run(args)
+    c3 = None if args.no_pressure else pressure_c3(args)
     cpu = None if args.no_cpu else cpu_baseline(args, r["dims"])
     nx, ny, nz = r["dims"]
     value = r["cells"] * args.steps / r["wall"]
@@ -189,6 +277,7 @@ def main():
         "cpu_baseline": cpu,
         "stage_ms": r["stage_ms"],
         "coarse_solve_ms": r["coarse_ms"],
+        "pressure": {"pool_frame": r["frame"], "standalone": c3},
         "copy_kernel_GBs": r["copy_gbs"],
         "last_stats": {"hrr_W": r["stats"].hrr, "max_T": r["stats"].max_t, "resid": r["stats"].resid},
     }
