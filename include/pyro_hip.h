@@ -135,13 +135,16 @@ typedef struct PyroPoisson PyroPoisson;
 enum PyroSolver {
   PYRO_SOLVER_VCYCLE = 0, /* the reference V-cycles (sim.rs:451-491) on FP32 */
   PYRO_SOLVER_REFINE = 1, /* FP64-residual iterative refinement, FP32 V-cycle corrections */
-  PYRO_SOLVER_MGPCG = 2   /* flexible CG, FP64 vectors/dots, FP32 V-cycle preconditioner */
+  PYRO_SOLVER_MGPCG = 2,  /* flexible CG, FP64 vectors/dots, FP32 V-cycle preconditioner */
+  PYRO_SOLVER_LEARNED = 3 /* MGPCG whose preconditioner first applies the attached network
+                             (pyro_poisson_attach_net), V-cycle fallback */
 };
 typedef struct {
   int iterations; /* outer iterations */
   int vcycles;    /* V-cycles applied */
   double rel_resid; /* sqrt(sum r^2 / sum b^2) over fluid cells, FP64 */
   float ms;       /* device time of the solve (HIP events) */
+  int fallbacks;  /* LEARNED: 1 if the network's direction was rejected and plain V-cycles took over */
 } PyroSolveInfo;
 const char *pyro_poisson_last_error(void);
 /* flags: N x u32, 1 = solid (Neumann). */
@@ -155,6 +158,61 @@ int pyro_poisson_get(PyroPoisson *p, double *phi);
 int pyro_poisson_residual(PyroPoisson *p, double *rel);
 /* Iterate until the relative residual is <= tol or max_iters outer iterations. */
 int pyro_poisson_solve(PyroPoisson *p, int method, double tol, int max_iters, PyroSolveInfo *info);
+
+/* Pressure problem of a simulation (same grid): solid flags from its level-0
+ * flags; b = its current level-0 right-hand side (set by the divergence
+ * stage), phi = its level-0 phi when warm != 0, else 0. */
+int pyro_poisson_set_flags_from_sim(PyroPoisson *p, PyroSim *sim);
+int pyro_poisson_set_from_sim(PyroPoisson *p, PyroSim *sim, int warm);
+
+/* ------------------------------------------------------------------------
+ * Learned preconditioner (no reference counterpart): a 3-level 3D U-Net,
+ * 16/32/64 channels, 3x3x3 convolutions, BF16 on MFMA with FP32
+ * accumulation, FP32 master weights, trained on the device with Adam on
+ * (b, phi) pairs solved by the multigrid solver.  Parameter order: see
+ * pyro_amd/csrc/net.hip.  Network input per voxel: (b / rms(b), fluid mask),
+ * output: phi / (h^2 rms(b) output_scale).
+ * ------------------------------------------------------------------------ */
+typedef struct PyroNet PyroNet;
+const char *pyro_net_last_error(void);
+size_t pyro_net_param_count(void);
+int pyro_net_create(uint64_t seed, PyroNet **out);
+int pyro_net_destroy(PyroNet *net);
+int pyro_net_get_params(PyroNet *net, float *dst, size_t n);
+int pyro_net_set_params(PyroNet *net, const float *src, size_t n);
+int pyro_net_get_grads(PyroNet *net, float *dst, size_t n);
+int pyro_net_get_output_scale(PyroNet *net, float *scale);
+int pyro_net_set_output_scale(PyroNet *net, float scale);
+/* x0: N x 2 floats (input, mask), rounded to BF16; y: N floats. */
+int pyro_net_forward(PyroNet *net, uint32_t nx, uint32_t ny, uint32_t nz, const float *x0, float *y);
+/* Forward, then gradients of sum(gy * y) for all parameters (read with get_grads). */
+int pyro_net_backward(PyroNet *net, uint32_t nx, uint32_t ny, uint32_t nz, const float *x0, const float *gy);
+int pyro_net_dataset_clear(PyroNet *net);
+int pyro_net_dataset_size(PyroNet *net, int *count);
+/* Normalised pair on the host: x0 N x 2 (input, mask), target N. */
+int pyro_net_dataset_add(PyroNet *net, uint32_t nx, uint32_t ny, uint32_t nz, const float *x0, const float *target);
+int pyro_net_dataset_get(PyroNet *net, int index, float *x0, float *target);
+/* Pair from the simulation's current pressure problem, solved from zero to
+ * tol by MGPCG on p (which must have the simulation's grid). */
+int pyro_net_collect(PyroNet *net, PyroSim *sim, PyroPoisson *p, double tol, PyroSolveInfo *info);
+/* `steps` Adam steps (one random pair each); losses[k] = relative squared
+ * error of step k (may be NULL). */
+int pyro_net_train(PyroNet *net, int steps, float lr, uint64_t seed, float *losses);
+int pyro_net_eval(PyroNet *net, int first, int count, double *mean_loss);
+/* mode 0: the network in every preconditioner application; 1: in the first
+ * only (a learned warm start).  net = NULL detaches. */
+int pyro_poisson_attach_net(PyroPoisson *p, PyroNet *net, int mode);
+
+/* Test hooks of the network kernels.  mfma: one v_mfma_f32_16x16x32_bf16 of
+ * lane fragments a, b (64 x 8 floats, rounded to BF16) -> d (64 x 4).
+ * conv: one 3x3x3 conv of the network's kinds (forward, or its adjoint with
+ * transposed = 1 applied to b = an N x cout gradient).  conv_dw: its weight
+ * and bias gradients. */
+int pyro_debug_mfma(const float *a, const float *b, float *d);
+int pyro_debug_conv(int ca, int cb, int cout, int transposed, int relu, uint32_t nx, uint32_t ny, uint32_t nz,
+                    const float *a, const float *b, const float *w, const float *bias, float *out);
+int pyro_debug_conv_dw(int ca, int cb, int cout, uint32_t nx, uint32_t ny, uint32_t nz, const float *a,
+                       const float *b, const float *g, float *dw, float *db);
 
 /* Test hook: apply the device's exp (fn 0), sqrt (1) or division in/in2 (2). */
 int pyro_debug_math(int fn, const float *in, const float *in2, float *out, size_t n);

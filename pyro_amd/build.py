@@ -24,7 +24,8 @@ CSRC = PKG / "csrc"
 HOST = PKG / "host"
 INCLUDE = ROOT / "include"
 
-HIP_SOURCES = ["k_fluid.hip", "k_advect.hip", "k_combust.hip", "k_mg.hip", "k_solve.hip", "sim.hip", "poisson.hip"]
+HIP_SOURCES = ["k_fluid.hip", "k_advect.hip", "k_combust.hip", "k_mg.hip", "k_solve.hip", "k_net.hip", "sim.hip", "poisson.hip",
+               "net.hip"]
 HIP_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",

@@ -165,5 +165,10 @@ __global__ __launch_bounds__(NT) void k_zero32(uint32_t n, float *__restrict__ a
   if (i < n) a[i] = 0.0f;
 }
 void launch_zero32(uint32_t n, float *a, hipStream_t s) { k_zero32<<<nblocks(n), NT, 0, s>>>(n, a); }
+__global__ __launch_bounds__(NT) void k_zero64(uint32_t n, double *__restrict__ a) {
+  uint32_t i = blockIdx.x * NT + threadIdx.x;
+  if (i < n) a[i] = 0.0;
+}
+void launch_zero64(uint32_t n, double *a, hipStream_t s) { k_zero64<<<nblocks(n), NT, 0, s>>>(n, a); }
 
 }  // namespace pyro

@@ -85,6 +85,7 @@ void launch_to_f32(uint32_t n, const double *a, float *b, hipStream_t s);
 void launch_to_f64(uint32_t n, const float *a, double *b, hipStream_t s);
 void launch_add_f32_to_f64(uint32_t n, const float *a, double *b, hipStream_t s);
 void launch_zero32(uint32_t n, float *a, hipStream_t s);
+void launch_zero64(uint32_t n, double *a, hipStream_t s);
 
 // Debug / test: apply device math to an array (0 exp, 1 sqrt, 2 div by in2).
 void launch_debug_math(int fn, const float *in, const float *in2, float *out, uint32_t n, hipStream_t s);

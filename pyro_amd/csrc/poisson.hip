@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "kernels.hpp"
+#include "net.hpp"
 #include "pyro_hip.h"
 
 using namespace pyro;
@@ -46,10 +47,14 @@ struct PyroPoisson {
   MgLevel ref[4];
   int nl_ref = 0;
   double *x = nullptr, *b = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *z = nullptr;
+  double *w = nullptr, *t = nullptr;  // learned preconditioner: N(r) and the residual after it
   double *part = nullptr, *dots = nullptr;
+  double nfluid = 0.0;
   hipStream_t stream = nullptr;
   std::vector<void *> allocs;
   int coarse_iters_pc = 80;
+  PyroNet *net = nullptr;
+  int learned_mode = 0;  // 0: network in every preconditioner application, 1: first only
 };
 
 namespace {
@@ -96,6 +101,48 @@ int rel_resid(PyroPoisson *P, double *rel, double *bb_out) {
   if (rc) return rc;
   *rel = d[1] > 0.0 ? std::sqrt(d[0] / d[1]) : std::sqrt(d[0]);
   if (bb_out) *bb_out = d[1];
+  return 0;
+}
+// solid flags from a simulation's level-0 flags (1 solid; 0 and 2, burnt out, are fluid)
+__global__ __launch_bounds__(256) void k_flags_from_sim(uint32_t n, const uint8_t *__restrict__ src,
+                                                        uint8_t *__restrict__ f) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i < n) f[i] = src[i] == 1 ? 1 : 0;
+}
+
+// s * N(r) + V(r - s A N(r)) with s = (N(r).r) / (N(r).A N(r)), the step that
+// minimises the energy norm of the error along N(r) (A is negative definite,
+// so both dots are negative for a useful direction and s > 0).  ok = false,
+// z untouched, if the direction is unusable (s not positive and finite).
+int learned_precond(PyroPoisson *P, const double *r, double rr, double *z, bool *ok) {
+  *ok = false;
+  const G3 g{P->d[0], P->d[1], P->d[2]};
+  int rc = net_apply64(P->net, g, r, P->pc[0].f, rr, P->nfluid, P->h, P->w, P->stream);
+  if (rc) return pfail(rc, std::string("learned preconditioner: ") + net_error());
+  launch_apply64(P->pc[0], P->w, P->t, P->stream);
+  double d[2];
+  rc = dots(P, {{P->w, r}, {P->w, P->t}}, d);
+  if (rc) return rc;
+  const double sc = d[1] != 0.0 ? d[0] / d[1] : 0.0;
+  if (!(sc > 0.0) || !std::isfinite(sc)) return 0;
+  // t = r - s A w, then z = V(t) + s w
+  launch_xpby64(P->n, r, -sc, P->t, P->stream);
+  precond(P, P->t, z);
+  launch_axpy64(P->n, sc, P->w, z, P->stream);
+  *ok = true;
+  return 0;
+}
+
+int set_flags_device(PyroPoisson *P, const uint8_t *sim_flags) {
+  k_flags_from_sim<<<nblocks(P->n), 256, 0, P->stream>>>(P->n, sim_flags, P->pc[0].f);
+  launch_mg_setup(P->pc, P->nl_pc, P->stream);
+  std::vector<uint8_t> h(P->n);
+  PCHK(hipMemcpyAsync(h.data(), P->pc[0].f, P->n, hipMemcpyDeviceToHost, P->stream));
+  int rc = sync(P);
+  if (rc) return rc;
+  double nf = 0.0;
+  for (uint32_t i = 0; i < P->n; i++) nf += h[i] ? 0.0 : 1.0;
+  P->nfluid = nf;
   return 0;
 }
 }  // namespace
@@ -154,11 +201,16 @@ int pyro_poisson_create(uint32_t nx, uint32_t ny, uint32_t nz, float h, const ui
   if (!rc) rc |= palloc(P, &P->p, P->n);
   if (!rc) rc |= palloc(P, &P->q, P->n);
   if (!rc) rc |= palloc(P, &P->z, P->n);
+  if (!rc) rc |= palloc(P, &P->w, P->n);
+  if (!rc) rc |= palloc(P, &P->t, P->n);
   if (!rc) rc |= palloc(P, &P->part, DOT_PART);
   if (!rc) rc |= palloc(P, &P->dots, 4);
   if (rc) return bail(rc < 0 ? rc : PYRO_E_OOM);
   std::vector<uint8_t> f8(P->n);
-  for (uint32_t i = 0; i < P->n; i++) f8[i] = flags[i] == 1u ? 1 : 0;
+  for (uint32_t i = 0; i < P->n; i++) {
+    f8[i] = flags[i] == 1u ? 1 : 0;
+    P->nfluid += f8[i] ? 0.0 : 1.0;
+  }
   if (hipMemcpy(P->pc[0].f, f8.data(), P->n, hipMemcpyHostToDevice) != hipSuccess)
     return bail(pfail(PYRO_E_HIP, "upload failed"));
   launch_mg_setup(P->pc, P->nl_pc, P->stream);
@@ -205,12 +257,13 @@ int pyro_poisson_residual(PyroPoisson *P, double *rel) {
 
 int pyro_poisson_solve(PyroPoisson *P, int method, double tol, int max_iters, PyroSolveInfo *info) {
   if (!P || max_iters < 1) return pfail(PYRO_E_INVALID, "bad argument");
-  if (method < PYRO_SOLVER_VCYCLE || method > PYRO_SOLVER_MGPCG) return pfail(PYRO_E_INVALID, "unknown method");
+  if (method < PYRO_SOLVER_VCYCLE || method > PYRO_SOLVER_LEARNED) return pfail(PYRO_E_INVALID, "unknown method");
+  if (method == PYRO_SOLVER_LEARNED && !P->net) return pfail(PYRO_E_STATE, "no network attached");
   hipEvent_t e0, e1;
   PCHK(hipEventCreate(&e0));
   PCHK(hipEventCreate(&e1));
   PCHK(hipEventRecord(e0, P->stream));
-  int it = 0, cycles = 0, rc = 0;
+  int it = 0, cycles = 0, rc = 0, fallbacks = 0;
   double rel = 0.0;
   const uint32_t n = P->n;
   if (method == PYRO_SOLVER_VCYCLE) {
@@ -240,10 +293,32 @@ int pyro_poisson_solve(PyroPoisson *P, int method, double tol, int max_iters, Py
       rc = rel_resid(P, &rel, nullptr);
     }
   } else {
-    // flexible CG (Notay's FCG(1)): p_{k+1} = z_{k+1} - (z_{k+1}.q_k / p_k.q_k) p_k
-    rc = rel_resid(P, &rel, nullptr);  // r = b - A x
-    if (!rc && rel > tol) {
+    // flexible CG (Notay's FCG(1)): p_{k+1} = z_{k+1} - (z_{k+1}.q_k / p_k.q_k) p_k.
+    // LEARNED: the preconditioner is s N(r) + V(r - s A N(r)) (the network's
+    // step, optimally scaled, then a V-cycle on what it leaves); the plain
+    // V-cycle takes over for good if the network's direction is unusable.
+    bool use_net = method == PYRO_SOLVER_LEARNED;
+    double rr = 0.0;
+    auto apply_m = [&](double rr_now) -> int {
+      if (use_net) {
+        bool ok = false;
+        int e = learned_precond(P, P->r, rr_now, P->z, &ok);
+        if (e) return e;
+        if (ok) {
+          if (P->learned_mode == 1) use_net = false;
+          return 0;
+        }
+        use_net = false;
+        fallbacks++;
+      }
       precond(P, P->r, P->z);
+      return 0;
+    };
+    double bb0 = 0.0;
+    rc = rel_resid(P, &rel, &bb0);  // r = b - A x
+    if (!rc && rel > tol) {
+      rr = rel * rel * (bb0 > 0.0 ? bb0 : 1.0);
+      rc = apply_m(rr);
       cycles++;
       PCHK(hipMemcpyAsync(P->p, P->z, n * sizeof(double), hipMemcpyDeviceToDevice, P->stream));
       double bb;
@@ -267,7 +342,8 @@ int pyro_poisson_solve(PyroPoisson *P, int method, double tol, int max_iters, Py
         if (rc) break;
         rel = bb > 0.0 ? std::sqrt(d3[0] / bb) : std::sqrt(d3[0]);
         if (rel <= tol) break;
-        precond(P, P->r, P->z);
+        rc = apply_m(d3[0]);
+        if (rc) break;
         cycles++;
         double d4[1];
         rc = dots(P, {{P->z, P->q}}, d4);
@@ -289,8 +365,65 @@ int pyro_poisson_solve(PyroPoisson *P, int method, double tol, int max_iters, Py
     info->vcycles = cycles;
     info->rel_resid = rel;
     info->ms = ms;
+    info->fallbacks = fallbacks;
   }
   return 0;
+}
+
+int pyro_poisson_attach_net(PyroPoisson *P, PyroNet *net, int mode) {
+  if (!P || (mode != 0 && mode != 1)) return pfail(PYRO_E_INVALID, "bad argument");
+  P->net = net;
+  P->learned_mode = mode;
+  return 0;
+}
+
+int pyro_poisson_set_flags_from_sim(PyroPoisson *P, PyroSim *sim) {
+  if (!P || !sim) return pfail(PYRO_E_INVALID, "null argument");
+  void *fp = nullptr;
+  size_t bytes = 0;
+  if (pyro_sim_sync(sim) || pyro_sim_device_ptr(sim, PYRO_FIELD_LEVEL_BASE + 2, &fp, &bytes))
+    return pfail(PYRO_E_INVALID, std::string("simulation: ") + pyro_last_error());
+  if (bytes != P->n) return pfail(PYRO_E_INVALID, "grid mismatch");
+  return set_flags_device(P, static_cast<const uint8_t *>(fp));
+}
+
+// b = the simulation's current level-0 right-hand side, phi = its level-0 phi
+// (warm start) or 0.
+int pyro_poisson_set_from_sim(PyroPoisson *P, PyroSim *sim, int warm) {
+  if (!P || !sim) return pfail(PYRO_E_INVALID, "null argument");
+  void *bp = nullptr, *pp = nullptr;
+  size_t nb = 0, npb = 0;
+  if (pyro_sim_sync(sim) || pyro_sim_device_ptr(sim, PYRO_FIELD_LEVEL_BASE + 1, &bp, &nb) ||
+      pyro_sim_device_ptr(sim, PYRO_FIELD_LEVEL_BASE + 0, &pp, &npb))
+    return pfail(PYRO_E_INVALID, std::string("simulation: ") + pyro_last_error());
+  if (nb != (size_t)P->n * 4) return pfail(PYRO_E_INVALID, "grid mismatch");
+  launch_to_f64(P->n, static_cast<const float *>(bp), P->b, P->stream);
+  if (warm)
+    launch_to_f64(P->n, static_cast<const float *>(pp), P->x, P->stream);
+  else
+    launch_zero64(P->n, P->x, P->stream);
+  return sync(P);
+}
+
+// Training pair from the simulation's current pressure problem: b as it
+// stands, phi = the MGPCG solution from zero to `tol`.
+int pyro_net_collect(PyroNet *net, PyroSim *sim, PyroPoisson *P, double tol, PyroSolveInfo *info) {
+  if (!net || !sim || !P) return pfail(PYRO_E_INVALID, "null argument");
+  int rc = pyro_poisson_set_flags_from_sim(P, sim);
+  if (!rc) rc = pyro_poisson_set_from_sim(P, sim, 0);
+  if (rc) return rc;
+  launch_zero64(P->n, P->x, P->stream);
+  PyroSolveInfo si;
+  rc = pyro_poisson_solve(P, PYRO_SOLVER_MGPCG, tol, 400, &si);
+  if (rc) return rc;
+  if (info) *info = si;
+  double bb[1];
+  rc = dots(P, {{P->b, P->b}}, bb);
+  if (rc) return rc;
+  const G3 g{P->d[0], P->d[1], P->d[2]};
+  rc = net_add_pair64(net, g, P->b, P->x, P->pc[0].f, bb[0], P->nfluid, P->h, P->stream);
+  if (rc) return pfail(rc, net_error());
+  return sync(P);
 }
 
 }  // extern "C"

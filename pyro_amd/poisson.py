@@ -22,11 +22,12 @@ import numpy as np
 
 from . import _native
 
-METHODS = {"vcycle": 0, "refine": 1, "mgpcg": 2}
+METHODS = {"vcycle": 0, "refine": 1, "mgpcg": 2, "learned": 3}
 
 
 class _Info(C.Structure):
-    _fields_ = [("iterations", C.c_int), ("vcycles", C.c_int), ("rel_resid", C.c_double), ("ms", C.c_float)]
+    _fields_ = [("iterations", C.c_int), ("vcycles", C.c_int), ("rel_resid", C.c_double), ("ms", C.c_float),
+                ("fallbacks", C.c_int)]
 
 
 @dataclass
@@ -35,6 +36,7 @@ class SolveInfo:
     vcycles: int
     rel_resid: float
     ms: float
+    fallbacks: int = 0
 
 
 def _lib():
@@ -49,6 +51,9 @@ def _lib():
         lib.pyro_poisson_get.argtypes = [P, P]
         lib.pyro_poisson_residual.argtypes = [P, C.POINTER(C.c_double)]
         lib.pyro_poisson_solve.argtypes = [P, C.c_int, C.c_double, C.c_int, C.POINTER(_Info)]
+        lib.pyro_poisson_attach_net.argtypes = [P, P, C.c_int]
+        lib.pyro_poisson_set_flags_from_sim.argtypes = [P, P]
+        lib.pyro_poisson_set_from_sim.argtypes = [P, P, C.c_int]
         lib._poisson_ready = True
     return lib
 
@@ -112,7 +117,21 @@ class Poisson:
     def solve(self, method: str = "mgpcg", tol: float = 1e-6, max_iters: int = 200) -> SolveInfo:
         info = _Info()
         _check(_lib().pyro_poisson_solve(self._h, METHODS[method], float(tol), int(max_iters), C.byref(info)))
-        return SolveInfo(info.iterations, info.vcycles, info.rel_resid, info.ms)
+        return SolveInfo(info.iterations, info.vcycles, info.rel_resid, info.ms, info.fallbacks)
+
+    def attach_net(self, net, mode: str = "every") -> None:
+        """Use `net` (a pyro_amd.net.Net) in the "learned" method: in every
+        preconditioner application ("every") or only the first ("first")."""
+        self._net = net
+        _check(_lib().pyro_poisson_attach_net(self._h, None if net is None else net._h,
+                                              {"every": 0, "first": 1}[mode]))
+
+    def set_from_sim(self, sim, warm: bool = True, flags: bool = True) -> None:
+        """b = the simulation's current level-0 right-hand side; phi = its
+        level-0 phi (warm) or 0; solid flags from the simulation (flags)."""
+        if flags:
+            _check(_lib().pyro_poisson_set_flags_from_sim(self._h, sim._h))
+        _check(_lib().pyro_poisson_set_from_sim(self._h, sim._h, 1 if warm else 0))
 
 
 def fluid_components(flags: np.ndarray):

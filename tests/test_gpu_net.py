@@ -1,0 +1,178 @@
+"""The learned preconditioner's kernels on the GPU against the oracle
+(oracle/unet_ref.py).
+
+- one v_mfma_f32_16x16x32_bf16: the lane layout the kernels assume, exactly;
+- every conv kind (forward, adjoint, weight gradient) bit-exact on small
+  integer data, where FP32 and float64 sums agree exactly;
+- the whole network forward and backward on real-valued data within BF16
+  tolerances (relative L2 <= 2e-2 on y, <= 5e-2 on each layer's gradient);
+- training on solved pressure pairs lowers the loss, and the learned MGPCG
+  converges, with the V-cycle fallback when the network is unusable.
+"""
+import numpy as np
+import pytest
+
+from oracle import unet_ref as U
+from pyro_amd import net as N
+
+pytestmark = pytest.mark.gpu
+
+KINDS = [(0, 16, 16), (0, 16, 32), (0, 32, 32), (0, 32, 64), (0, 64, 64), (64, 32, 32), (32, 16, 16)]
+
+
+def test_mfma_lane_layout():
+    rng = np.random.default_rng(0)
+    a = rng.integers(-4, 5, size=(64, 8)).astype(np.float32)
+    b = rng.integers(-4, 5, size=(64, 8)).astype(np.float32)
+    d = N.debug_mfma(a, b)
+    A = np.zeros((16, 32))
+    B = np.zeros((32, 16))
+    for lane in range(64):
+        j, c = divmod(lane, 16)
+        A[c, 8 * j:8 * j + 8] = a[lane]
+        B[8 * j:8 * j + 8, c] = b[lane]
+    D = A @ B
+    expect = np.zeros((64, 4))
+    for lane in range(64):
+        j, c = divmod(lane, 16)
+        expect[lane] = D[4 * j:4 * j + 4, c]
+    assert np.array_equal(d, expect)
+
+
+def _ints(rng, shape, lo=-2, hi=3):
+    return rng.integers(lo, hi, size=shape).astype(np.float32)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_conv_kinds_bitexact(kind):
+    ca, cb, co = kind
+    rng = np.random.default_rng(ca * 7 + cb * 3 + co)
+    shape = (9, 7, 21)  # partial tiles in x, y and z; odd dims for the half grid
+    cs = tuple((s + 1) // 2 for s in shape)
+    a = _ints(rng, cs + (ca,)) if ca else None
+    b = _ints(rng, shape + (cb,))
+    W = _ints(rng, (co, ca + cb, 27), -1, 2)
+    bias = _ints(rng, (co,), -3, 4)
+    got = N.debug_conv(ca, cb, co, a, b, W, bias, relu=True)
+    ref = U.conv_layer(a, b, W, bias, relu_on=True)
+    assert np.array_equal(got, ref.astype(np.float32))
+    g = _ints(rng, shape + (co,))
+    got = N.debug_conv(ca, cb, co, None, g, W, transposed=True)
+    ref = U.conv_layer_adj(g, W)
+    assert np.array_equal(got, ref.astype(np.float32))
+    dW, db = N.debug_conv_dw(ca, cb, co, a, b, g)
+    rW, rb = U.conv_layer_dw(a, b, g, co)
+    assert np.array_equal(dW, rW.astype(np.float32))
+    assert np.array_equal(db, rb.astype(np.float32))
+
+
+def test_first_layer_dw_bitexact():
+    rng = np.random.default_rng(11)
+    shape = (6, 10, 18)
+    b = _ints(rng, shape + (2,))
+    g = _ints(rng, shape + (16,))
+    dW, db = N.debug_conv_dw(0, 2, 16, None, b, g)
+    rW, rb = U.conv_layer_dw(None, b, g, 16)
+    assert np.array_equal(dW, rW.astype(np.float32))
+    assert np.array_equal(db, rb.astype(np.float32))
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-30))
+
+
+@pytest.mark.parametrize("shape", [(8, 12, 16), (9, 13, 19)])
+def test_network_forward_backward_vs_oracle(shape):
+    net = N.Net(seed=3)
+    p = net.params()
+    rng = np.random.default_rng(4)
+    x0 = np.zeros(shape + (2,), np.float32)
+    m = (rng.random(shape) > 0.15).astype(np.float32)
+    x0[..., 0] = rng.normal(size=shape) * m
+    x0[..., 1] = m
+    y = net.forward(x0)
+    yr = U.forward(p, x0)
+    assert rel(y, yr) < 2e-2, rel(y, yr)
+    gy = rng.normal(size=shape).astype(np.float32)
+    g = net.backward(x0, gy)
+    gr = U.backward(p, x0, gy)
+    G, GR = N.split_params(g), U.split(gr)
+    for k in GR:
+        assert rel(G[k], GR[k]) < 5e-2, (k, rel(G[k], GR[k]))
+
+
+def _poisson_pairs(n, count, seed):
+    """(flags, b, phi) triples solved by MGPCG to 1e-10"""
+    from pyro_amd.poisson import Poisson, make_c3_problem
+
+    out = []
+    for k in range(count):
+        flags, b = make_c3_problem(n, seed=seed + k, h=1.0 / n, blobs=8)
+        P = Poisson(flags, 1.0 / n)
+        P.set(b=b, phi=np.zeros_like(b))
+        info = P.solve("mgpcg", 1e-10, 200)
+        assert info.rel_resid <= 1e-10
+        out.append((flags, b, P.get()))
+        P.close()
+    return out
+
+
+def _normalised(flags, b, phi, h, scale):
+    m = flags == 0
+    rms = np.sqrt((b[m] ** 2).mean())
+    x0 = np.zeros(b.shape + (2,), np.float32)
+    x0[..., 0] = np.where(m, b / rms, 0.0)
+    x0[..., 1] = m
+    t = np.where(m, phi / (h * h * rms * scale), 0.0).astype(np.float32)
+    return x0, t
+
+
+def test_training_lowers_loss_and_learned_solver_converges():
+    from pyro_amd.poisson import Poisson
+
+    n = 32
+    h = 1.0 / n
+    pairs = _poisson_pairs(n, 6, 100)
+    net = N.Net(seed=5)
+    net.output_scale = 16.0
+    for flags, b, phi in pairs[:5]:
+        x0, t = _normalised(flags, b, phi, h, 16.0)
+        net.dataset_add(x0, t)
+    assert net.dataset_size() == 5
+    x0b, tb = net.dataset_get(0, (n, n, n))
+    x0r, tr = _normalised(*pairs[0], h, 16.0)
+    assert np.array_equal(tb, tr)
+    before = net.eval()
+    losses = net.train(300, lr=2e-3, seed=1)
+    after = net.eval()
+    print("loss before", before, "after", after, "first/last", losses[:3], losses[-3:])
+    assert np.all(np.isfinite(losses))
+    assert after < 0.5 * before
+    # the learned MGPCG on an unseen problem
+    flags, b, phi = pairs[5]
+    P = Poisson(flags, h)
+    P.attach_net(net)
+    P.set(b=b, phi=np.zeros_like(b))
+    li = P.solve("learned", 1e-8, 100)
+    P.set(phi=np.zeros_like(b))
+    mi = P.solve("mgpcg", 1e-8, 100)
+    print("learned", li, "mgpcg", mi)
+    assert li.rel_resid <= 1e-8 and li.fallbacks == 0
+    fluid = flags == 0
+    assert rel(P.get()[fluid], phi[fluid]) < 1e-6
+
+
+def test_learned_solver_falls_back_to_vcycles():
+    """A network whose output is zero is rejected; plain V-cycles finish the solve."""
+    from pyro_amd.poisson import Poisson, make_c3_problem
+
+    n = 24
+    flags, b = make_c3_problem(n, seed=9, h=1.0 / n, blobs=4)
+    net = N.Net(seed=0)
+    net.set_params(np.zeros(net.n, np.float32))
+    net.output_scale = 1.0
+    P = Poisson(flags, 1.0 / n)
+    P.attach_net(net)
+    P.set(b=b, phi=np.zeros_like(b))
+    info = P.solve("learned", 1e-8, 100)
+    assert info.fallbacks == 1 and info.rel_resid <= 1e-8
