@@ -24,7 +24,9 @@ Extra fields:
   pressure      -- time to a relative residual of 1e-4 (sqrt of sum r^2 over
                    sum b^2 on fluid cells, FP64) on the next step's pressure
                    problem of this run, warm-started as the step would be,
-                   by the reference V-cycles and by MGPCG; and the standalone
+                   by the reference V-cycles, FP64-residual refinement,
+                   MGPCG and MGPCG with the learned U-Net (trained first, on
+                   the GPU, from 4 cm pool frames); and the standalone
                    256^3 case (random solid boxes, 32 Gaussian sources):
                    V-cycles to 1e-4, FP64-residual refinement and MGPCG to
                    1e-6, with the error against an FP64 solution.
@@ -46,6 +48,12 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 SWEEP_BYTES_PER_CELL = 16.0  # read phi 4 + b 4 + flags 4, write phi 4 (reference layout)
 TRAFFIC_FILE = ROOT / "profiles" / "mg_sweep_l0_pool_2cm_traffic.json"
+# U-Net FLOPs per level-0 voxel: 2*27*cin*cout per conv at its level's
+# resolution (1/8 per halving) plus the 1x1x1 head: level 0 2*27*(2*16 + 16*16
+# + 48*16) + 2*16, level 1 2*27*(16*32 + 32*32 + 96*32)/8, level 2
+# 2*27*(32*64 + 64*64)/64
+UNET_FLOP_PER_VOXEL = 2 * 27 * (2 * 16 + 16 * 16 + 48 * 16) + 2 * 16 + 2 * 27 * (16 * 32 + 32 * 32 + 96 * 32) / 8 \
+    + 2 * 27 * (32 * 64 + 64 * 64) / 64
 
 
 def log(msg: str) -> None:
@@ -105,42 +113,68 @@ def gpu_run(args):
     copy_ms = pa.sim.debug_copy(64 << 20, 20)
     copy_gbs = 2 * (64 << 20) * 4 / (copy_ms * 1e-3) / 1e9
     log(f"[bench] plain copy: {copy_gbs:.0f} GB/s")
-    frame = None if args.no_pressure else pressure_frame(sim, args, p["dx"])
+    net, learn = (None, None) if args.no_pressure else learned_net(args)
+    frame = None if args.no_pressure else pressure_frame(sim, args, p["dx"], net)
+    if learn is not None:
+        learn["forward_ms_pool_2cm"] = net.time_forward((nz, ny, nx), 10)
+        learn["forward_tflops_pool_2cm"] = UNET_FLOP_PER_VOXEL * cells / (learn["forward_ms_pool_2cm"] * 1e-3) / 1e12
+        log(f"[bench] U-Net forward on {nx}x{ny}x{nz}: {learn['forward_ms_pool_2cm']:.3f} ms, "
+            f"{learn['forward_tflops_pool_2cm']:.0f} TFLOP/s")
     return dict(cells=cells, wall=wall, stats=st, stage_ms=stage_ms, smooth_ms=smooth_ms,
                 coarse_ms=coarse_ms, achieved=achieved, algo_bytes=algo_bytes, dims=(nx, ny, nz),
-                copy_gbs=copy_gbs, frame=frame)
+                copy_gbs=copy_gbs, frame=frame, learned=learn)
 
 
 def _info(i):
     return {"ms": round(i.ms, 3), "vcycles": i.vcycles, "iterations": i.iterations, "rel_resid": i.rel_resid}
 
 
-def pressure_frame(sim, args, dx):
+def learned_net(args):
+    """The U-Net trained on the GPU by this library's kernels: pairs from the
+    pool at 4 cm (80 x 160 x 80) solved by MGPCG to 1e-6, then Adam."""
+    from pyro_amd import learned
+
+    t0 = time.perf_counter()
+    net, rep = learned.train_pool_net(frames=args.train_pairs, steps=args.train_steps, lr=1e-3, log=log)
+    rep["total_seconds"] = time.perf_counter() - t0
+    rep["flop_per_voxel"] = UNET_FLOP_PER_VOXEL
+    log(f"[bench] U-Net trained: held-out loss {rep['heldout_loss']:.4f} in {rep['total_seconds']:.1f} s")
+    return net, rep
+
+
+def pressure_frame(sim, args, dx, net=None):
     """Time to a 1e-4 relative residual on the pressure problem of the next
     step of the benchmark run: its right-hand side, solid flags and the warm
-    start the step itself would use (phi of the previous step)."""
+    start the step itself would use (phi of the previous step).  Methods:
+    the reference V-cycles (plain MG as the reference runs it), FP64-residual
+    refinement (plain MG iteration on the consistent hierarchy), MGPCG, and
+    MGPCG with the learned network (every preconditioner application, or
+    the first only)."""
     import pyro_amd as pa
     from pyro_amd.poisson import Poisson
 
     sim.begin_step(args.dt, False)
     for stg in pa.STAGES[: pa.STAGES.index("divergence") + 1]:
         sim.run_stage(stg)
-    b = sim.read_field(pa.sim.level_field(0, "b")).astype(np.float64)
-    phi = sim.read_field(pa.sim.level_field(0, "phi")).astype(np.float64)
-    flags = sim.read_field(pa.sim.level_field(0, "flags"))
+    P = Poisson(sim.read_field(pa.sim.level_field(0, "flags")), dx)
+    P.set_from_sim(sim, warm=True)
+    out = {"tol": 1e-4, "start": "warm (phi of the previous step)", "initial_rel_resid": P.residual()}
+    runs = [("vcycle", None, 400), ("refine", None, 100), ("mgpcg", None, 100)]
+    if net is not None:
+        runs += [("learned", "every", 100), ("learned", "first", 100)]
+    for m, mode, cap in runs:
+        if mode:
+            P.attach_net(net, mode)
+        key = m if mode is None else f"{m}_{mode}"
+        P.set_from_sim(sim, warm=True, flags=False)
+        P.solve(m, 1e-4, min(cap, 20))  # first call: allocation / code-object warm-up
+        P.set_from_sim(sim, warm=True, flags=False)
+        out[key] = _info(P.solve(m, 1e-4, cap))
+        out[key]["reached"] = out[key]["rel_resid"] <= 1e-4
+        log(f"[bench] frame pressure {key}: {out[key]}")
     for stg in pa.STAGES[pa.STAGES.index("divergence") + 1:]:
         sim.run_stage(stg)
     sim.end_step()
-    P = Poisson(flags, dx)
-    out = {"tol": 1e-4, "start": "warm (phi of the previous step)"}
-    for m, cap in (("vcycle", 400), ("mgpcg", 100)):
-        P.set(b=b, phi=phi)
-        P.solve(m, 1e-4, cap)  # first call: allocation / code-object warm-up
-        P.set(b=b, phi=phi)
-        out[m] = _info(P.solve(m, 1e-4, cap))
-        log(f"[bench] frame pressure {m}: {out[m]}")
-    P.set(b=b, phi=phi)
-    out["initial_rel_resid"] = P.residual()
     P.close()
     return out
 
@@ -224,6 +258,8 @@ def main():
     ap.add_argument("--no-pressure", action="store_true", help="skip the pressure-solver legs")
     ap.add_argument("--c3-n", type=int, default=256)
     ap.add_argument("--c3-cpu", action="store_true", help="also solve C3 with the FP64 CPU oracle (~1 min)")
+    ap.add_argument("--train-pairs", type=int, default=2000, help="U-Net training pairs (pool, 4 cm)")
+    ap.add_argument("--train-steps", type=int, default=3000, help="U-Net Adam steps")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -277,7 +313,7 @@ def main():
         "cpu_baseline": cpu,
         "stage_ms": r["stage_ms"],
         "coarse_solve_ms": r["coarse_ms"],
-        "pressure": {"pool_frame": r["frame"], "standalone": c3},
+        "pressure": {"pool_frame": r["frame"], "standalone": c3, "learned_training": r["learned"]},
         "copy_kernel_GBs": r["copy_gbs"],
         "last_stats": {"hrr_W": r["stats"].hrr, "max_T": r["stats"].max_t, "resid": r["stats"].resid},
     }

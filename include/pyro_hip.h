@@ -164,6 +164,11 @@ int pyro_poisson_solve(PyroPoisson *p, int method, double tol, int max_iters, Py
  * stage), phi = its level-0 phi when warm != 0, else 0. */
 int pyro_poisson_set_flags_from_sim(PyroPoisson *p, PyroSim *sim);
 int pyro_poisson_set_from_sim(PyroPoisson *p, PyroSim *sim, int warm);
+/* One step (as pyro_sim_step) whose pressure solve runs `method` on p to a
+ * relative residual of tol (at most max_iters outer iterations), warm-started
+ * from the previous phi, instead of the fixed V-cycle count. */
+int pyro_sim_step_solve(PyroSim *sim, PyroPoisson *p, int method, double tol, int max_iters, float dt,
+                        int want_stats, PyroSolveInfo *info);
 
 /* ------------------------------------------------------------------------
  * Learned preconditioner (no reference counterpart): a 3-level 3D U-Net,
@@ -195,10 +200,12 @@ int pyro_net_dataset_get(PyroNet *net, int index, float *x0, float *target);
 /* Pair from the simulation's current pressure problem, solved from zero to
  * tol by MGPCG on p (which must have the simulation's grid). */
 int pyro_net_collect(PyroNet *net, PyroSim *sim, PyroPoisson *p, double tol, PyroSolveInfo *info);
-/* `steps` Adam steps (one random pair each); losses[k] = relative squared
- * error of step k (may be NULL). */
-int pyro_net_train(PyroNet *net, int steps, float lr, uint64_t seed, float *losses);
+/* `steps` Adam steps, each on one random pair of [first, first + count);
+ * losses[k] = relative squared error of step k (may be NULL). */
+int pyro_net_train(PyroNet *net, int first, int count, int steps, float lr, uint64_t seed, float *losses);
 int pyro_net_eval(PyroNet *net, int first, int count, double *mean_loss);
+/* Average device time (HIP events) of `reps` forward passes on a grid. */
+int pyro_net_time_forward(PyroNet *net, uint32_t nx, uint32_t ny, uint32_t nz, int reps, float *avg_ms);
 /* mode 0: the network in every preconditioner application; 1: in the first
  * only (a learned warm start).  net = NULL detaches. */
 int pyro_poisson_attach_net(PyroPoisson *p, PyroNet *net, int mode);

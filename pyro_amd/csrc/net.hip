@@ -499,11 +499,13 @@ int pyro_net_dataset_get(PyroNet *N, int index, float *x0, float *target) {
   return 0;
 }
 
-// Adam on the relative squared error of random training pairs, one pair per
-// step; losses[k] = the loss of step k before its update.
-int pyro_net_train(PyroNet *N, int steps, float lr, uint64_t seed, float *losses) {
+// Adam on the relative squared error of random training pairs from
+// [first, first + count), one pair per step; losses[k] = the loss of step k
+// before its update.
+int pyro_net_train(PyroNet *N, int first, int count, int steps, float lr, uint64_t seed, float *losses) {
   if (!N || steps < 0 || !(lr > 0.0f)) return nfail(PYRO_E_INVALID, "bad argument");
   if (N->data.empty()) return nfail(PYRO_E_STATE, "empty training set");
+  if (first < 0 || count < 1 || first + count > (int)N->data.size()) return nfail(PYRO_E_INVALID, "bad pair range");
   int rc = work_alloc(N->tr, N->data_g, true);
   if (rc) return rc;
   if (N->losses_cap < steps) {
@@ -523,7 +525,7 @@ int pyro_net_train(PyroNet *N, int steps, float lr, uint64_t seed, float *losses
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   Work &W = N->tr;
   for (int k = 0; k < steps; k++) {
-    const Sample &S = N->data[R.next() % N->data.size()];
+    const Sample &S = N->data[first + (int)(R.next() % (uint64_t)count)];
     forward(N, W, S.x0);
     launch_loss(W.g0, W.y, S.t, S.x0, W.lpart, N->losses + k, N->stream);
     launch_loss_grad(W.g0, W.y, S.t, S.x0, W.lpart + 2 * LOSS_BLOCKS + 1, W.gy, N->stream);
@@ -557,6 +559,31 @@ int pyro_net_eval(PyroNet *N, int first, int count, double *mean_loss) {
   }
   *mean_loss = sum / count;
   return 0;
+}
+
+// Average device time of `reps` forward passes on an nx x ny x nz grid (the
+// inference workspace's current input; zero after allocation).
+int pyro_net_time_forward(PyroNet *N, uint32_t nx, uint32_t ny, uint32_t nz, int reps, float *avg_ms) {
+  if (!N || !avg_ms || reps < 1) return nfail(PYRO_E_INVALID, "bad argument");
+  int rc = check_dims(nx, ny, nz);
+  if (rc) return rc;
+  const G3 g{(int)nx, (int)ny, (int)nz};
+  rc = work_alloc(N->inf, g, false);
+  if (rc) return rc;
+  forward(N, N->inf, N->inf.x0);
+  hipEvent_t a, b;
+  NCHK(hipEventCreate(&a));
+  NCHK(hipEventCreate(&b));
+  NCHK(hipEventRecord(a, N->stream));
+  for (int k = 0; k < reps; k++) forward(N, N->inf, N->inf.x0);
+  NCHK(hipEventRecord(b, N->stream));
+  NCHK(hipEventSynchronize(b));
+  float ms = 0.0f;
+  NCHK(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *avg_ms = ms / reps;
+  return nsync(N);
 }
 
 int pyro_debug_mfma(const float *a, const float *b, float *d) {

@@ -50,6 +50,7 @@ struct PyroPoisson {
   double *w = nullptr, *t = nullptr;  // learned preconditioner: N(r) and the residual after it
   double *part = nullptr, *dots = nullptr;
   double nfluid = 0.0;
+  unsigned int *count = nullptr;  // device fluid-cell counter
   hipStream_t stream = nullptr;
   std::vector<void *> allocs;
   int coarse_iters_pc = 80;
@@ -133,16 +134,25 @@ int learned_precond(PyroPoisson *P, const double *r, double rr, double *z, bool 
   return 0;
 }
 
+// number of fluid cells (integer atomics: exact and order-independent)
+__global__ __launch_bounds__(256) void k_count_fluid(uint32_t n, const uint8_t *__restrict__ f,
+                                                     unsigned int *__restrict__ cnt) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  const bool fl = i < n && f[i] != 1;
+  const unsigned long long m = __ballot(fl);
+  if ((threadIdx.x & 63u) == 0 && m) atomicAdd(cnt, (unsigned int)__popcll(m));
+}
+
 int set_flags_device(PyroPoisson *P, const uint8_t *sim_flags) {
   k_flags_from_sim<<<nblocks(P->n), 256, 0, P->stream>>>(P->n, sim_flags, P->pc[0].f);
   launch_mg_setup(P->pc, P->nl_pc, P->stream);
-  std::vector<uint8_t> h(P->n);
-  PCHK(hipMemcpyAsync(h.data(), P->pc[0].f, P->n, hipMemcpyDeviceToHost, P->stream));
+  PCHK(hipMemsetAsync(P->count, 0, sizeof(unsigned int), P->stream));
+  k_count_fluid<<<nblocks(P->n), 256, 0, P->stream>>>(P->n, P->pc[0].f, P->count);
+  unsigned int c = 0;
+  PCHK(hipMemcpyAsync(&c, P->count, sizeof(c), hipMemcpyDeviceToHost, P->stream));
   int rc = sync(P);
   if (rc) return rc;
-  double nf = 0.0;
-  for (uint32_t i = 0; i < P->n; i++) nf += h[i] ? 0.0 : 1.0;
-  P->nfluid = nf;
+  P->nfluid = (double)c;
   return 0;
 }
 }  // namespace
@@ -203,6 +213,7 @@ int pyro_poisson_create(uint32_t nx, uint32_t ny, uint32_t nz, float h, const ui
   if (!rc) rc |= palloc(P, &P->z, P->n);
   if (!rc) rc |= palloc(P, &P->w, P->n);
   if (!rc) rc |= palloc(P, &P->t, P->n);
+  if (!rc) rc |= palloc(P, &P->count, 4);
   if (!rc) rc |= palloc(P, &P->part, DOT_PART);
   if (!rc) rc |= palloc(P, &P->dots, 4);
   if (rc) return bail(rc < 0 ? rc : PYRO_E_OOM);
@@ -403,6 +414,35 @@ int pyro_poisson_set_from_sim(PyroPoisson *P, PyroSim *sim, int warm) {
   else
     launch_zero64(P->n, P->x, P->stream);
   return sync(P);
+}
+
+// One simulation step whose pressure solve runs `method` to `tol` (at most
+// max_iters outer iterations) from the previous step's phi, instead of the
+// fixed V-cycle count: stages up to the divergence (sim.rs:246-403), the
+// solve, phi back to the simulation in FP32, then the residual statistics
+// (when requested) and the projection (sim.rs:405-423).
+int pyro_sim_step_solve(PyroSim *sim, PyroPoisson *P, int method, double tol, int max_iters, float dt,
+                        int want_stats, PyroSolveInfo *info) {
+  if (!sim || !P) return pfail(PYRO_E_INVALID, "null argument");
+  auto simfail = [&]() { return pfail(PYRO_E_STATE, std::string("simulation: ") + pyro_last_error()); };
+  if (pyro_sim_begin_step(sim, dt, want_stats)) return simfail();
+  for (int st = PYRO_STAGE_SOLID; st <= PYRO_STAGE_DIVERGENCE; st++)
+    if (pyro_sim_run_stage(sim, st)) return simfail();
+  int rc = pyro_poisson_set_flags_from_sim(P, sim);
+  if (!rc) rc = pyro_poisson_set_from_sim(P, sim, 1);
+  PyroSolveInfo si{};
+  if (!rc) rc = pyro_poisson_solve(P, method, tol, max_iters, &si);
+  if (rc) return rc;
+  void *pp = nullptr;
+  size_t nb = 0;
+  if (pyro_sim_device_ptr(sim, PYRO_FIELD_LEVEL_BASE + 0, &pp, &nb)) return simfail();
+  launch_to_f32(P->n, P->x, static_cast<float *>(pp), P->stream);
+  rc = sync(P);
+  if (rc) return rc;
+  if (want_stats && pyro_sim_run_stage(sim, PYRO_STAGE_RESID)) return simfail();
+  if (pyro_sim_run_stage(sim, PYRO_STAGE_PROJECT) || pyro_sim_end_step(sim)) return simfail();
+  if (info) *info = si;
+  return 0;
 }
 
 // Training pair from the simulation's current pressure problem: b as it

@@ -45,8 +45,9 @@ def _lib():
         lib.pyro_net_dataset_add.argtypes = [P, U, U, U, P, P]
         lib.pyro_net_dataset_get.argtypes = [P, C.c_int, P, P]
         lib.pyro_net_collect.argtypes = [P, P, P, C.c_double, C.POINTER(_Info)]
-        lib.pyro_net_train.argtypes = [P, C.c_int, C.c_float, C.c_uint64, P]
+        lib.pyro_net_train.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint64, P]
         lib.pyro_net_eval.argtypes = [P, C.c_int, C.c_int, C.POINTER(C.c_double)]
+        lib.pyro_net_time_forward.argtypes = [P, U, U, U, C.c_int, F]
         lib.pyro_debug_mfma.argtypes = [P, P, P]
         lib.pyro_debug_conv.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, U, U, U, P, P, P, P, P]
         lib.pyro_debug_conv_dw.argtypes = [C.c_int, C.c_int, C.c_int, U, U, U, P, P, P, P, P]
@@ -171,10 +172,21 @@ class Net:
             raise _native.PyroError(rc, _native.hip().pyro_poisson_last_error().decode(errors="replace"))
         return SolveInfo(info.iterations, info.vcycles, info.rel_resid, info.ms, info.fallbacks)
 
-    def train(self, steps: int, lr: float = 1e-3, seed: int = 0) -> np.ndarray:
+    def train(self, steps: int, lr: float = 1e-3, seed: int = 0, first: int = 0, count: int | None = None) -> np.ndarray:
+        """Adam steps on random pairs of [first, first + count); per-step losses."""
+        if count is None:
+            count = self.dataset_size() - first
         losses = np.zeros(max(steps, 1), np.float32)
-        _check(_lib().pyro_net_train(self._h, int(steps), float(lr), int(seed), losses.ctypes.data))
+        _check(_lib().pyro_net_train(self._h, int(first), int(count), int(steps), float(lr), int(seed),
+                                     losses.ctypes.data))
         return losses[:steps]
+
+    def time_forward(self, shape, reps: int = 10) -> float:
+        """average device ms of one forward pass on an (nz, ny, nx) grid"""
+        nz, ny, nx = shape
+        ms = C.c_float()
+        _check(_lib().pyro_net_time_forward(self._h, nx, ny, nz, int(reps), C.byref(ms)))
+        return ms.value
 
     def eval(self, first: int = 0, count: int | None = None) -> float:
         if count is None:

@@ -54,6 +54,8 @@ def _lib():
         lib.pyro_poisson_attach_net.argtypes = [P, P, C.c_int]
         lib.pyro_poisson_set_flags_from_sim.argtypes = [P, P]
         lib.pyro_poisson_set_from_sim.argtypes = [P, P, C.c_int]
+        lib.pyro_sim_step_solve.argtypes = [P, P, C.c_int, C.c_double, C.c_int, C.c_float, C.c_int,
+                                            C.POINTER(_Info)]
         lib._poisson_ready = True
     return lib
 
@@ -182,3 +184,12 @@ def make_c3_problem(n: int = 256, seed: int = 0, h: float = 1.0 / 256, solid_fra
         mean[0] = 0.0
         b -= mean[lab]
     return flags, b
+
+
+def step_solve(sim, P: Poisson, dt: float, method: str = "mgpcg", tol: float = 1e-4, max_iters: int = 100,
+               want_stats: bool = False) -> SolveInfo:
+    """One simulation step whose pressure solve runs `method` on P to `tol`."""
+    info = _Info()
+    _check(_lib().pyro_sim_step_solve(sim._h, P._h, METHODS[method], float(tol), int(max_iters), float(dt),
+                                      int(bool(want_stats)), C.byref(info)))
+    return SolveInfo(info.iterations, info.vcycles, info.rel_resid, info.ms, info.fallbacks)
