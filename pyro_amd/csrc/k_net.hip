@@ -99,8 +99,100 @@ struct ConvArgs {
   bf16 *out;
 };
 
+// footprint offset (in voxels) of tap t relative to footprint voxel (col, w, nb)
+__device__ __forceinline__ constexpr int tap_off(int t) {
+  return (t / 9) * (CV_FX * CV_FY) + ((t / 3) % 3) * CV_FX + (t % 3);
+}
+
+// Stage one channel chunk (CC channels from `a` through nearest upsampling
+// when FROMA, else from `b`) of the 18 x 6 x 6 footprint into LDS.
+template <int CC, int CSRC, bool FROMA>
+__device__ __forceinline__ void conv_stage(const ConvArgs &p, int c0, int x0, int y0, int z0, bf16 *lds) {
+  constexpr int PIECES = CC / 8, STRIDE = CC + 8;
+  for (int s = threadIdx.x; s < CV_FN * PIECES; s += 256) {
+    const int vox = s / PIECES, pc = s - vox * PIECES;
+    const int fx = vox % CV_FX, r2 = vox / CV_FX, fy = r2 % CV_FY, fz = r2 / CV_FY;
+    const int gx = x0 + fx - 1, gy = y0 + fy - 1, gz = z0 + fz - 1;
+    uint4 val = make_uint4(0u, 0u, 0u, 0u);
+    if (gx >= 0 && gx < p.nx && gy >= 0 && gy < p.ny && gz >= 0 && gz < p.nz) {
+      size_t v;
+      if (FROMA)
+        v = (size_t)(gx >> 1) + (size_t)p.anx * ((size_t)(gy >> 1) + (size_t)p.any * (size_t)(gz >> 1));
+      else
+        v = (size_t)gx + (size_t)p.nx * ((size_t)gy + (size_t)p.ny * (size_t)gz);
+      val = *reinterpret_cast<const uint4 *>((FROMA ? p.a : p.b) + v * CSRC + c0 + 8 * pc);
+    }
+    *reinterpret_cast<uint4 *>(lds + vox * STRIDE + 8 * pc) = val;
+  }
+}
+
+// MFMAs of one staged chunk: NG K32 groups (27 taps of 32 channels, or 14
+// tap pairs of 16 channels).  With one 16-row output block (MB = 1) every
+// weight fragment of the chunk is loaded into registers up front and the
+// group loop is fully unrolled, so the loop body is LDS reads and MFMAs only.
+template <int CC, int MB, int COUT>
+__device__ __forceinline__ void conv_chunk(const bf16 *__restrict__ wp, const bf16 *lds, f32x4 (&acc)[MB][4], int w,
+                                           int j, int col) {
+  constexpr int NG = CC == 32 ? 27 : 14, STRIDE = CC + 8;
+  // output voxel (col, w, nb) sits at footprint (col+1, w+1, nb+1); tap t reads
+  // footprint (col+kx, w+ky, nb+kz) = base + nb*FX*FY + tap_off(t)
+  const int base = w * CV_FX + col;
+  auto group = [&](int g, const bf16x8 (&A)[MB]) {
+    int off, ch;
+    if (CC == 32) {
+      off = tap_off(g);
+      ch = 8 * j;
+    } else {
+      int t = 2 * g + (j >> 1);
+      t = t > 26 ? 26 : t;  // the 28th tap has zero weights
+      off = tap_off(t);
+      ch = 8 * (j & 1);
+    }
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++) {
+      const bf16x8 B = *reinterpret_cast<const bf16x8 *>(lds + (base + nb * (CV_FX * CV_FY) + off) * STRIDE + ch);
+#pragma unroll
+      for (int m = 0; m < MB; m++) acc[m][nb] = mfma16x16x32(A[m], B, acc[m][nb]);
+    }
+  };
+  if constexpr (MB == 1) {
+    // batches of BS groups, the next batch's weight fragments loaded while
+    // the current batch computes
+    constexpr int BS = CC == 32 ? 9 : 7, NBATCH = NG / BS;
+    static_assert(NBATCH * BS == NG, "group batches");
+    bf16x8 Acur[BS][1], Anext[BS][1];
+#pragma unroll
+    for (int g = 0; g < BS; g++)
+      Acur[g][0] = *reinterpret_cast<const bf16x8 *>(wp + ((size_t)g * COUT + col) * 32 + 8 * j);
+#pragma unroll 1
+    for (int bt = 0; bt < NBATCH; bt++) {
+      if (bt + 1 < NBATCH) {
+#pragma unroll
+        for (int g = 0; g < BS; g++)
+          Anext[g][0] =
+              *reinterpret_cast<const bf16x8 *>(wp + ((size_t)((bt + 1) * BS + g) * COUT + col) * 32 + 8 * j);
+      }
+#pragma unroll
+      for (int g = 0; g < BS; g++) group(bt * BS + g, Acur[g]);
+      if (bt + 1 < NBATCH) {
+#pragma unroll
+        for (int g = 0; g < BS; g++) Acur[g][0] = Anext[g][0];
+      }
+    }
+  } else {
+#pragma unroll 3
+    for (int g = 0; g < NG; g++) {
+      bf16x8 A[MB];
+#pragma unroll
+      for (int m = 0; m < MB; m++)
+        A[m] = *reinterpret_cast<const bf16x8 *>(wp + ((size_t)g * COUT + m * 16 + col) * 32 + 8 * j);
+      group(g, A);
+    }
+  }
+}
+
 template <int CA, int CB, int COUT, bool RELU>
-__global__ __launch_bounds__(256) void k_conv3(ConvArgs p) {
+__global__ __launch_bounds__(256, 3) void k_conv3(ConvArgs p) {
   constexpr int MB = COUT / 16;
   constexpr int NCA = CA / 32;
   constexpr int NCB = CB == 16 ? 1 : CB / 32;
@@ -117,57 +209,22 @@ __global__ __launch_bounds__(256) void k_conv3(ConvArgs p) {
   for (int m = 0; m < MB; m++)
 #pragma unroll
     for (int nb = 0; nb < 4; nb++) acc[m][nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-  int G0 = 0;
+  const bf16 *wp = p.wp;
 #pragma unroll 1
-  for (int q = 0; q < NCA + NCB; q++) {
-    const bool froma = q < NCA;
-    const int cc = froma ? 32 : CCB;
-    const int stride = cc + 8;  // LDS voxel stride in bf16 (16-byte multiple)
-    const int pieces = cc / 8;
+  for (int q = 0; q < NCA; q++) {
     __syncthreads();
-    for (int s = threadIdx.x; s < CV_FN * pieces; s += 256) {
-      const int vox = s / pieces, pc = s - vox * pieces;
-      const int fx = vox % CV_FX, r2 = vox / CV_FX, fy = r2 % CV_FY, fz = r2 / CV_FY;
-      const int gx = x0 + fx - 1, gy = y0 + fy - 1, gz = z0 + fz - 1;
-      uint4 val = make_uint4(0u, 0u, 0u, 0u);
-      if (gx >= 0 && gx < p.nx && gy >= 0 && gy < p.ny && gz >= 0 && gz < p.nz) {
-        if (froma) {
-          const size_t u = (size_t)(gx >> 1) + (size_t)p.anx * ((size_t)(gy >> 1) + (size_t)p.any * (size_t)(gz >> 1));
-          val = *reinterpret_cast<const uint4 *>(p.a + u * CA + 32 * q + 8 * pc);
-        } else {
-          const size_t v = (size_t)gx + (size_t)p.nx * ((size_t)gy + (size_t)p.ny * (size_t)gz);
-          val = *reinterpret_cast<const uint4 *>(p.b + v * CB + 32 * (q - NCA) + 8 * pc);
-        }
-      }
-      *reinterpret_cast<uint4 *>(lds + vox * stride + 8 * pc) = val;
-    }
+    conv_stage<32, CA, true>(p, 32 * q, x0, y0, z0, lds);
     __syncthreads();
-    const int ngroups = cc == 32 ? 27 : 14;
+    conv_chunk<32, MB, COUT>(wp, lds, acc, w, j, col);
+    wp += (size_t)27 * COUT * 32;
+  }
 #pragma unroll 1
-    for (int g = 0; g < ngroups; g++) {
-      int tap, ch;
-      if (cc == 32) {
-        tap = g;
-        ch = 8 * j;
-      } else {
-        tap = 2 * g + (j >> 1);
-        tap = tap > 26 ? 26 : tap;  // the 28th tap has zero weights
-        ch = 8 * (j & 1);
-      }
-      const int kx = tap % 3, ky = (tap / 3) % 3, kz = tap / 9;
-      bf16x8 A[MB];
-#pragma unroll
-      for (int m = 0; m < MB; m++)
-        A[m] = *reinterpret_cast<const bf16x8 *>(p.wp + ((size_t)(G0 + g) * COUT + m * 16 + col) * 32 + 8 * j);
-#pragma unroll
-      for (int nb = 0; nb < 4; nb++) {
-        const int fidx = (nb + kz) * (CV_FX * CV_FY) + (w + ky) * CV_FX + (col + kx);
-        const bf16x8 B = *reinterpret_cast<const bf16x8 *>(lds + fidx * stride + ch);
-#pragma unroll
-        for (int m = 0; m < MB; m++) acc[m][nb] = mfma16x16x32(A[m], B, acc[m][nb]);
-      }
-    }
-    G0 += ngroups;
+  for (int q = 0; q < NCB; q++) {
+    __syncthreads();
+    conv_stage<CCB, CB, false>(p, 32 * q, x0, y0, z0, lds);
+    __syncthreads();
+    conv_chunk<CCB, MB, COUT>(wp, lds, acc, w, j, col);
+    wp += (size_t)(CCB == 32 ? 27 : 14) * COUT * 32;
   }
   const int gx = x0 + col, gy = y0 + w;
   if (gx >= p.nx || gy >= p.ny) return;
@@ -223,41 +280,83 @@ bool launch_conv3(const G3 &g, const bf16 *a, const G3 &ga, int ca, const bf16 *
 }
 
 // ------------------------------------------------------------ first layer, head, pool
+// First layer on MFMA: 27 taps x 2 channels = 54 of the K = 64 of two K32
+// groups (group g holds taps 16g .. 16g+15, channel-minor; taps past 26 have
+// zero weights).  Lane group j of a B fragment holds 4 taps x 2 channels, one
+// 4-byte LDS word per tap.  Packed weights: [2][16][32].
+__global__ __launch_bounds__(256) void k_pack_conv_in(const float *__restrict__ W, bf16 *__restrict__ wp) {
+  const int e = threadIdx.x + 256 * blockIdx.x;
+  if (e >= 2 * 16 * 32) return;
+  const int kk = e % 32, co = (e / 32) % 16, g = e / 512;
+  const int tap = 16 * g + kk / 2, ci = kk % 2;
+  wp[e] = tobf(tap < 27 ? W[(co * 2 + ci) * 27 + tap] : 0.0f);
+}
+void launch_pack_conv_in(const float *W, bf16 *wp, hipStream_t s) { k_pack_conv_in<<<4, 256, 0, s>>>(W, wp); }
+
 __global__ __launch_bounds__(256) void k_conv_in(int nx, int ny, int nz, const bf16 *__restrict__ x0,
-                                                 const bf16 *__restrict__ w, const float *__restrict__ bias,
+                                                 const bf16 *__restrict__ wp, const float *__restrict__ bias,
                                                  bf16 *__restrict__ out) {
-  __shared__ float sw[16 * 2 * 27];
-  for (int i = threadIdx.x; i < 16 * 2 * 27; i += 256) sw[i] = bf(w[i]);
+  __shared__ uint32_t lds[CV_FN];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int j = lane >> 4, col = lane & 15;
+  const int ntx = (nx + CV_TX - 1) / CV_TX, nty = (ny + CV_TY - 1) / CV_TY;
+  const int bid = xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = bid % ntx, rr = bid / ntx, ty = rr % nty, tz = rr / nty;
+  const int x0v = tx * CV_TX, y0 = ty * CV_TY, z0 = tz * CV_TZ;
+  const uint32_t *src = reinterpret_cast<const uint32_t *>(x0);
+  for (int s = threadIdx.x; s < CV_FN; s += 256) {
+    const int fx = s % CV_FX, r2 = s / CV_FX, fy = r2 % CV_FY, fz = r2 / CV_FY;
+    const int gx = x0v + fx - 1, gy = y0 + fy - 1, gz = z0 + fz - 1;
+    uint32_t v = 0u;
+    if (gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz)
+      v = src[(size_t)gx + (size_t)nx * ((size_t)gy + (size_t)ny * (size_t)gz)];
+    lds[s] = v;
+  }
+  bf16x8 A[2];
+#pragma unroll
+  for (int g = 0; g < 2; g++) A[g] = *reinterpret_cast<const bf16x8 *>(wp + (g * 16 + col) * 32 + 8 * j);
   __syncthreads();
-  const uint32_t n = (uint32_t)nx * ny * nz;
-  const uint32_t v = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * 256u + threadIdx.x;
-  if (v >= n) return;
-  const int x = (int)(v % (uint32_t)nx), r = (int)(v / (uint32_t)nx), y = r % ny, z = r / ny;
-  float acc[16];
+  f32x4 acc[4];
 #pragma unroll
-  for (int co = 0; co < 16; co++) acc[co] = bias[co];
-  for (int t = 0; t < 27; t++) {
-    const int xx = x + t % 3 - 1, yy = y + (t / 3) % 3 - 1, zz = z + t / 9 - 1;
-    if (xx < 0 || xx >= nx || yy < 0 || yy >= ny || zz < 0 || zz >= nz) continue;
-    const size_t u = (size_t)xx + (size_t)nx * ((size_t)yy + (size_t)ny * (size_t)zz);
-    const float i0 = bf(x0[2 * u]), i1 = bf(x0[2 * u + 1]);
+  for (int nb = 0; nb < 4; nb++) acc[nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  const int base = w * CV_FX + col;
 #pragma unroll
-    for (int co = 0; co < 16; co++) {
-      acc[co] += sw[(co * 2 + 0) * 27 + t] * i0;
-      acc[co] += sw[(co * 2 + 1) * 27 + t] * i1;
+  for (int g = 0; g < 2; g++) {
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++) {
+      uint4 u;
+      uint32_t t4[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        int t = 16 * g + 4 * j + i;
+        t = t > 26 ? 26 : t;  // zero weights beyond the 27 taps
+        t4[i] = lds[base + nb * (CV_FX * CV_FY) + tap_off(t)];
+      }
+      u.x = t4[0]; u.y = t4[1]; u.z = t4[2]; u.w = t4[3];
+      acc[nb] = mfma16x16x32(A[g], __builtin_bit_cast(bf16x8, u), acc[nb]);
     }
   }
-  bf16x8 o0, o1;
+  const int gx = x0v + col, gy = y0 + w;
+  if (gx >= nx || gy >= ny) return;
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
-    o0[i] = tobf(acc[i] > 0.0f ? acc[i] : 0.0f);
-    o1[i] = tobf(acc[8 + i] > 0.0f ? acc[8 + i] : 0.0f);
+  for (int nb = 0; nb < 4; nb++) {
+    const int gz = z0 + nb;
+    if (gz >= nz) continue;
+    const size_t v = (size_t)gx + (size_t)nx * ((size_t)gy + (size_t)ny * (size_t)gz);
+    const int co = 4 * j;
+    bf16x4 o;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const float t = acc[nb][i] + bias[co + i];
+      o[i] = tobf(t > 0.0f ? t : 0.0f);
+    }
+    *reinterpret_cast<bf16x4 *>(out + v * 16 + co) = o;
   }
-  *reinterpret_cast<bf16x8 *>(out + 16 * (size_t)v) = o0;
-  *reinterpret_cast<bf16x8 *>(out + 16 * (size_t)v + 8) = o1;
 }
-void launch_conv_in(const G3 &g, const bf16 *x0, const bf16 *w, const float *bias, bf16 *out, hipStream_t s) {
-  k_conv_in<<<nblk(g.n()), 256, 0, s>>>(g.nx, g.ny, g.nz, x0, w, bias, out);
+void launch_conv_in(const G3 &g, const bf16 *x0, const bf16 *wp, const float *bias, bf16 *out, hipStream_t s) {
+  const unsigned nb = (unsigned)(((g.nx + CV_TX - 1) / CV_TX) * ((g.ny + CV_TY - 1) / CV_TY) *
+                                 ((g.nz + CV_TZ - 1) / CV_TZ));
+  k_conv_in<<<nb, 256, 0, s>>>(g.nx, g.ny, g.nz, x0, wp, bias, out);
 }
 
 __global__ __launch_bounds__(256) void k_head(uint32_t n, const bf16 *__restrict__ d0, const float *__restrict__ wh,

@@ -150,7 +150,7 @@ int work_alloc(Work &W, const G3 &g, bool train) {
 }
 
 void repack(PyroNet *N) {
-  launch_pack_bf16(N->params + N->w0a, 16 * 2 * 27, N->w0a_bf, N->stream);
+  launch_pack_conv_in(N->params + N->w0a, N->w0a_bf, N->stream);
   for (int l = 0; l < NCONV; l++) {
     launch_pack_conv(N->params + N->w[l], kSpec[l], false, N->wp[l], N->stream);
     launch_pack_conv(N->params + N->w[l], kSpec[l], true, N->wpt[l], N->stream);
@@ -320,7 +320,7 @@ int pyro_net_create(uint64_t seed, PyroNet **out) {
   if (!rc) rc |= dalloc(N->allocs, &N->grads, o);
   if (!rc) rc |= dalloc(N->allocs, &N->am, o);
   if (!rc) rc |= dalloc(N->allocs, &N->av, o);
-  if (!rc) rc |= dalloc(N->allocs, &N->w0a_bf, 16 * 2 * 27);
+  if (!rc) rc |= dalloc(N->allocs, &N->w0a_bf, 2 * 16 * 32);
   for (int l = 0; l < NCONV && !rc; l++) {
     const ConvSpec &c = kSpec[l];
     rc |= dalloc(N->allocs, &N->wp[l], (size_t)conv_groups(c.ca, c.cb) * c.cout * 32);
@@ -618,7 +618,8 @@ int pyro_debug_conv(int ca, int cb, int cout, int transposed, int relu, uint32_t
   const G3 g{(int)nx, (int)ny, (int)nz}, gh = half_of(g);
   int rc = check_dims(nx, ny, nz);
   if (rc) return rc;
-  bool known = false;
+  const bool first = ca == 0 && cb == 2 && cout == 16 && !transposed && relu && bias;
+  bool known = first;
   for (int l = 0; l < NCONV; l++)
     if (kSpec[l].ca == ca && kSpec[l].cb == cb && kSpec[l].cout == cout) known = true;
   if (!known || !b || !W || !out || (ca && !a && !transposed)) return nfail(PYRO_E_INVALID, "unsupported conv");
@@ -627,7 +628,7 @@ int pyro_debug_conv(int ca, int cb, int cout, int transposed, int relu, uint32_t
   bf16 *da = nullptr, *dbb, *wp, *dout;
   float *tmp, *dW, *dbias = nullptr;
   const size_t n = g.n(), nh = gh.n();
-  const size_t nwp = (size_t)(transposed ? conv_groups(0, cout) * c.cin() : conv_groups(ca, cb) * cout) * 32;
+  const size_t nwp = first ? 1024 : (size_t)(transposed ? conv_groups(0, cout) * c.cin() : conv_groups(ca, cb) * cout) * 32;
   const size_t big = std::max(std::max(n * in_b, nh * (size_t)(ca ? ca : 1)), n * out_c);
   rc = dalloc(L, &dbb, n * in_b);
   if (!rc && ca) rc = dalloc(L, &da, nh * ca);
@@ -647,9 +648,15 @@ int pyro_debug_conv(int ca, int cb, int cout, int transposed, int relu, uint32_t
     rc = PYRO_E_HIP;
   if (!rc && dbias && hipMemcpy(dbias, bias, cout * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) rc = PYRO_E_HIP;
   if (!rc) {
-    launch_pack_conv(dW, c, transposed != 0, wp, nullptr);
-    const bool ok = transposed ? launch_conv3(g, nullptr, gh, 0, dbb, cout, wp, nullptr, c.cin(), false, dout, nullptr)
-                               : launch_conv3(g, da, gh, ca, dbb, cb, wp, dbias, cout, relu != 0, dout, nullptr);
+    bool ok = true;
+    if (first) {
+      launch_pack_conv_in(dW, wp, nullptr);
+      launch_conv_in(g, dbb, wp, dbias, dout, nullptr);
+    } else {
+      launch_pack_conv(dW, c, transposed != 0, wp, nullptr);
+      ok = transposed ? launch_conv3(g, nullptr, gh, 0, dbb, cout, wp, nullptr, c.cin(), false, dout, nullptr)
+                      : launch_conv3(g, da, gh, ca, dbb, cb, wp, dbias, cout, relu != 0, dout, nullptr);
+    }
     if (!ok) rc = PYRO_E_INVALID;
     launch_bf16_to_f32((uint32_t)(n * out_c), dout, tmp, nullptr);
     if (hipDeviceSynchronize() != hipSuccess) rc = PYRO_E_HIP;

@@ -53,8 +53,10 @@ void launch_pack_bf16(const float *w, uint32_t n, bf16 *out, hipStream_t s);
 // nothing launched) for a channel combination the network does not use
 bool launch_conv3(const G3 &g, const bf16 *a, const G3 &ga, int ca, const bf16 *b, int cb, const bf16 *wp,
                   const float *bias, int cout, bool relu, bf16 *out, hipStream_t s);
-// first layer (2 input channels, VALU): a0 = relu(b + W * x0), W as BF16 [16][2][27]
-void launch_conv_in(const G3 &g, const bf16 *x0, const bf16 *w, const float *bias, bf16 *out, hipStream_t s);
+// first layer (2 input channels): a0 = relu(b + W * x0) on MFMA, weights
+// packed by launch_pack_conv_in into [2][16][32] BF16
+void launch_pack_conv_in(const float *W, bf16 *wp, hipStream_t s);
+void launch_conv_in(const G3 &g, const bf16 *x0, const bf16 *wp, const float *bias, bf16 *out, hipStream_t s);
 // head: y[v] = bh + sum_c wh[c] * d0[v][c]
 void launch_head(const G3 &g, const bf16 *d0, const float *wh, const float *bh, float *y, hipStream_t s);
 void launch_pool(const G3 &g, const bf16 *in, int c, bf16 *out, hipStream_t s);

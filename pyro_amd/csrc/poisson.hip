@@ -134,20 +134,28 @@ int learned_precond(PyroPoisson *P, const double *r, double rr, double *z, bool 
   return 0;
 }
 
-// number of fluid cells (integer atomics: exact and order-independent)
+// number of fluid cells: per-wave ballots summed in LDS, one integer atomic
+// per block (exact and order-independent)
+constexpr int COUNT_BLOCKS = 512;
 __global__ __launch_bounds__(256) void k_count_fluid(uint32_t n, const uint8_t *__restrict__ f,
                                                      unsigned int *__restrict__ cnt) {
-  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-  const bool fl = i < n && f[i] != 1;
-  const unsigned long long m = __ballot(fl);
-  if ((threadIdx.x & 63u) == 0 && m) atomicAdd(cnt, (unsigned int)__popcll(m));
+  unsigned int c = 0;
+  for (uint32_t base = blockIdx.x * 256u; base < n; base += COUNT_BLOCKS * 256u) {
+    const uint32_t i = base + threadIdx.x;
+    const unsigned long long m = __ballot(i < n && f[i] != 1);
+    c += (unsigned int)__popcll(m);
+  }
+  __shared__ unsigned int sh[4];
+  if ((threadIdx.x & 63u) == 0) sh[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(cnt, sh[0] + sh[1] + sh[2] + sh[3]);
 }
 
 int set_flags_device(PyroPoisson *P, const uint8_t *sim_flags) {
   k_flags_from_sim<<<nblocks(P->n), 256, 0, P->stream>>>(P->n, sim_flags, P->pc[0].f);
   launch_mg_setup(P->pc, P->nl_pc, P->stream);
   PCHK(hipMemsetAsync(P->count, 0, sizeof(unsigned int), P->stream));
-  k_count_fluid<<<nblocks(P->n), 256, 0, P->stream>>>(P->n, P->pc[0].f, P->count);
+  k_count_fluid<<<COUNT_BLOCKS, 256, 0, P->stream>>>(P->n, P->pc[0].f, P->count);
   unsigned int c = 0;
   PCHK(hipMemcpyAsync(&c, P->count, sizeof(c), hipMemcpyDeviceToHost, P->stream));
   int rc = sync(P);

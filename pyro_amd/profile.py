@@ -3,8 +3,11 @@
 Builds the pool scene at 2 cm (160 x 320 x 160), runs N full steps (every
 kernel of the step appears in the trace), then R launches of the level-0
 red-black sweep and R launches of the plain copy kernel over 64 Mi floats
-(the calibration for FETCH_SIZE / WRITE_SIZE counters).  Prints one line per
-phase so a profiled run is never silent.
+(the calibration for FETCH_SIZE / WRITE_SIZE counters).  Optional phases:
+``--solves K`` K MGPCG solves to 1e-4 of the next step's pressure problem
+(warm start); ``--unet F`` F U-Net forward passes on the same grid;
+``--train T`` T training steps of the U-Net on 8 pairs of the pool at 4 cm.
+Prints one line per phase so a profiled run is never silent.
 """
 from __future__ import annotations
 
@@ -25,6 +28,9 @@ def main() -> None:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--sweeps", type=int, default=20)
     ap.add_argument("--copies", type=int, default=10)
+    ap.add_argument("--solves", type=int, default=0)
+    ap.add_argument("--unet", type=int, default=0)
+    ap.add_argument("--train", type=int, default=0)
     args = ap.parse_args()
     s = pa.Scene.load(str(ROOT / "pyro_amd" / "scenes" / "pool.toml"))
     s.set("domain.cell", args.cell)
@@ -38,6 +44,31 @@ def main() -> None:
     print(f"[profile] {args.sweeps} level-0 sweeps: {ms * 1e3:.1f} us each", flush=True)
     ms = pa.sim.debug_copy(64 << 20, args.copies)
     print(f"[profile] {args.copies} copies of 256 MiB: {ms * 1e3:.1f} us each", flush=True)
+    nx, ny, nz = s.grid().dims
+    if args.solves:
+        from pyro_amd.poisson import Poisson
+
+        sim.begin_step(0.002, False)
+        for stg in pa.STAGES[: pa.STAGES.index("divergence") + 1]:
+            sim.run_stage(stg)
+        P = Poisson(sim.read_field(pa.level_field(0, "flags")), s.grid().dx)
+        for k in range(args.solves):
+            P.set_from_sim(sim, warm=True, flags=k == 0)
+            info = P.solve("mgpcg", 1e-4, 100)
+        print(f"[profile] {args.solves} MGPCG solves to 1e-4: {info}", flush=True)
+    if args.unet or args.train:
+        from pyro_amd.net import Net
+
+        net = Net(seed=0)
+        if args.unet:
+            ms = net.time_forward((nz, ny, nx), args.unet)
+            print(f"[profile] {args.unet} U-Net forwards on {nx}x{ny}x{nz}: {ms:.3f} ms each", flush=True)
+        if args.train:
+            from pyro_amd.learned import collect_pool_pairs
+
+            collect_pool_pairs(net, frames=8)
+            losses = net.train(args.train, lr=1e-3)
+            print(f"[profile] {args.train} training steps at 4 cm: last loss {losses[-1]:.4f}", flush=True)
 
 
 if __name__ == "__main__":

@@ -66,6 +66,17 @@ def test_conv_kinds_bitexact(kind):
     assert np.array_equal(db, rb.astype(np.float32))
 
 
+def test_first_layer_forward_bitexact():
+    rng = np.random.default_rng(12)
+    shape = (9, 7, 21)
+    b = _ints(rng, shape + (2,))
+    W = _ints(rng, (16, 2, 27), -1, 2)
+    bias = _ints(rng, (16,), -3, 4)
+    got = N.debug_conv(0, 2, 16, None, b, W, bias, relu=True)
+    ref = U.conv_layer(None, b, W, bias, relu_on=True)
+    assert np.array_equal(got, ref.astype(np.float32))
+
+
 def test_first_layer_dw_bitexact():
     rng = np.random.default_rng(11)
     shape = (6, 10, 18)
