@@ -100,6 +100,13 @@ int pyro_sim_sync(PyroSim *sim);
 /* Device pointer of a field for interop (renderer hand-off); *bytes = size. */
 int pyro_sim_device_ptr(PyroSim *sim, int field, void **dptr, size_t *bytes);
 
+/* Checkpoint (no reference counterpart): everything a step reads -- vel,
+ * s0, s1, sol, the level-0 pressure warm start, Params, time, frame -- in one
+ * binary file.  load requires a simulation of the same grid and restores the
+ * saved Params; the continuation is bit-identical to the uninterrupted run. */
+int pyro_sim_save(PyroSim *sim, const char *path);
+int pyro_sim_load(PyroSim *sim, const char *path);
+
 /* Fine-grained hooks (per-kernel parity tests, profiling). begin_step sets
  * dt/time/ign_on exactly as step does; run_stage enqueues one stage with the
  * current ping-pong roles; end_step finalises stats and swaps roles. */

@@ -66,6 +66,8 @@ def hip() -> C.CDLL:
         lib.pyro_sim_time_kernel.argtypes = [P, i32, i32, i32, C.POINTER(f32)]
         lib.pyro_sim_add_velocity_noise.argtypes = [P, f32, u32]
         lib.pyro_sim_check_finite.argtypes = [P]
+        lib.pyro_sim_save.argtypes = [P, C.c_char_p]
+        lib.pyro_sim_load.argtypes = [P, C.c_char_p]
         lib.pyro_debug_math.argtypes = [i32, P, P, P, sz]
         lib.pyro_debug_copy.argtypes = [sz, i32, C.POINTER(f32)]
         _hip = lib

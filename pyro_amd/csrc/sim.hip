@@ -9,6 +9,7 @@
 // ign_on and frame into the device copy of Params (the reference's 576-byte
 // write_buffer, sim.rs:250-251).  Graphs are keyed by the ping-pong roles
 // (vel[cur], sol[sol_cur]) and by want_stats, so at most 8 exist.
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -428,6 +429,105 @@ int pyro_sim_write_field(PyroSim *s, int field, const void *src, size_t bytes) {
     return sync_check(s);
   }
   return 0;
+}
+
+// ---------------------------------------------------------------- checkpoint
+// A step reads only vel[cur], s0, s1, sol[sol_cur] (flags follow from it),
+// level-0 phi (the warm start), time and frame; everything else is written
+// before it is read.  The file: a fixed header (magic, version, dims, the
+// 576-byte Params, time, frame, cycles, ign_until) followed by the seven
+// fields as raw little-endian FP32 in the device layout.
+}  // extern "C"
+namespace {
+struct CkptHeader {
+  char magic[8];
+  uint32_t version, nx, ny, nz, params_bytes;
+  float time, ign_until;
+  uint32_t frame, cycles, nfields, pad;
+};
+constexpr char kCkptMagic[8] = {'P', 'Y', 'R', 'O', 'C', 'K', 'P', 'T'};
+
+struct CkptField {
+  void *ptr;
+  size_t bytes;
+};
+std::vector<CkptField> ckpt_fields(PyroSim *s) {
+  const size_t n4 = (size_t)s->g.n * 16, nv = (size_t)s->g.nv * 4;
+  return {{s->vel[s->cur][0], nv}, {s->vel[s->cur][1], nv}, {s->vel[s->cur][2], nv}, {s->s0, n4},
+          {s->s1, n4},           {s->sol[s->sol_cur], n4}, {s->lv[0].phi, (size_t)s->g.n * 4}};
+}
+}  // namespace
+extern "C" {
+
+int pyro_sim_save(PyroSim *s, const char *path) {
+  if (!s || !path) return fail(PYRO_E_INVALID, "null argument");
+  if (s->in_step) return fail(PYRO_E_STATE, "save inside a step");
+  int rc = sync_check(s);
+  if (rc) return rc;
+  CkptHeader h{};
+  std::memcpy(h.magic, kCkptMagic, 8);
+  h.version = 1;
+  h.nx = s->hp.nx; h.ny = s->hp.ny; h.nz = s->hp.nz;
+  h.params_bytes = (uint32_t)sizeof(Params);
+  h.time = s->time; h.ign_until = s->ign_until;
+  h.frame = s->frame; h.cycles = s->cycles;
+  const std::vector<CkptField> fl = ckpt_fields(s);
+  h.nfields = (uint32_t)fl.size();
+  std::FILE *f = std::fopen(path, "wb");
+  if (!f) return fail(PYRO_E_INVALID, std::string("cannot open ") + path + " for writing");
+  bool ok = std::fwrite(&h, sizeof h, 1, f) == 1 && std::fwrite(&s->hp, sizeof(Params), 1, f) == 1;
+  std::vector<char> buf;
+  for (const CkptField &c : fl) {
+    if (!ok) break;
+    buf.resize(c.bytes);
+    ok = hipMemcpy(buf.data(), c.ptr, c.bytes, hipMemcpyDeviceToHost) == hipSuccess &&
+         std::fwrite(buf.data(), 1, c.bytes, f) == c.bytes;
+  }
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) return fail(PYRO_E_HIP, std::string("writing checkpoint ") + path + " failed");
+  return 0;
+}
+
+int pyro_sim_load(PyroSim *s, const char *path) {
+  if (!s || !path) return fail(PYRO_E_INVALID, "null argument");
+  if (s->in_step) return fail(PYRO_E_STATE, "load inside a step");
+  int rc = sync_check(s);
+  if (rc) return rc;
+  std::FILE *f = std::fopen(path, "rb");
+  if (!f) return fail(PYRO_E_INVALID, std::string("cannot open ") + path);
+  CkptHeader h{};
+  Params hp;
+  const std::vector<CkptField> fl = ckpt_fields(s);
+  std::string err;
+  if (std::fread(&h, sizeof h, 1, f) != 1 || std::memcmp(h.magic, kCkptMagic, 8) != 0)
+    err = "not a checkpoint file";
+  else if (h.version != 1)
+    err = "unsupported checkpoint version " + std::to_string(h.version);
+  else if (h.params_bytes != sizeof(Params) || h.nfields != fl.size())
+    err = "checkpoint layout mismatch";
+  else if (h.nx != s->hp.nx || h.ny != s->hp.ny || h.nz != s->hp.nz)
+    err = "checkpoint grid " + std::to_string(h.nx) + "x" + std::to_string(h.ny) + "x" + std::to_string(h.nz) +
+          " does not match the simulation's";
+  else if (std::fread(&hp, sizeof(Params), 1, f) != 1)
+    err = "truncated checkpoint";
+  std::vector<std::vector<char>> data(fl.size());
+  for (size_t k = 0; k < fl.size() && err.empty(); k++) {
+    data[k].resize(fl[k].bytes);
+    if (std::fread(data[k].data(), 1, fl[k].bytes, f) != fl[k].bytes) err = "truncated checkpoint";
+  }
+  std::fclose(f);
+  if (!err.empty()) return fail(PYRO_E_INVALID, err + " (" + path + ")");
+  // nothing on the device changes before the whole file has been read
+  s->hp = hp;
+  HIPCHK(hipMemcpy(s->dP, &s->hp, sizeof(Params), hipMemcpyHostToDevice));
+  for (size_t k = 0; k < fl.size(); k++) HIPCHK(hipMemcpy(fl[k].ptr, data[k].data(), fl[k].bytes, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(s->sol[1 - s->sol_cur], data[5].data(), fl[5].bytes, hipMemcpyHostToDevice));
+  launch_init_flags(s->dP, s->g, s->sol[s->sol_cur], s->flags, s->stream);
+  s->time = h.time;
+  s->frame = h.frame;
+  s->cycles = h.cycles;
+  s->ign_until = h.ign_until;
+  return sync_check(s);
 }
 
 int pyro_sim_info(const PyroSim *s, uint32_t dims[3], uint64_t *cells, double *time, uint32_t *frame,

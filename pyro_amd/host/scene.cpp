@@ -7,6 +7,7 @@
 // contracts), so the Params blob and SOL array are the ones scene.rs produces.
 #include "pyro_scene.h"
 
+#include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -793,6 +794,19 @@ int pyro_scene_get(const PyroScene *sc, const char *key, double *out) {
   else if (k == "ignition.until") *out = s.has_ign ? s.ign.until : 0.0;  // sim.rs:204
   else if (k == "output.preroll") *out = s.preroll;
   else if (k == "objects") *out = (double)s.objects.size();
+  else if (k.rfind("object.", 0) == 0) {
+    // object.<i>.radius, object.<i>.pool (1 if kind == "pool"; main.rs:264-272 uses both)
+    const size_t dot = k.find('.', 7);
+    size_t i = 0;
+    bool ok = dot != std::string::npos && dot > 7;
+    for (size_t c = 7; ok && c < dot; c++) ok = std::isdigit((unsigned char)k[c]) != 0;
+    if (ok) i = (size_t)std::stoul(k.substr(7, dot - 7));
+    if (!ok || i >= s.objects.size()) { g_err = "unknown scene key " + k; return PYRO_SCENE_E_ARG; }
+    const std::string f = k.substr(dot + 1);
+    if (f == "radius") *out = s.objects[i].radius;
+    else if (f == "pool") *out = s.objects[i].kind == "pool" ? 1.0 : 0.0;
+    else { g_err = "unknown scene key " + k; return PYRO_SCENE_E_ARG; }
+  }
   else { g_err = "unknown scene key " + k; return PYRO_SCENE_E_ARG; }
   return 0;
 }

@@ -175,6 +175,14 @@ class Sim:
         _native.check_hip(_native.hip().pyro_sim_read_field(self._h, fid, out.ctypes.data, out.nbytes))
         return out
 
+    def save(self, path) -> None:
+        """Checkpoint everything a step reads (pyro_sim_save)."""
+        _native.check_hip(_native.hip().pyro_sim_save(self._h, str(path).encode()))
+
+    def load(self, path) -> None:
+        """Restore a checkpoint of a simulation on the same grid (pyro_sim_load)."""
+        _native.check_hip(_native.hip().pyro_sim_load(self._h, str(path).encode()))
+
     def write_field(self, name, arr: np.ndarray) -> None:
         fid = FIELDS[name] if isinstance(name, str) else int(name)
         shape, dt = self._shape(fid)

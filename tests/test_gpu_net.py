@@ -173,6 +173,27 @@ def test_training_lowers_loss_and_learned_solver_converges():
     assert rel(P.get()[fluid], phi[fluid]) < 1e-6
 
 
+def test_training_is_deterministic():
+    """same pairs, same seeds: bit-identical losses and weights (fixed-order reductions)"""
+    rng = np.random.default_rng(21)
+    shape = (8, 16, 16)
+    xs = []
+    for _ in range(3):
+        x0 = np.zeros(shape + (2,), np.float32)
+        x0[..., 0] = rng.normal(size=shape)
+        x0[..., 1] = 1.0
+        xs.append((x0, rng.normal(size=shape).astype(np.float32)))
+    runs = []
+    for _ in range(2):
+        net = N.Net(seed=4)
+        net.output_scale = 1.0
+        for x0, t in xs:
+            net.dataset_add(x0, t)
+        runs.append((net.train(20, lr=1e-3, seed=3), net.params()))
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert np.array_equal(runs[0][1], runs[1][1])
+
+
 def test_learned_solver_falls_back_to_vcycles():
     """A network whose output is zero is rejected; plain V-cycles finish the solve."""
     from pyro_amd.poisson import Poisson, make_c3_problem
