@@ -29,7 +29,9 @@ Extra fields:
                    the GPU, from 4 cm pool frames); and the standalone
                    256^3 case (random solid boxes, 32 Gaussian sources):
                    V-cycles to 1e-4, FP64-residual refinement and MGPCG to
-                   1e-6, with the error against an FP64 solution.
+                   1e-6, with the error against an FP64 solution; the
+                   bonfire at 10 cm stepped from one checkpointed mid-fire
+                   state with each pressure method to 1e-4.
 """
 from __future__ import annotations
 
@@ -122,7 +124,7 @@ def gpu_run(args):
             f"{learn['forward_tflops_pool_2cm']:.0f} TFLOP/s")
     return dict(cells=cells, wall=wall, stats=st, stage_ms=stage_ms, smooth_ms=smooth_ms,
                 coarse_ms=coarse_ms, achieved=achieved, algo_bytes=algo_bytes, dims=(nx, ny, nz),
-                copy_gbs=copy_gbs, frame=frame, learned=learn)
+                copy_gbs=copy_gbs, frame=frame, learned=learn, net=net)
 
 
 def _info(i):
@@ -246,6 +248,68 @@ def cpu_baseline(args, dims):
                        f"workload on the oracle (OpenMP, {threads} threads)")
 
 
+def bonfire_c5(args, net):
+    """The large-fire comparison: scenes/bonfire.toml at 10 cm (160 x 240 x
+    160), pre-rolled with the reference step, checkpointed, then `c5_steps`
+    steps per pressure method from that same state: the reference step
+    (fixed 4 V-cycles), and pressure solves to 1e-4 by the reference V-cycles
+    (at most 50), refinement, MGPCG and learned MGPCG; wall time per step,
+    solver iterations and the FP64 statistics (HRR, pyrolysis, solid mass)."""
+    import tempfile
+
+    import pyro_amd as pa
+    from pyro_amd.poisson import Poisson, step_solve
+
+    s = pa.Scene.load(str(ROOT / "pyro_amd" / "scenes" / "bonfire.toml"))
+    s.set("domain.cell", 0.1)
+    nx, ny, nz = s.grid().dims
+    dt = args.c5_dt
+    sim = pa.Sim(s.params(), s.voxelize(), None, 4, s.get("ignition.until"))
+    sim.add_velocity_noise(1e-3, 0)
+    t0 = time.perf_counter()
+    for k in range(args.c5_preroll):
+        sim.step(dt, False)
+    sim.sync()
+    log(f"[bench] bonfire {nx}x{ny}x{nz}: {args.c5_preroll} pre-roll steps in {time.perf_counter() - t0:.1f} s")
+    ck = Path(tempfile.mkdtemp()) / "bonfire.ckpt"
+    sim.save(ck)
+    out = {"grid": [nx, ny, nz], "cells": nx * ny * nz, "dt": dt, "preroll_steps": args.c5_preroll,
+           "steps": args.c5_steps, "tol": 1e-4}
+    P = Poisson(sim.read_field(pa.level_field(0, "flags")), s.grid().dx)
+    runs = [("reference_step", None, None), ("vcycle", "vcycle", 50), ("refine", "refine", 100),
+            ("mgpcg", "mgpcg", 100)]
+    if net is not None:
+        runs.append(("learned_first", "learned", 100))
+        P.attach_net(net, "first")
+    for key, method, cap in runs:
+        sim.load(ck)
+        its, worst, reached = [], 0.0, 0
+        sim.sync()
+        t0 = time.perf_counter()
+        for k in range(args.c5_steps):
+            stats = k % 10 == 9
+            if method is None:
+                sim.step(dt, stats)
+            else:
+                info = step_solve(sim, P, dt, method, 1e-4, cap, stats)
+                its.append(info.iterations)
+                worst = max(worst, info.rel_resid)
+                reached += info.rel_resid <= 1e-4
+        sim.sync()
+        wall = time.perf_counter() - t0
+        st = sim.read_stats()
+        d = {"ms_per_step": 1e3 * wall / args.c5_steps, "hrr_W": st.hrr, "pyrolysis_kg_s": st.pyro,
+             "solid_kg": st.solid, "max_T": st.max_t, "step_resid": st.resid}
+        if its:
+            d.update(mean_iterations=float(np.mean(its)), worst_rel_resid=worst,
+                     steps_reaching_tol=int(reached))
+        out[key] = d
+        log(f"[bench] bonfire {key}: {d}")
+    P.close()
+    ck.unlink()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -260,6 +324,9 @@ def main():
     ap.add_argument("--c3-cpu", action="store_true", help="also solve C3 with the FP64 CPU oracle (~1 min)")
     ap.add_argument("--train-pairs", type=int, default=2000, help="U-Net training pairs (pool, 4 cm)")
     ap.add_argument("--train-steps", type=int, default=3000, help="U-Net Adam steps")
+    ap.add_argument("--c5-steps", type=int, default=200, help="bonfire steps per pressure method (0: skip)")
+    ap.add_argument("--c5-preroll", type=int, default=300)
+    ap.add_argument("--c5-dt", type=float, default=0.005)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -269,6 +336,7 @@ def main():
         sys.exit(0)
     r = gpu_run(args)
     c3 = None if args.no_pressure else pressure_c3(args)
+    c5 = None if (args.no_pressure or args.c5_steps <= 0) else bonfire_c5(args, r.get("net"))
     cpu = None if args.no_cpu else cpu_baseline(args, r["dims"])
     nx, ny, nz = r["dims"]
     value = r["cells"] * args.steps / r["wall"]
@@ -313,7 +381,8 @@ def main():
         "cpu_baseline": cpu,
         "stage_ms": r["stage_ms"],
         "coarse_solve_ms": r["coarse_ms"],
-        "pressure": {"pool_frame": r["frame"], "standalone": c3, "learned_training": r["learned"]},
+        "pressure": {"pool_frame": r["frame"], "standalone": c3, "learned_training": r["learned"],
+                     "bonfire_10cm": c5},
         "copy_kernel_GBs": r["copy_gbs"],
         "last_stats": {"hrr_W": r["stats"].hrr, "max_T": r["stats"].max_t, "resid": r["stats"].resid},
     }
