@@ -243,34 +243,55 @@ __global__ __launch_bounds__(SW_NT) void k_mg_sweep(LvArgs a, int ntx, int nty, 
   const int x0 = tx * SW_TX - 4, y0 = ty * SW_TY - 2, z0 = tz * SW_TZ - 2;  // footprint origin, x0 % 4 == 0
   const int sy = a.d0, sz = a.d0 * a.d1;
   if (VEC) {
-    for (int s = threadIdx.x; s < SW_Q * SW_PY * SW_PZ; s += SW_NT) {
+    // every global load of the footprint is issued before the first LDS store
+    constexpr int NP = SW_Q * SW_PY * SW_PZ, IP = (NP + SW_NT - 1) / SW_NT;
+    constexpr int NBV = SW_Q * SW_BY * SW_BZ, IB = (NBV + SW_NT - 1) / SW_NT;
+    float4 pv[IP], bv[IB];
+    uint32_t cv[IP];
+#pragma unroll
+    for (int it = 0; it < IP; it++) {
+      const int s = threadIdx.x + it * SW_NT;
       const int q = s % SW_Q, row = s / SW_Q, ly = row % SW_PY, lz = row / SW_PY;
       const int gx = x0 + 4 * q, gy = y0 + ly, gz = z0 + lz;
-      float4 pv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      uint32_t cv = 0x40404040u;  // outside: never updated
-      if (gx >= 0 && gx < a.d0 && gy >= 0 && gy < a.d1 && gz >= 0 && gz < a.d2) {
+      pv[it] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      cv[it] = 0x40404040u;  // outside: never updated
+      if (s < NP && gx >= 0 && gx < a.d0 && gy >= 0 && gy < a.d1 && gz >= 0 && gz < a.d2) {
         const int g = gx + sy * gy + sz * gz;
-        pv = *reinterpret_cast<const float4 *>(pin + g);
-        cv = *reinterpret_cast<const uint32_t *>(nc + g);
-        if (cv & 0x80808080u) {
-          if (cv & 0x80u) pv.x = 0.0f;
-          if (cv & 0x8000u) pv.y = 0.0f;
-          if (cv & 0x800000u) pv.z = 0.0f;
-          if (cv & 0x80000000u) pv.w = 0.0f;
-        }
+        pv[it] = *reinterpret_cast<const float4 *>(pin + g);
+        cv[it] = *reinterpret_cast<const uint32_t *>(nc + g);
       }
-      *reinterpret_cast<float4 *>(sp + 4 * s) = pv;
-      *reinterpret_cast<uint32_t *>(sc + 4 * s) = cv;
     }
-    for (int s = threadIdx.x; s < SW_Q * SW_BY * SW_BZ; s += SW_NT) {
+#pragma unroll
+    for (int it = 0; it < IB; it++) {
+      const int s = threadIdx.x + it * SW_NT;
       const int q = s % SW_Q, row = s / SW_Q, ly = row % SW_BY, lz = row / SW_BY;
       const int gx = x0 + 4 * q, gy = y0 + 1 + ly, gz = z0 + 1 + lz;
-      float4 bv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      if (gx >= 0 && gx < a.d0 && gy >= 0 && gy < a.d1 && gz >= 0 && gz < a.d2) {
-        bv = *reinterpret_cast<const float4 *>(b + gx + sy * gy + sz * gz);
-        bv.x = a.hh * bv.x; bv.y = a.hh * bv.y; bv.z = a.hh * bv.z; bv.w = a.hh * bv.w;
+      bv[it] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (s < NBV && gx >= 0 && gx < a.d0 && gy >= 0 && gy < a.d1 && gz >= 0 && gz < a.d2)
+        bv[it] = *reinterpret_cast<const float4 *>(b + gx + sy * gy + sz * gz);
+    }
+#pragma unroll
+    for (int it = 0; it < IP; it++) {
+      const int s = threadIdx.x + it * SW_NT;
+      if (s >= NP) break;
+      float4 v = pv[it];
+      const uint32_t c = cv[it];
+      if (c & 0x80808080u) {
+        if (c & 0x80u) v.x = 0.0f;
+        if (c & 0x8000u) v.y = 0.0f;
+        if (c & 0x800000u) v.z = 0.0f;
+        if (c & 0x80000000u) v.w = 0.0f;
       }
-      *reinterpret_cast<float4 *>(sb + 4 * s) = bv;
+      *reinterpret_cast<float4 *>(sp + 4 * s) = v;
+      *reinterpret_cast<uint32_t *>(sc + 4 * s) = c;
+    }
+#pragma unroll
+    for (int it = 0; it < IB; it++) {
+      const int s = threadIdx.x + it * SW_NT;
+      if (s >= NBV) break;
+      float4 v = bv[it];
+      v.x = a.hh * v.x; v.y = a.hh * v.y; v.z = a.hh * v.z; v.w = a.hh * v.w;
+      *reinterpret_cast<float4 *>(sb + 4 * s) = v;
     }
   } else {
     for (int i = threadIdx.x; i < SW_PN; i += SW_NT) {
@@ -297,7 +318,10 @@ __global__ __launch_bounds__(SW_NT) void k_mg_sweep(LvArgs a, int ntx, int nty, 
   __syncthreads();
   // colour 0 on the tile + ring 1 (x in [3, 37)): one thread per x-pair
   constexpr int RP = (SW_TX + 2) / 2, RN = RP * SW_BY * SW_BZ;
-  for (int i = threadIdx.x; i < RN; i += SW_NT) {
+#pragma unroll
+  for (int it = 0; it < (RN + SW_NT - 1) / SW_NT; it++) {
+    const int i = threadIdx.x + it * SW_NT;
+    if (i >= RN) break;
     const int px = i % RP, row = i / RP, ly = row % SW_BY, lz = row / SW_BY;
     const int lx = 3 + 2 * px + ((3 + 2 * px + y0 + ly + 1 + z0 + lz + 1) & 1);  // (gx+gy+gz) even
     const int pi = ((lz + 1) * SW_PY + (ly + 1)) * SW_PX + lx;
@@ -308,7 +332,10 @@ __global__ __launch_bounds__(SW_NT) void k_mg_sweep(LvArgs a, int ntx, int nty, 
   __syncthreads();
   // colour 1 on the tile (x in [4, 36))
   constexpr int BP = SW_TX / 2, BN = BP * SW_TY * SW_TZ;
-  for (int i = threadIdx.x; i < BN; i += SW_NT) {
+#pragma unroll
+  for (int it = 0; it < (BN + SW_NT - 1) / SW_NT; it++) {
+    const int i = threadIdx.x + it * SW_NT;
+    if (i >= BN) break;
     const int px = i % BP, row = i / BP, ly = row % SW_TY, lz = row / SW_TY;
     const int lx = 4 + 2 * px + ((4 + 2 * px + y0 + ly + 2 + z0 + lz + 2 + 1) & 1);  // (gx+gy+gz) odd
     const int pi = ((lz + 2) * SW_PY + (ly + 2)) * SW_PX + lx;

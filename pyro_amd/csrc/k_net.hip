@@ -33,7 +33,14 @@ __global__ __launch_bounds__(64) void k_mfma_probe(const bf16 *a, const bf16 *b,
 void launch_mfma_probe(const bf16 *a, const bf16 *b, float *d, hipStream_t s) { k_mfma_probe<<<1, 64, 0, s>>>(a, b, d); }
 
 // ------------------------------------------------------------ packing
-int conv_groups(int ca, int cb) { return (ca / 32) * 27 + (cb == 16 ? 14 : (cb / 32) * 27); }
+// The upsampled input of a layer with one 16-row output block (cout = 16) is
+// staged in 16-channel chunks (smaller LDS footprint, higher occupancy);
+// otherwise in 32-channel chunks.
+int conv_a_chunk(int cout) { return cout == 16 ? 16 : 32; }
+int conv_groups(int ca, int cb, int cout) {
+  const int cca = conv_a_chunk(cout);
+  return (ca / cca) * (cca == 32 ? 27 : 14) + (cb == 16 ? 14 : (cb / 32) * 27);
+}
 
 __global__ __launch_bounds__(256) void k_pack_conv(const float *__restrict__ W, int cout, int cin, int ca_e, int cb_e,
                                                    int cout_e, int transposed, uint32_t total, bf16 *__restrict__ wp) {
@@ -42,13 +49,19 @@ __global__ __launch_bounds__(256) void k_pack_conv(const float *__restrict__ W, 
   const int kk = (int)(e % 32u);
   const int co_e = (int)((e / 32u) % (uint32_t)cout_e);
   int grp = (int)(e / (32u * (uint32_t)cout_e));
-  const int nA = ca_e / 32;
+  const int cca = cout_e == 16 ? 16 : 32, ga = cca == 32 ? 27 : 14;
+  const int nA = ca_e / cca;
   int tap, ci_e;
-  if (grp < nA * 27) {
-    tap = grp % 27;
-    ci_e = 32 * (grp / 27) + kk;
+  if (grp < nA * ga) {
+    if (cca == 32) {
+      tap = grp % 27;
+      ci_e = 32 * (grp / 27) + kk;
+    } else {
+      tap = 2 * (grp % 14) + kk / 16;
+      ci_e = 16 * (grp / 14) + (kk % 16);
+    }
   } else {
-    grp -= nA * 27;
+    grp -= nA * ga;
     if (cb_e == 16) {
       tap = 2 * grp + kk / 16;
       ci_e = ca_e + (kk % 16);
@@ -69,7 +82,7 @@ __global__ __launch_bounds__(256) void k_pack_conv(const float *__restrict__ W, 
 void launch_pack_conv(const float *W, const ConvSpec &c, bool transposed, bf16 *wp, hipStream_t s) {
   const int ca_e = transposed ? 0 : c.ca, cb_e = transposed ? c.cout : c.cb;
   const int cout_e = transposed ? c.cin() : c.cout;
-  const uint32_t total = (uint32_t)conv_groups(ca_e, cb_e) * (uint32_t)cout_e * 32u;
+  const uint32_t total = (uint32_t)conv_groups(ca_e, cb_e, cout_e) * (uint32_t)cout_e * 32u;
   k_pack_conv<<<nblk(total), 256, 0, s>>>(W, c.cout, c.cin(), ca_e, cb_e, cout_e, transposed ? 1 : 0, total, wp);
 }
 __global__ __launch_bounds__(256) void k_pack_bf16(const float *__restrict__ w, uint32_t n, bf16 *__restrict__ o) {
@@ -194,10 +207,12 @@ __device__ __forceinline__ void conv_chunk(const bf16 *__restrict__ wp, const bf
 template <int CA, int CB, int COUT, bool RELU>
 __global__ __launch_bounds__(256, 3) void k_conv3(ConvArgs p) {
   constexpr int MB = COUT / 16;
-  constexpr int NCA = CA / 32;
+  constexpr int CCA = MB == 1 ? 16 : 32;  // conv_a_chunk
+  constexpr int NCA = CA / CCA;
   constexpr int NCB = CB == 16 ? 1 : CB / 32;
   constexpr int CCB = CB == 16 ? 16 : 32;
-  __shared__ __align__(16) bf16 lds[CV_FN * 40];
+  constexpr int CCMAX = ((NCA > 0 && CCA == 32) || CCB == 32) ? 32 : 16;
+  __shared__ __align__(16) bf16 lds[CV_FN * (CCMAX + 8)];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int j = lane >> 4, col = lane & 15;
   const int ntx = (p.nx + CV_TX - 1) / CV_TX, nty = (p.ny + CV_TY - 1) / CV_TY;
@@ -213,10 +228,10 @@ __global__ __launch_bounds__(256, 3) void k_conv3(ConvArgs p) {
 #pragma unroll 1
   for (int q = 0; q < NCA; q++) {
     __syncthreads();
-    conv_stage<32, CA, true>(p, 32 * q, x0, y0, z0, lds);
+    conv_stage<CCA, CA, true>(p, CCA * q, x0, y0, z0, lds);
     __syncthreads();
-    conv_chunk<32, MB, COUT>(wp, lds, acc, w, j, col);
-    wp += (size_t)27 * COUT * 32;
+    conv_chunk<CCA, MB, COUT>(wp, lds, acc, w, j, col);
+    wp += (size_t)(CCA == 32 ? 27 : 14) * COUT * 32;
   }
 #pragma unroll 1
   for (int q = 0; q < NCB; q++) {

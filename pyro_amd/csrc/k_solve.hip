@@ -171,4 +171,143 @@ __global__ __launch_bounds__(NT) void k_zero64(uint32_t n, double *__restrict__ 
 }
 void launch_zero64(uint32_t n, double *a, hipStream_t s) { k_zero64<<<nblocks(n), NT, 0, s>>>(n, a); }
 
+// ---------------------------------------------------------------- fused CG steps
+// Per-block FP64 partial sums go to part[k * nb + block] (nb = blocks of the
+// launch); k_sum_final adds them in a fixed order into dev[slot + k], so the
+// scalars of the iteration stay on the device and every run is bit-reproducible.
+__global__ __launch_bounds__(1024) void k_sum_final(const double *__restrict__ part, int nb, int k, double *dev,
+                                                    int slot) {
+  __shared__ double sh[1024];
+  for (int q = 0; q < k; q++) {
+    double a = 0.0;
+    for (int i = threadIdx.x; i < nb; i += 1024) a += part[(size_t)q * nb + i];
+    sh[threadIdx.x] = a;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) dev[slot + q] = sh[0];
+    __syncthreads();
+  }
+}
+
+// q = A p with partials of p.q and p.r
+__global__ __launch_bounds__(NT) void k_apply_dots(Op64 o, const uint8_t *__restrict__ f, const uint8_t *__restrict__ nc,
+                                                    const double *__restrict__ p, const double *__restrict__ r,
+                                                    double *__restrict__ q, double *__restrict__ part) {
+  const uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
+  double pq = 0.0, pr = 0.0;
+  if (i < o.n) {
+    const double qi = apply_at(o, f, nc, p, i);
+    q[i] = qi;
+    pq = p[i] * qi;
+    pr = p[i] * r[i];
+  }
+  __shared__ double sh[NT / 64];
+  const double a = block_sum<NT>(pq, sh);
+  const double b = block_sum<NT>(pr, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = a;
+    part[gridDim.x + blockIdx.x] = b;
+  }
+}
+// alpha = dev[1] / dev[0]; x += alpha p; r -= alpha q; partials of r.r
+__global__ __launch_bounds__(NT) void k_update_rr(uint32_t n, const double *__restrict__ dev,
+                                                   const double *__restrict__ p, const double *__restrict__ q,
+                                                   double *__restrict__ x, double *__restrict__ r,
+                                                   double *__restrict__ part) {
+  const double alpha = dev[0] != 0.0 ? dev[1] / dev[0] : 0.0;
+  const uint32_t i = blockIdx.x * NT + threadIdx.x;
+  double rr = 0.0;
+  if (i < n) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    rr = ri * ri;
+  }
+  __shared__ double sh[NT / 64];
+  const double a = block_sum<NT>(rr, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = a;
+}
+// preconditioner input: b32 = (float)r, phi = 0
+__global__ __launch_bounds__(NT) void k_to_f32_zero(uint32_t n, const double *__restrict__ r, float *__restrict__ b,
+                                                     float *__restrict__ phi) {
+  const uint32_t i = blockIdx.x * NT + threadIdx.x;
+  if (i < n) {
+    b[i] = (float)r[i];
+    phi[i] = 0.0f;
+  }
+}
+// z = (double)phi with partials of z.q
+__global__ __launch_bounds__(NT) void k_to_f64_dot(uint32_t n, const float *__restrict__ phi,
+                                                    const double *__restrict__ q, double *__restrict__ z,
+                                                    double *__restrict__ part) {
+  const uint32_t i = blockIdx.x * NT + threadIdx.x;
+  double zq = 0.0;
+  if (i < n) {
+    const double zi = (double)phi[i];
+    z[i] = zi;
+    zq = zi * q[i];
+  }
+  __shared__ double sh[NT / 64];
+  const double a = block_sum<NT>(zq, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = a;
+}
+// p = z + beta p with beta = -dev[3] / dev[0]
+__global__ __launch_bounds__(NT) void k_xpby_dev(uint32_t n, const double *__restrict__ z,
+                                                  const double *__restrict__ dev, double *__restrict__ p) {
+  const double beta = dev[0] != 0.0 ? -dev[3] / dev[0] : 0.0;
+  const uint32_t i = blockIdx.x * NT + threadIdx.x;
+  if (i < n) p[i] = z[i] + beta * p[i];
+}
+
+// r = b - A x with partials of r.r
+__global__ __launch_bounds__(NT) void k_resid_rr(Op64 o, const uint8_t *__restrict__ f, const uint8_t *__restrict__ nc,
+                                                  const double *__restrict__ x, const double *__restrict__ b,
+                                                  double *__restrict__ r, double *__restrict__ part) {
+  const uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
+  double rr = 0.0;
+  if (i < o.n) {
+    const double ri = (nc[i] & 0x80) ? 0.0 : b[i] - apply_at(o, f, nc, x, i);
+    r[i] = ri;
+    rr = ri * ri;
+  }
+  __shared__ double sh[NT / 64];
+  const double a = block_sum<NT>(rr, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = a;
+}
+
+uint32_t cg_part_blocks(uint32_t n) { return nblocks(n); }
+void launch_resid_rr(const MgLevel &L, const double *x, const double *b, double *r, double *part, double *dev,
+                     hipStream_t s) {
+  const unsigned nb = nblocks(L.n);
+  k_resid_rr<<<nb, NT, 0, s>>>(op64_of(L), L.f, L.nc, x, b, r, part);
+  k_sum_final<<<1, 1024, 0, s>>>(part, (int)nb, 1, dev, 2);
+}
+void launch_apply_dots(const MgLevel &L, const double *p, const double *r, double *q, double *part, double *dev,
+                       hipStream_t s) {
+  const unsigned nb = nblocks(L.n);
+  k_apply_dots<<<nb, NT, 0, s>>>(op64_of(L), L.f, L.nc, p, r, q, part);
+  k_sum_final<<<1, 1024, 0, s>>>(part, (int)nb, 2, dev, 0);
+}
+void launch_update_rr(uint32_t n, const double *dev, const double *p, const double *q, double *x, double *r,
+                      double *part, double *dev_out, hipStream_t s) {
+  const unsigned nb = nblocks(n);
+  k_update_rr<<<nb, NT, 0, s>>>(n, dev, p, q, x, r, part);
+  k_sum_final<<<1, 1024, 0, s>>>(part, (int)nb, 1, dev_out, 2);
+}
+void launch_to_f32_zero(uint32_t n, const double *r, float *b, float *phi, hipStream_t s) {
+  k_to_f32_zero<<<nblocks(n), NT, 0, s>>>(n, r, b, phi);
+}
+void launch_to_f64_dot(uint32_t n, const float *phi, const double *q, double *z, double *part, double *dev,
+                       hipStream_t s) {
+  const unsigned nb = nblocks(n);
+  k_to_f64_dot<<<nb, NT, 0, s>>>(n, phi, q, z, part);
+  k_sum_final<<<1, 1024, 0, s>>>(part, (int)nb, 1, dev, 3);
+}
+void launch_xpby_dev(uint32_t n, const double *z, const double *dev, double *p, hipStream_t s) {
+  k_xpby_dev<<<nblocks(n), NT, 0, s>>>(n, z, dev, p);
+}
+
 }  // namespace pyro

@@ -86,6 +86,21 @@ void launch_to_f64(uint32_t n, const float *a, double *b, hipStream_t s);
 void launch_add_f32_to_f64(uint32_t n, const float *a, double *b, hipStream_t s);
 void launch_zero32(uint32_t n, float *a, hipStream_t s);
 void launch_zero64(uint32_t n, double *a, hipStream_t s);
+// fused flexible-CG steps with the iteration's scalars on the device:
+// dev[0] = p.q, dev[1] = p.r, dev[2] = r.r, dev[3] = z.q; part holds
+// cg_part_blocks(n) partials per sum (2 sums at most)
+uint32_t cg_part_blocks(uint32_t n);
+void launch_apply_dots(const MgLevel &L, const double *p, const double *r, double *q, double *part, double *dev,
+                       hipStream_t s);
+void launch_update_rr(uint32_t n, const double *dev, const double *p, const double *q, double *x, double *r,
+                      double *part, double *dev_out, hipStream_t s);
+void launch_to_f32_zero(uint32_t n, const double *r, float *b, float *phi, hipStream_t s);
+void launch_to_f64_dot(uint32_t n, const float *phi, const double *q, double *z, double *part, double *dev,
+                       hipStream_t s);
+void launch_xpby_dev(uint32_t n, const double *z, const double *dev, double *p, hipStream_t s);
+// r = b - A x, dev[2] = r.r
+void launch_resid_rr(const MgLevel &L, const double *x, const double *b, double *r, double *part, double *dev,
+                     hipStream_t s);
 
 // Debug / test: apply device math to an array (0 exp, 1 sqrt, 2 div by in2).
 void launch_debug_math(int fn, const float *in, const float *in2, float *out, uint32_t n, hipStream_t s);

@@ -323,8 +323,8 @@ int pyro_net_create(uint64_t seed, PyroNet **out) {
   if (!rc) rc |= dalloc(N->allocs, &N->w0a_bf, 2 * 16 * 32);
   for (int l = 0; l < NCONV && !rc; l++) {
     const ConvSpec &c = kSpec[l];
-    rc |= dalloc(N->allocs, &N->wp[l], (size_t)conv_groups(c.ca, c.cb) * c.cout * 32);
-    if (!rc) rc |= dalloc(N->allocs, &N->wpt[l], (size_t)conv_groups(0, c.cout) * c.cin() * 32);
+    rc |= dalloc(N->allocs, &N->wp[l], (size_t)conv_groups(c.ca, c.cb, c.cout) * c.cout * 32);
+    if (!rc) rc |= dalloc(N->allocs, &N->wpt[l], (size_t)conv_groups(0, c.cout, c.cin()) * c.cin() * 32);
   }
   if (!rc) rc |= dalloc(N->allocs, &N->dpart, DOT_PART);
   if (!rc) rc |= dalloc(N->allocs, &N->dout, 4);
@@ -628,7 +628,7 @@ int pyro_debug_conv(int ca, int cb, int cout, int transposed, int relu, uint32_t
   bf16 *da = nullptr, *dbb, *wp, *dout;
   float *tmp, *dW, *dbias = nullptr;
   const size_t n = g.n(), nh = gh.n();
-  const size_t nwp = first ? 1024 : (size_t)(transposed ? conv_groups(0, cout) * c.cin() : conv_groups(ca, cb) * cout) * 32;
+  const size_t nwp = first ? 1024 : (size_t)(transposed ? conv_groups(0, cout, c.cin()) * c.cin() : conv_groups(ca, cb, cout) * cout) * 32;
   const size_t big = std::max(std::max(n * in_b, nh * (size_t)(ca ? ca : 1)), n * out_c);
   rc = dalloc(L, &dbb, n * in_b);
   if (!rc && ca) rc = dalloc(L, &da, nh * ca);

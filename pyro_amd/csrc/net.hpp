@@ -43,9 +43,11 @@ struct ConvSpec {
 
 // Packed BF16 weights: K32 groups of [cout_eff][32].  A 32-channel chunk
 // contributes 27 groups (one per tap), a 16-channel chunk 14 groups (two taps
-// each, the 28th tap zero).  `transposed` packs the adjoint conv
+// each, the 28th tap zero); the upsampled input is chunked by
+// conv_a_chunk(cout_eff) channels.  `transposed` packs the adjoint conv
 // (cout_eff = cin, taps mirrored) used for input gradients.
-int conv_groups(int ca, int cb);
+int conv_a_chunk(int cout);
+int conv_groups(int ca, int cb, int cout);
 void launch_pack_conv(const float *W, const ConvSpec &c, bool transposed, bf16 *wp, hipStream_t s);
 void launch_pack_bf16(const float *w, uint32_t n, bf16 *out, hipStream_t s);
 

@@ -49,6 +49,7 @@ struct PyroPoisson {
   double *x = nullptr, *b = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *z = nullptr;
   double *w = nullptr, *t = nullptr;  // learned preconditioner: N(r) and the residual after it
   double *part = nullptr, *dots = nullptr;
+  double *cgpart = nullptr, *dev = nullptr;  // fused-CG partial sums and the iteration's scalars
   double nfluid = 0.0;
   unsigned int *count = nullptr;  // device fluid-cell counter
   hipStream_t stream = nullptr;
@@ -222,6 +223,8 @@ int pyro_poisson_create(uint32_t nx, uint32_t ny, uint32_t nz, float h, const ui
   if (!rc) rc |= palloc(P, &P->w, P->n);
   if (!rc) rc |= palloc(P, &P->t, P->n);
   if (!rc) rc |= palloc(P, &P->count, 4);
+  if (!rc) rc |= palloc(P, &P->cgpart, 2 * (size_t)cg_part_blocks(P->n));
+  if (!rc) rc |= palloc(P, &P->dev, 4);
   if (!rc) rc |= palloc(P, &P->part, DOT_PART);
   if (!rc) rc |= palloc(P, &P->dots, 4);
   if (rc) return bail(rc < 0 ? rc : PYRO_E_OOM);
@@ -290,84 +293,92 @@ int pyro_poisson_solve(PyroPoisson *P, int method, double tol, int max_iters, Py
     MgLevel &L0 = P->ref[0];
     launch_to_f32(n, P->x, L0.phi, P->stream);
     launch_to_f32(n, P->b, L0.b, P->stream);
-    rc = rel_resid(P, &rel, nullptr);
+    double bb = 0.0;
+    rc = rel_resid(P, &rel, &bb);
+    if (bb <= 0.0) bb = 1.0;
     while (!rc && rel > tol && it < max_iters) {
       enqueue_vcycle(P->ref, P->nl_ref, 80, P->stream);
       launch_to_f64(n, L0.phi, P->x, P->stream);
       cycles++;
       it++;
-      rc = rel_resid(P, &rel, nullptr);
+      launch_resid_rr(P->pc[0], P->x, P->b, P->r, P->cgpart, P->dev, P->stream);
+      double rr = 0.0;
+      PCHK(hipMemcpyAsync(&rr, P->dev + 2, sizeof(double), hipMemcpyDeviceToHost, P->stream));
+      rc = sync(P);
+      rel = std::sqrt(rr / bb);
     }
   } else if (method == PYRO_SOLVER_REFINE) {
     // x += V(b - A x) with the residual in FP64
-    rc = rel_resid(P, &rel, nullptr);
+    double bb = 0.0;
+    rc = rel_resid(P, &rel, &bb);
+    if (bb <= 0.0) bb = 1.0;
     while (!rc && rel > tol && it < max_iters) {
       MgLevel &L0 = P->pc[0];
-      launch_to_f32(n, P->r, L0.b, P->stream);
-      launch_zero32(n, L0.phi, P->stream);
+      launch_to_f32_zero(n, P->r, L0.b, L0.phi, P->stream);
       enqueue_vcycle(P->pc, P->nl_pc, P->coarse_iters_pc, P->stream);
       launch_add_f32_to_f64(n, L0.phi, P->x, P->stream);
       cycles++;
       it++;
-      rc = rel_resid(P, &rel, nullptr);
+      launch_resid_rr(L0, P->x, P->b, P->r, P->cgpart, P->dev, P->stream);
+      double rr = 0.0;
+      PCHK(hipMemcpyAsync(&rr, P->dev + 2, sizeof(double), hipMemcpyDeviceToHost, P->stream));
+      rc = sync(P);
+      rel = std::sqrt(rr / bb);
     }
   } else {
     // flexible CG (Notay's FCG(1)): p_{k+1} = z_{k+1} - (z_{k+1}.q_k / p_k.q_k) p_k.
     // LEARNED: the preconditioner is s N(r) + V(r - s A N(r)) (the network's
     // step, optimally scaled, then a V-cycle on what it leaves); the plain
     // V-cycle takes over for good if the network's direction is unusable.
+    // The iteration's scalars (p.q, p.r, r.r, z.q) stay on the device; the
+    // host reads only r.r for the convergence test.
     bool use_net = method == PYRO_SOLVER_LEARNED;
-    double rr = 0.0;
-    auto apply_m = [&](double rr_now) -> int {
+    MgLevel &L0 = P->pc[0];
+    auto apply_m = [&](double rr_now, bool with_dot) -> int {
       if (use_net) {
         bool ok = false;
         int e = learned_precond(P, P->r, rr_now, P->z, &ok);
         if (e) return e;
         if (ok) {
           if (P->learned_mode == 1) use_net = false;
+          if (with_dot) {
+            const double *a[1] = {P->z}, *b[1] = {P->q};
+            launch_dots(a, b, 1, n, P->part, P->dev + 3, P->stream);
+          }
           return 0;
         }
         use_net = false;
         fallbacks++;
       }
-      precond(P, P->r, P->z);
+      launch_to_f32_zero(n, P->r, L0.b, L0.phi, P->stream);
+      enqueue_vcycle(P->pc, P->nl_pc, P->coarse_iters_pc, P->stream);
+      if (with_dot)
+        launch_to_f64_dot(n, L0.phi, P->q, P->z, P->cgpart, P->dev, P->stream);
+      else
+        launch_to_f64(n, L0.phi, P->z, P->stream);
       return 0;
     };
-    double bb0 = 0.0;
-    rc = rel_resid(P, &rel, &bb0);  // r = b - A x
+    double bb = 0.0;
+    rc = rel_resid(P, &rel, &bb);  // r = b - A x
+    if (bb <= 0.0) bb = 1.0;
     if (!rc && rel > tol) {
-      rr = rel * rel * (bb0 > 0.0 ? bb0 : 1.0);
-      rc = apply_m(rr);
+      rc = apply_m(rel * rel * bb, false);
       cycles++;
-      PCHK(hipMemcpyAsync(P->p, P->z, n * sizeof(double), hipMemcpyDeviceToDevice, P->stream));
-      double bb;
-      {
-        double d1[1];
-        rc = dots(P, {{P->b, P->b}}, d1);
-        bb = d1[0];
-      }
+      if (!rc) PCHK(hipMemcpyAsync(P->p, P->z, n * sizeof(double), hipMemcpyDeviceToDevice, P->stream));
       while (!rc && it < max_iters) {
-        launch_apply64(P->pc[0], P->p, P->q, P->stream);
-        double d2[2];
-        rc = dots(P, {{P->p, P->q}, {P->p, P->r}}, d2);
-        if (rc) break;
-        if (d2[0] == 0.0) break;
-        const double alpha = d2[1] / d2[0];
-        launch_axpy64(n, alpha, P->p, P->x, P->stream);
-        launch_axpy64(n, -alpha, P->q, P->r, P->stream);
+        launch_apply_dots(L0, P->p, P->r, P->q, P->cgpart, P->dev, P->stream);  // dev[0] p.q, dev[1] p.r
+        launch_update_rr(n, P->dev, P->p, P->q, P->x, P->r, P->cgpart, P->dev, P->stream);  // dev[2] r.r
         it++;
-        double d3[1];
-        rc = dots(P, {{P->r, P->r}}, d3);
+        double rr = 0.0;
+        PCHK(hipMemcpyAsync(&rr, P->dev + 2, sizeof(double), hipMemcpyDeviceToHost, P->stream));
+        rc = sync(P);
         if (rc) break;
-        rel = bb > 0.0 ? std::sqrt(d3[0] / bb) : std::sqrt(d3[0]);
+        rel = std::sqrt(rr / bb);
         if (rel <= tol) break;
-        rc = apply_m(d3[0]);
+        rc = apply_m(rr, true);  // z, dev[3] z.q
         if (rc) break;
         cycles++;
-        double d4[1];
-        rc = dots(P, {{P->z, P->q}}, d4);
-        if (rc) break;
-        launch_xpby64(n, P->z, -d4[0] / d2[0], P->p, P->stream);
+        launch_xpby_dev(n, P->z, P->dev, P->p, P->stream);
       }
       if (!rc) rc = rel_resid(P, &rel, nullptr);  // true residual of the final x
     }
