@@ -66,72 +66,139 @@ __device__ __forceinline__ float resid_at(const LvArgs &F, const float *__restri
   return b[i] - (sum - cnt * pcen) / F.hh;
 }
 
+// Neighbour codes (setup pass below): bit 7 solid, bit 6 no counted
+// neighbour, bits 0-2 fluid neighbours, bits 3-5 neighbours outside the box.
+constexpr uint8_t NC_SOLID = 0x80, NC_NONE = 0x40;
+__device__ __forceinline__ float nc_count(uint32_t code, float wdir) {
+  return (float)(code & 7u) + wdir * (float)((code >> 3) & 7u);
+}
+
 // Residual + average over fluid children, coarse phi = 0 (mg.wgsl:67-81).
 // The coarse flags (mg.wgsl:82) are produced once per step by the setup pass.
-__global__ __launch_bounds__(NT) void k_mg_restrict(LvArgs F, const float *__restrict__ pf,
-                                                     const float *__restrict__ bf, const uint8_t *__restrict__ ff,
-                                                     int c0, int c1, int c2, FastDiv dc0, FastDiv dc1,
-                                                     float *__restrict__ bc, float *__restrict__ pc) {
-  uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
-  if (i >= (uint32_t)c0 * (uint32_t)c1 * (uint32_t)c2) return;
-  uint32_t r = fdiv(i, dc0);
-  int x = (int)(i - r * (uint32_t)c0);
-  uint32_t zq = fdiv(r, dc1);
-  int y = (int)(r - zq * (uint32_t)c1), z = (int)zq;
+// A block takes 16 x 4 x 4 coarse cells (one per thread); the fine φ and
+// neighbour codes of their children plus a ring of 1 are staged in LDS, φ
+// zeroed on solid and outside cells, so each child's neighbour sum adds the
+// six LDS values in the order of nsum (x+1, x-1, y+1, y-1, z+1, z-1): a
+// skipped neighbour contributes +0.0, which never changes the running sum.
+constexpr int RS_TX = 16, RS_TY = 4, RS_TZ = 4;
+constexpr int RS_FX = 2 * RS_TX + 2, RS_FY = 2 * RS_TY + 2, RS_FZ = 2 * RS_TZ + 2;
+constexpr int RS_FN = RS_FX * RS_FY * RS_FZ;
+__global__ __launch_bounds__(NT) void k_mg_restrict(LvArgs F, int ntx, int nty, const float *__restrict__ pf,
+                                                     const float *__restrict__ bf, const uint8_t *__restrict__ ncf,
+                                                     int c0, int c1, int c2, float *__restrict__ bc,
+                                                     float *__restrict__ pc) {
+  __shared__ float sp[RS_FN];
+  __shared__ uint8_t sc[RS_FN];
+  const int bid = xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = bid % ntx, rr = bid / ntx, ty = rr % nty, tz = rr / nty;
+  const int fx0 = 2 * RS_TX * tx - 1, fy0 = 2 * RS_TY * ty - 1, fz0 = 2 * RS_TZ * tz - 1;  // footprint origin
+  const int sy = F.d0, sz = F.d0 * F.d1;
+  for (int s = threadIdx.x; s < RS_FN; s += NT) {
+    const int lx = s % RS_FX, r2 = s / RS_FX, ly = r2 % RS_FY, lz = r2 / RS_FY;
+    const int gx = fx0 + lx, gy = fy0 + ly, gz = fz0 + lz;
+    float v = 0.0f;
+    uint8_t c = NC_NONE;
+    if (gx >= 0 && gx < F.d0 && gy >= 0 && gy < F.d1 && gz >= 0 && gz < F.d2) {
+      const int g = gx + sy * gy + sz * gz;
+      c = ncf[g];
+      if (!(c & NC_SOLID)) v = pf[g];
+    }
+    sp[s] = v;
+    sc[s] = c;
+  }
+  __syncthreads();
+  const int lcx = threadIdx.x % RS_TX, lr = threadIdx.x / RS_TX, lcy = lr % RS_TY, lcz = lr / RS_TY;
+  const int x = RS_TX * tx + lcx, y = RS_TY * ty + lcy, z = RS_TZ * tz + lcz;
+  if (x >= c0 || y >= c1 || z >= c2) return;
   float acc = 0.0f, nfl = 0.0f;
 #pragma unroll
   for (int k = 0; k < 8; k++) {
-    int a = 2 * x + (k & 1), bb = 2 * y + ((k >> 1) & 1), c = 2 * z + ((k >> 2) & 1);
+    const int a = 2 * x + (k & 1), bb = 2 * y + ((k >> 1) & 1), c = 2 * z + ((k >> 2) & 1);
     if (a >= F.d0 || bb >= F.d1 || c >= F.d2) continue;
-    if (ff[(uint32_t)a + (uint32_t)F.d0 * ((uint32_t)bb + (uint32_t)F.d1 * (uint32_t)c)] != 1) {
-      nfl += 1.0f;
-      acc += resid_at(F, pf, bf, ff, a, bb, c);
-    }
+    const int pi = ((c - fz0) * RS_FY + (bb - fy0)) * RS_FX + (a - fx0);
+    const uint8_t code = sc[pi];
+    if (code & NC_SOLID) continue;
+    nfl += 1.0f;
+    float sum = 0.0f;
+    sum += sp[pi + 1];
+    sum += sp[pi - 1];
+    sum += sp[pi + RS_FX];
+    sum += sp[pi - RS_FX];
+    sum += sp[pi + RS_FX * RS_FY];
+    sum += sp[pi - RS_FX * RS_FY];
+    const float cnt = nc_count(code, F.wdir);
+    acc += bf[a + sy * bb + sz * c] - (sum - cnt * sp[pi]) / F.hh;
   }
+  const uint32_t i = (uint32_t)x + (uint32_t)c0 * ((uint32_t)y + (uint32_t)c1 * (uint32_t)z);
   bc[i] = nfl > 0.0f ? acc / fmax_(nfl, 1.0f) : 0.0f;
   pc[i] = 0.0f;
 }
 void launch_mg_restrict(const MgLevel &Fl, const MgLevel &C, hipStream_t s) {
   LvArgs F = lv_args(Fl);
-  k_mg_restrict<<<nblocks(C.n), NT, 0, s>>>(F, Fl.phi, Fl.b, Fl.f, C.d[0], C.d[1], C.d[2],
-                                            make_fastdiv((uint32_t)C.d[0]), make_fastdiv((uint32_t)C.d[1]), C.b,
-                                            C.phi);
+  const int ntx = (C.d[0] + RS_TX - 1) / RS_TX, nty = (C.d[1] + RS_TY - 1) / RS_TY, ntz = (C.d[2] + RS_TZ - 1) / RS_TZ;
+  k_mg_restrict<<<ntx * nty * ntz, NT, 0, s>>>(F, ntx, nty, Fl.phi, Fl.b, Fl.nc, C.d[0], C.d[1], C.d[2], C.b, C.phi);
 }
 
-// Masked, renormalised trilinear correction (mg.wgsl:97-120).
-__global__ __launch_bounds__(NT) void k_mg_prolong(int f0, int f1, int f2, FastDiv df0, FastDiv df1,
+// Masked, renormalised trilinear correction (mg.wgsl:97-120).  A block takes
+// 32 x 8 x 4 fine cells (4 per thread along y); the coarse φ and flags they
+// interpolate from (18 x 6 x 4 cells, with the clamping of mg.wgsl applied
+// while staging) sit in LDS.  Weights and the order of the sums are those of
+// the per-cell formula.
+constexpr int PL_TX = 32, PL_TY = 8, PL_TZ = 4;
+constexpr int PL_CX = PL_TX / 2 + 2, PL_CY = PL_TY / 2 + 2, PL_CZ = PL_TZ / 2 + 2;
+constexpr int PL_CN = PL_CX * PL_CY * PL_CZ;
+__global__ __launch_bounds__(NT) void k_mg_prolong(int f0, int f1, int f2, int ntx, int nty,
                                                     const uint8_t *__restrict__ ff, float *__restrict__ pf, int c0,
                                                     int c1, int c2, const float *__restrict__ pc,
                                                     const uint8_t *__restrict__ fc) {
-  uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
-  if (i >= (uint32_t)f0 * (uint32_t)f1 * (uint32_t)f2) return;
-  if (ff[i] == 1) return;
-  uint32_t r = fdiv(i, df0);
-  int x = (int)(i - r * (uint32_t)f0);
-  uint32_t zq = fdiv(r, df1);
-  int y = (int)(r - zq * (uint32_t)f1), z = (int)zq;
-  float px = ((float)x + 0.5f) * 0.5f - 0.5f, py = ((float)y + 0.5f) * 0.5f - 0.5f,
-        pz = ((float)z + 0.5f) * 0.5f - 0.5f;
-  int i0x = (int)floorf(px), i0y = (int)floorf(py), i0z = (int)floorf(pz);
-  float frx = px - (float)i0x, fry = py - (float)i0y, frz = pz - (float)i0z;
-  float acc = 0.0f, ws = 0.0f;
-#pragma unroll
-  for (int k = 0; k < 8; k++) {
-    int ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
-    int cx = clampi(i0x + ox, 0, c0 - 1), cy = clampi(i0y + oy, 0, c1 - 1), cz = clampi(i0z + oz, 0, c2 - 1);
-    uint32_t j = (uint32_t)cx + (uint32_t)c0 * ((uint32_t)cy + (uint32_t)c1 * (uint32_t)cz);
-    if (fc[j] == 1) continue;
-    float wx = ox == 1 ? frx : 1.0f - frx, wy = oy == 1 ? fry : 1.0f - fry, wz = oz == 1 ? frz : 1.0f - frz;
-    float w = wx * wy * wz;
-    acc += w * pc[j];
-    ws += w;
+  __shared__ float sp[PL_CN];
+  __shared__ uint8_t sf[PL_CN];
+  const int bid = xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = bid % ntx, rr = bid / ntx, ty = rr % nty, tz = rr / nty;
+  const int x0 = PL_TX * tx, y0 = PL_TY * ty, z0 = PL_TZ * tz;
+  const int cx0 = x0 / 2 - 1, cy0 = y0 / 2 - 1, cz0 = z0 / 2 - 1;  // coarse footprint origin (unclamped)
+  for (int s = threadIdx.x; s < PL_CN; s += NT) {
+    const int u = s % PL_CX, r2 = s / PL_CX, v = r2 % PL_CY, w = r2 / PL_CY;
+    const int cx = clampi(cx0 + u, 0, c0 - 1), cy = clampi(cy0 + v, 0, c1 - 1), cz = clampi(cz0 + w, 0, c2 - 1);
+    const uint32_t j = (uint32_t)cx + (uint32_t)c0 * ((uint32_t)cy + (uint32_t)c1 * (uint32_t)cz);
+    sp[s] = pc[j];
+    sf[s] = fc[j];
   }
-  if (ws > 1e-4f) pf[i] += acc / ws;
+  __syncthreads();
+  const int lx = threadIdx.x % PL_TX, lyz = threadIdx.x / PL_TX;  // 8 (y, z) slots of 4 cells
+  const int x = x0 + lx;
+  if (x >= f0) return;
+  const float px = ((float)x + 0.5f) * 0.5f - 0.5f;
+  const int i0x = (int)floorf(px);
+  const float frx = px - (float)i0x;
+#pragma unroll
+  for (int q = 0; q < (PL_TY * PL_TZ) / (NT / PL_TX); q++) {
+    const int slot = lyz + q * (NT / PL_TX);
+    const int y = y0 + slot % PL_TY, z = z0 + slot / PL_TY;
+    if (y >= f1 || z >= f2) continue;
+    const uint32_t i = (uint32_t)x + (uint32_t)f0 * ((uint32_t)y + (uint32_t)f1 * (uint32_t)z);
+    if (ff[i] == 1) continue;
+    const float py = ((float)y + 0.5f) * 0.5f - 0.5f, pz = ((float)z + 0.5f) * 0.5f - 0.5f;
+    const int i0y = (int)floorf(py), i0z = (int)floorf(pz);
+    const float fry = py - (float)i0y, frz = pz - (float)i0z;
+    float acc = 0.0f, ws = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
+      const int si = ((i0z + oz - cz0) * PL_CY + (i0y + oy - cy0)) * PL_CX + (i0x + ox - cx0);
+      if (sf[si] == 1) continue;
+      const float wx = ox == 1 ? frx : 1.0f - frx, wy = oy == 1 ? fry : 1.0f - fry, wz = oz == 1 ? frz : 1.0f - frz;
+      const float wgt = wx * wy * wz;
+      acc += wgt * sp[si];
+      ws += wgt;
+    }
+    if (ws > 1e-4f) pf[i] += acc / ws;
+  }
 }
 void launch_mg_prolong(const MgLevel &C, const MgLevel &F, hipStream_t s) {
-  k_mg_prolong<<<nblocks(F.n), NT, 0, s>>>(F.d[0], F.d[1], F.d[2], make_fastdiv((uint32_t)F.d[0]),
-                                           make_fastdiv((uint32_t)F.d[1]), F.f, F.phi, C.d[0], C.d[1], C.d[2],
-                                           C.phi, C.f);
+  const int ntx = (F.d[0] + PL_TX - 1) / PL_TX, nty = (F.d[1] + PL_TY - 1) / PL_TY, ntz = (F.d[2] + PL_TZ - 1) / PL_TZ;
+  k_mg_prolong<<<ntx * nty * ntz, NT, 0, s>>>(F.d[0], F.d[1], F.d[2], ntx, nty, F.f, F.phi, C.d[0], C.d[1], C.d[2],
+                                              C.phi, C.f);
 }
 
 // ---------------------------------------------------------------- setup
@@ -142,10 +209,6 @@ void launch_mg_prolong(const MgLevel &C, const MgLevel &F, hipStream_t s) {
 //   bits 0-2: fluid neighbours; bits 3-5: neighbours outside the box on a side
 //   or the top.  cnt of mg.wgsl:25-33 = fluid + wdir * outside; with
 //   wdir = 1 it is the same small integer, exactly.
-constexpr uint8_t NC_SOLID = 0x80, NC_NONE = 0x40;
-__device__ __forceinline__ float nc_count(uint32_t code, float wdir) {
-  return (float)(code & 7u) + wdir * (float)((code >> 3) & 7u);
-}
 
 __global__ __launch_bounds__(NT) void k_mg_counts(LvArgs a, FastDiv d0, const uint8_t *__restrict__ f,
                                                    uint8_t *__restrict__ nc) {
