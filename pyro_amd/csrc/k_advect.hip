@@ -16,9 +16,35 @@ struct VelPtrs {
   const float *__restrict__ w;
 };
 
+// Read-only arrays are gathered through buffer resources: a sample's eight
+// loads share one 32-bit byte offset (+4 and the row/plane strides), instead
+// of eight 64-bit address computations.  Arrays stay below 4 GiB
+// (pyro_sim_create limits the grid).
+struct RB {
+  __amdgpu_buffer_rsrc_t r;
+};
+__device__ __forceinline__ RB rb_of(const void *p, uint32_t bytes) {
+  RB b;
+  b.r = __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, (int)bytes, 0x00020000);
+  return b;
+}
+__device__ __forceinline__ float ldf(const RB &b, uint32_t idx) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(b.r, idx * 4u, 0, 0));
+}
+__device__ __forceinline__ float4 ld4(const RB &b, uint32_t idx) {
+  return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(b.r, idx * 16u, 0, 0));
+}
+struct VelR {
+  RB u, v, w;
+};
+__device__ __forceinline__ VelR velr_of(const Geo &g, const VelPtrs &V) {
+  const uint32_t bytes = g.nv * 4u;
+  return VelR{rb_of(V.u, bytes), rb_of(V.v, bytes), rb_of(V.w, bytes)};
+}
+
 // Trilinear sample of a staggered component whose nodes are integer points
 // of g (tmpl_comp.wgsl:2-15); hi = largest valid node index.
-__device__ __forceinline__ float triC(const Geo &g, const float *__restrict__ X, float gx0, float gy0, float gz0,
+__device__ __forceinline__ float triC(const Geo &g, const RB &X, float gx0, float gy0, float gz0,
                                       int hx, int hy, int hz) {
   float gx = clampf_(gx0, 0.0f, (float)hx), gy = clampf_(gy0, 0.0f, (float)hy), gz = clampf_(gz0, 0.0f, (float)hz);
   int bx = (int)floorf(gx), by = (int)floorf(gy), bz = (int)floorf(gz);
@@ -28,14 +54,14 @@ __device__ __forceinline__ float triC(const Geo &g, const float *__restrict__ X,
   float fx = gx - (float)bx, fy = gy - (float)by, fz = gz - (float)bz;
   uint32_t sy = (uint32_t)g.vx, sz = (uint32_t)g.vx * (uint32_t)g.vy;
   uint32_t i = vidx(g, bx, by, bz);
-  float c00 = mixf_(X[i], X[i + 1], fx);
-  float c10 = mixf_(X[i + sy], X[i + sy + 1], fx);
-  float c01 = mixf_(X[i + sz], X[i + sz + 1], fx);
-  float c11 = mixf_(X[i + sy + sz], X[i + sy + sz + 1], fx);
+  float c00 = mixf_(ldf(X, i), ldf(X, i + 1), fx);
+  float c10 = mixf_(ldf(X, i + sy), ldf(X, i + sy + 1), fx);
+  float c01 = mixf_(ldf(X, i + sz), ldf(X, i + sz + 1), fx);
+  float c11 = mixf_(ldf(X, i + sy + sz), ldf(X, i + sy + sz + 1), fx);
   return mixf_(mixf_(c00, c10, fy), mixf_(c01, c11, fy), fz);
 }
 // 8-corner min/max at the sample's base cell (tmpl_comp.wgsl:16-25).
-__device__ __forceinline__ void mmC(const Geo &g, const float *__restrict__ X, float gx0, float gy0, float gz0,
+__device__ __forceinline__ void mmC(const Geo &g, const RB &X, float gx0, float gy0, float gz0,
                                     int hx, int hy, int hz, float &lo, float &hi) {
   float gx = clampf_(gx0, 0.0f, (float)hx), gy = clampf_(gy0, 0.0f, (float)hy), gz = clampf_(gz0, 0.0f, (float)hz);
   int bx = (int)floorf(gx), by = (int)floorf(gy), bz = (int)floorf(gz);
@@ -48,7 +74,7 @@ __device__ __forceinline__ void mmC(const Geo &g, const float *__restrict__ X, f
   float l = 1e30f, h = -1e30f;
 #pragma unroll
   for (int j = 0; j < 8; j++) {
-    float v = X[i + o[j]];
+    float v = ldf(X, i + o[j]);
     l = fmin_(l, v);
     h = fmax_(h, v);
   }
@@ -57,14 +83,14 @@ __device__ __forceinline__ void mmC(const Geo &g, const float *__restrict__ X, f
 }
 
 // velAt / traceBack / traceFwd (vel.wgsl:2-19) in cell units.
-__device__ __forceinline__ void velAt(const Geo &g, const VelPtrs &V, float px, float py, float pz, float &ux,
+__device__ __forceinline__ void velAt(const Geo &g, const VelR &V, float px, float py, float pz, float &ux,
                                       float &uy, float &uz) {
   ux = triC(g, V.u, px, py - 0.5f, pz - 0.5f, g.nx, g.ny - 1, g.nz - 1);
   uy = triC(g, V.v, px - 0.5f, py, pz - 0.5f, g.nx - 1, g.ny, g.nz - 1);
   uz = triC(g, V.w, px - 0.5f, py - 0.5f, pz, g.nx - 1, g.ny - 1, g.nz);
 }
 template <int SIGN>
-__device__ __forceinline__ void trace(const Geo &g, const VelPtrs &V, float k, float x, float y, float z, float &ox,
+__device__ __forceinline__ void trace(const Geo &g, const VelR &V, float k, float x, float y, float z, float &ox,
                                       float &oy, float &oz) {
   float u1x, u1y, u1z, u2x, u2y, u2z;
   velAt(g, V, x, y, z, u1x, u1y, u1z);
@@ -78,7 +104,7 @@ __device__ __forceinline__ void trace(const Geo &g, const VelPtrs &V, float k, f
   }
 }
 // traceBack and traceFwd from one point share their first sample u1 = velAt(x).
-__device__ __forceinline__ void trace2(const Geo &g, const VelPtrs &V, float k, float x, float y, float z, float &bx,
+__device__ __forceinline__ void trace2(const Geo &g, const VelR &V, float k, float x, float y, float z, float &bx,
                                        float &by, float &bz, float &fx, float &fy, float &fz) {
   float u1x, u1y, u1z, u2x, u2y, u2z;
   velAt(g, V, x, y, z, u1x, u1y, u1z);
@@ -90,11 +116,12 @@ __device__ __forceinline__ void trace2(const Geo &g, const VelPtrs &V, float k, 
 }
 
 // ---------------------------------------------------------------- adv_vel1
-__global__ __launch_bounds__(NT) void k_adv_vel1(const Params *__restrict__ P, Geo g, VelPtrs V,
+__global__ __launch_bounds__(NT) void k_adv_vel1(const Params *__restrict__ P, Geo g, VelPtrs VP,
                                                   float *__restrict__ hu, float *__restrict__ hv,
                                                   float *__restrict__ hw) {
   uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   if (i >= g.nv) return;
+  const VelR V = velr_of(g, VP);
   int x, y, z;
   decode_face(g, i, x, y, z);
   const float k = P->dt / P->dx;
@@ -122,13 +149,15 @@ __device__ __forceinline__ bool solidC(const Geo &g, const uint8_t *__restrict__
   return inb(g, x, y, z) && fl[sidx(g, x, y, z)] == 1;
 }
 
-__global__ __launch_bounds__(NT) void k_adv_vel2(const Params *__restrict__ P, Geo g, VelPtrs V,
+__global__ __launch_bounds__(NT) void k_adv_vel2(const Params *__restrict__ P, Geo g, VelPtrs VP,
                                                   const float *__restrict__ hu, const float *__restrict__ hv,
                                                   const float *__restrict__ hw, float *__restrict__ ou,
                                                   float *__restrict__ ov, float *__restrict__ ow,
                                                   const uint8_t *__restrict__ fl) {
   uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   if (i >= g.nv) return;
+  const VelR V = velr_of(g, VP);
+  const RB H[3] = {rb_of(hu, g.nv * 4u), rb_of(hv, g.nv * 4u), rb_of(hw, g.nv * 4u)};
   int x, y, z;
   decode_face(g, i, x, y, z);
   const float k = P->dt / P->dx;
@@ -143,8 +172,8 @@ __global__ __launch_bounds__(NT) void k_adv_vel2(const Params *__restrict__ P, G
       continue;
     }
     const float offx = c == 0 ? 0.0f : 0.5f, offy = c == 1 ? 0.0f : 0.5f, offz = c == 2 ? 0.0f : 0.5f;
-    const float *Vc = c == 0 ? V.u : (c == 1 ? V.v : V.w);
-    const float *Hc = c == 0 ? hu : (c == 1 ? hv : hw);
+    const RB &Vc = c == 0 ? V.u : (c == 1 ? V.v : V.w);
+    const RB &Hc = H[c];
     float px = (float)x + offx, py = (float)y + offy, pz = (float)z + offz;
     float bx, by, bz, fx, fy, fz;
     trace2(g, V, k, px, py, pz, bx, by, bz, fx, fy, fz);
@@ -153,7 +182,7 @@ __global__ __launch_bounds__(NT) void k_adv_vel2(const Params *__restrict__ P, G
     float lo, hi;
     mmC(g, Vc, bx, by, bz, hx, hy, hz, lo, hi);
     float t = triC(g, Hc, fx, fy, fz, hx, hy, hz);
-    O[i] = clampf_(Hc[i] + 0.5f * (Vc[i] - t), lo, hi);
+    O[i] = clampf_(ldf(Hc, i) + 0.5f * (ldf(Vc, i) - t), lo, hi);
   }
 }
 void launch_adv_vel2(const Fields &f, hipStream_t s) {
@@ -185,22 +214,21 @@ __device__ __forceinline__ SSample sbase(const Geo &g, float gx0, float gy0, flo
 __device__ __forceinline__ float4 mix4(float4 a, float4 b, float t) {
   return make_float4(mixf_(a.x, b.x, t), mixf_(a.y, b.y, t), mixf_(a.z, b.z, t), mixf_(a.w, b.w, t));
 }
-__device__ __forceinline__ float4 triS(const Geo &g, const float4 *__restrict__ X, const SSample &s) {
+__device__ __forceinline__ float4 triS(const Geo &g, const RB &X, const SSample &s) {
   uint32_t sy = (uint32_t)g.nx, sz = (uint32_t)g.nx * (uint32_t)g.ny, i = s.i;
-  float4 c00 = mix4(X[i], X[i + 1], s.fx);
-  float4 c10 = mix4(X[i + sy], X[i + sy + 1], s.fx);
-  float4 c01 = mix4(X[i + sz], X[i + sz + 1], s.fx);
-  float4 c11 = mix4(X[i + sy + sz], X[i + sy + sz + 1], s.fx);
+  float4 c00 = mix4(ld4(X, i), ld4(X, i + 1), s.fx);
+  float4 c10 = mix4(ld4(X, i + sy), ld4(X, i + sy + 1), s.fx);
+  float4 c01 = mix4(ld4(X, i + sz), ld4(X, i + sz + 1), s.fx);
+  float4 c11 = mix4(ld4(X, i + sy + sz), ld4(X, i + sy + sz + 1), s.fx);
   return mix4(mix4(c00, c10, s.fy), mix4(c01, c11, s.fy), s.fz);
 }
-__device__ __forceinline__ void mmS(const Geo &g, const float4 *__restrict__ X, const SSample &s, float4 &lo,
-                                    float4 &hi) {
+__device__ __forceinline__ void mmS(const Geo &g, const RB &X, const SSample &s, float4 &lo, float4 &hi) {
   uint32_t sy = (uint32_t)g.nx, sz = (uint32_t)g.nx * (uint32_t)g.ny, i = s.i;
   const uint32_t o[8] = {0, 1, sy, sy + 1, sz, sz + 1, sy + sz, sy + sz + 1};
   float4 l = make_float4(1e30f, 1e30f, 1e30f, 1e30f), h = make_float4(-1e30f, -1e30f, -1e30f, -1e30f);
 #pragma unroll
   for (int j = 0; j < 8; j++) {
-    float4 v = X[i + o[j]];
+    float4 v = ld4(X, i + o[j]);
     l.x = fmin_(l.x, v.x); l.y = fmin_(l.y, v.y); l.z = fmin_(l.z, v.z); l.w = fmin_(l.w, v.w);
     h.x = fmax_(h.x, v.x); h.y = fmax_(h.y, v.y); h.z = fmax_(h.z, v.z); h.w = fmax_(h.w, v.w);
   }
@@ -213,11 +241,13 @@ __device__ __forceinline__ float outside(const Geo &g, float x, float y, float z
                fmax_(fmax_(-0.5f - z, z - ((float)g.nz - 0.5f)), y - ((float)g.ny - 0.5f)));
 }
 
-__global__ __launch_bounds__(NT) void k_adv_s1(const Params *__restrict__ P, Geo g, VelPtrs V,
-                                                const float4 *__restrict__ s0, const float4 *__restrict__ s1,
+__global__ __launch_bounds__(NT) void k_adv_s1(const Params *__restrict__ P, Geo g, VelPtrs VP,
+                                                const float4 *__restrict__ s0p, const float4 *__restrict__ s1p,
                                                 float4 *__restrict__ h0, float4 *__restrict__ h1) {
   uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   if (i >= g.n) return;
+  const VelR V = velr_of(g, VP);
+  const RB s0 = rb_of(s0p, g.n * 16u), s1 = rb_of(s1p, g.n * 16u);
   int x, y, z;
   decode_cell(g, i, x, y, z);
   const float k = P->dt / P->dx;
@@ -240,12 +270,15 @@ __device__ __forceinline__ float mcl(float r, float s, float t, float lo, float 
   return clampf_(r + 0.5f * (s - t), lo, hi);
 }
 
-__global__ __launch_bounds__(NT) void k_adv_s2(const Params *__restrict__ P, Geo g, VelPtrs V,
-                                                const float4 *__restrict__ s0, const float4 *__restrict__ s1,
-                                                const float4 *__restrict__ h0, const float4 *__restrict__ h1,
+__global__ __launch_bounds__(NT) void k_adv_s2(const Params *__restrict__ P, Geo g, VelPtrs VP,
+                                                const float4 *__restrict__ s0p, const float4 *__restrict__ s1p,
+                                                const float4 *__restrict__ h0p, const float4 *__restrict__ h1p,
                                                 float4 *__restrict__ o0, float4 *__restrict__ o1) {
   uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   if (i >= g.n) return;
+  const VelR V = velr_of(g, VP);
+  const RB s0 = rb_of(s0p, g.n * 16u), s1 = rb_of(s1p, g.n * 16u), h0 = rb_of(h0p, g.n * 16u),
+           h1 = rb_of(h1p, g.n * 16u);
   int x, y, z;
   decode_cell(g, i, x, y, z);
   const float k = P->dt / P->dx;
@@ -254,14 +287,14 @@ __global__ __launch_bounds__(NT) void k_adv_s2(const Params *__restrict__ P, Geo
   trace2(g, V, k, px, py, pz, bx, by, bz, fx, fy, fz);
   bx = bx - 0.5f; by = by - 0.5f; bz = bz - 0.5f;
   fx = fx - 0.5f; fy = fy - 0.5f; fz = fz - 0.5f;
-  float4 r0 = h0[i], r1 = h1[i];
+  float4 r0 = ld4(h0, i), r1 = ld4(h1, i);
   if (outside(g, bx, by, bz) <= 0.0f && outside(g, fx, fy, fz) <= 0.0f) {
     SSample sb = sbase(g, bx, by, bz), sf = sbase(g, fx, fy, fz);
     float4 l0, u0, l1, u1;
     mmS(g, s0, sb, l0, u0);
     mmS(g, s1, sb, l1, u1);
     float4 t0 = triS(g, h0, sf), t1 = triS(g, h1, sf);
-    float4 a = s0[i], c = s1[i];
+    float4 a = ld4(s0, i), c = ld4(s1, i);
     r0 = make_float4(mcl(r0.x, a.x, t0.x, l0.x, u0.x), mcl(r0.y, a.y, t0.y, l0.y, u0.y),
                      mcl(r0.z, a.z, t0.z, l0.z, u0.z), mcl(r0.w, a.w, t0.w, l0.w, u0.w));
     r1 = make_float4(mcl(r1.x, c.x, t1.x, l1.x, u1.x), mcl(r1.y, c.y, t1.y, l1.y, u1.y),

@@ -226,6 +226,8 @@ int pyro_sim_create(const void *params, size_t params_bytes, const float *sol_in
   if (hp.nx < 2 || hp.ny < 2 || hp.nz < 2) return fail(PYRO_E_INVALID, "grid dims must be >= 2");
   uint64_t n64 = (uint64_t)hp.nx * hp.ny * hp.nz, nv64 = (uint64_t)(hp.nx + 1) * (hp.ny + 1) * (hp.nz + 1);
   if (nv64 >= (1ull << 31)) return fail(PYRO_E_INVALID, "grid too large for 32-bit indexing");
+  // vec4 fields are gathered through buffer resources with 32-bit byte offsets
+  if (n64 * 16ull >= (1ull << 32)) return fail(PYRO_E_INVALID, "grid too large: a vec4 field must stay below 4 GiB");
   PyroSim *s = new PyroSim();
   s->hp = hp;
   s->hp.dt = 0.005f;
