@@ -460,7 +460,7 @@ void launch_mg_sweep(const MgLevel &L, const float *pin, float *pout, float omeg
 // keeps their LDS index, count and h^2 b in registers.  Within one colour
 // pass no cell reads a cell of its own colour, so the loads of a chunk of
 // CO_C cells are issued together before any store.
-constexpr int CO_NT = 512, CO_K = 16, CO_C = 4;
+constexpr int CO_NT = 1024, CO_K = 8, CO_C = 2;
 constexpr int CO_PAD_MAX = 36000;  // padded cells: 144 KB of LDS (< 65536: 16-bit index)
 
 __global__ __launch_bounds__(CO_NT) void k_mg_coarse(LvArgs a, const float *__restrict__ b,
