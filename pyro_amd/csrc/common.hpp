@@ -15,6 +15,26 @@
 
 namespace pyro {
 
+// Buffer resource over a read-only array below 4 GiB (pyro_sim_create limits
+// the grid): loads take a 32-bit byte offset, and the range check of raw
+// buffer loads returns 0 for an offset at or past `bytes` (RB_OOB), which
+// replaces bounds branches where 0 is the value wanted outside.
+struct RB {
+  __amdgpu_buffer_rsrc_t r;
+};
+__device__ __forceinline__ RB rb_of(const void *p, uint32_t bytes) {
+  RB b;
+  b.r = __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, (int)bytes, 0x00020000);
+  return b;
+}
+constexpr uint32_t RB_OOB = 0x80000000u;  // past every array: reads 0
+__device__ __forceinline__ float4 rb_ld4(const RB &b, uint32_t byte_off) {
+  return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(b.r, byte_off, 0, 0));
+}
+__device__ __forceinline__ uint32_t rb_ld32(const RB &b, uint32_t byte_off) {
+  return __builtin_amdgcn_raw_buffer_load_b32(b.r, byte_off, 0, 0);
+}
+
 struct Mat { float a[4], b[4], c[4]; };
 struct Params {
   uint32_t nx, ny, nz, frame;

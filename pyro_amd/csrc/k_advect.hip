@@ -16,18 +16,9 @@ struct VelPtrs {
   const float *__restrict__ w;
 };
 
-// Read-only arrays are gathered through buffer resources: a sample's eight
-// loads share one 32-bit byte offset (+4 and the row/plane strides), instead
-// of eight 64-bit address computations.  Arrays stay below 4 GiB
-// (pyro_sim_create limits the grid).
-struct RB {
-  __amdgpu_buffer_rsrc_t r;
-};
-__device__ __forceinline__ RB rb_of(const void *p, uint32_t bytes) {
-  RB b;
-  b.r = __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, (int)bytes, 0x00020000);
-  return b;
-}
+// Read-only arrays are gathered through buffer resources (RB, common.hpp):
+// a sample's eight loads share one 32-bit byte offset (+4 and the row/plane
+// strides), instead of eight 64-bit address computations.
 __device__ __forceinline__ float ldf(const RB &b, uint32_t idx) {
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(b.r, idx * 4u, 0, 0));
 }

@@ -73,6 +73,26 @@ __device__ __forceinline__ float nc_count(uint32_t code, float wdir) {
   return (float)(code & 7u) + wdir * (float)((code >> 3) & 7u);
 }
 
+// x / c for an integer count c in 1..6 given r = RN(1/c) and x != -0:
+// q0 = x r, the remainder e = x - c q0 (exact through the FMA), q = q0 + e r
+// (Markstein's correction).  Checked exhaustively over every finite float
+// x != -0 for each c: q equals the IEEE quotient whenever q is not subnormal
+// (the only mismatches are c = 6 with 0 < |x| < 2^-125), and a subnormal q
+// takes the IEEE division.  The callers' x = (neighbour sum from +0) - h^2 b
+// is never -0: a sum that starts at +0 never becomes -0, and s - t is -0
+// only for s = -0.
+__device__ __forceinline__ float div_count(float x, float c, float r) {
+  const float q0 = x * r;
+  const float e = __builtin_fmaf(-q0, c, x);
+  float q = __builtin_fmaf(e, r, q0);
+  if (__builtin_amdgcn_classf(q, 0x90)) {  // -subnormal | +subnormal
+    float xs = x;
+    asm volatile("" : "+v"(xs));  // a branch, not a select: the division runs only for these x
+    q = xs / c;
+  }
+  return q;
+}
+
 // Residual + average over fluid children, coarse phi = 0 (mg.wgsl:67-81).
 // The coarse flags (mg.wgsl:82) are produced once per step by the setup pass.
 // A block takes 16 x 4 x 4 coarse cells (one per thread); the fine φ and
@@ -281,8 +301,16 @@ constexpr int SW_BY = SW_TY + 2, SW_BZ = SW_TZ + 2;                      // ring
 constexpr int SW_PN = SW_PX * SW_PY * SW_PZ, SW_BN = SW_PX * SW_BY * SW_BZ;
 constexpr int SW_Q = SW_PX / 4;  // float4 per footprint row
 
-__device__ __forceinline__ float sweep_update(const float *sp, int pi, uint8_t code, float hb, float omega,
-                                              float wdir) {
+// One cell of a colour pass at LDS index pi (h^2 b at bi), without branches:
+// the update is computed for every cell and stored only for cells with a
+// counted neighbour that are not solid.  ICNT (outside weight 1): the count
+// is the integer fluid + outside and ct[code & 63] holds (count,
+// RN(1/count)) for div_count (code 0x40 / 0x80 reads (0, 0): a finite,
+// discarded result).
+template <bool ICNT>
+__device__ __forceinline__ void sweep_cell(float *sp, const uint8_t *sc, const float *sb, const float2 *ct, int pi,
+                                           int bi, float omega, float wdir) {
+  const uint32_t code = sc[pi];
   float sum = 0.0f;
   sum += sp[pi + 1];
   sum += sp[pi - 1];
@@ -290,74 +318,100 @@ __device__ __forceinline__ float sweep_update(const float *sp, int pi, uint8_t c
   sum += sp[pi - SW_PX];
   sum += sp[pi + SW_PX * SW_PY];
   sum += sp[pi - SW_PX * SW_PY];
-  const float cnt = nc_count(code, wdir);
-  return mixf_(sp[pi], (sum - hb) / cnt, omega);
+  float q;
+  if (ICNT) {
+    const float2 cr = ct[code & 63u];
+    q = div_count(sum - sb[bi], cr.x, cr.y);
+  } else {
+    q = (sum - sb[bi]) / nc_count(code, wdir);
+  }
+  const float nv = mixf_(sp[pi], q, omega);
+  if (!(code & (NC_SOLID | NC_NONE))) sp[pi] = nv;
 }
 
-template <bool FINE, bool VEC>
+// Staging of the VEC path: thread t < 240 loads x-group q = t % 10 of
+// footprint row ly = (t / 10) % 12 in planes (t / 120) + 2 it, it = 0..3
+// (LDS float4 index t + 240 it); h^2 b: thread t < 200, x-group t % 10, row
+// (t / 10) % 10 of the ring-1 box, planes (t / 100) + 2 it, it = 0..2 (LDS
+// index t + 200 it).  Loads outside the grid take the out-of-range offset
+// and read 0 (codes: 0x40, never updated).  The VEC path needs the level's
+// arrays below 2 GiB (launch_mg_sweep checks).
+constexpr int SW_PT = SW_Q * SW_PY * 2, SW_BT = SW_Q * SW_BY * 2;
+static_assert(SW_PZ == 8 && SW_BZ == 6 && SW_PT <= SW_NT && SW_BT <= SW_NT, "staging map");
+
+template <bool FINE, bool VEC, bool ICNT>
 __global__ __launch_bounds__(SW_NT) void k_mg_sweep(LvArgs a, int ntx, int nty, const float *__restrict__ b,
                                                      const uint8_t *__restrict__ nc, const float *__restrict__ pin,
                                                      float *__restrict__ pout, float omega) {
   __shared__ __align__(16) float sp[SW_PN];
   __shared__ __align__(16) float sb[SW_BN];
   __shared__ __align__(16) uint8_t sc[SW_PN];
+  __shared__ float2 ct[64];
+  const int t = threadIdx.x;
+  if (ICNT && t < 64) {
+    const uint32_t cnt = (t & 7u) + ((t >> 3) & 7u);
+    ct[t] = make_float2((float)cnt, cnt ? 1.0f / (float)cnt : 0.0f);
+  }
   const int bid = xcd_tile(blockIdx.x, gridDim.x);
   const int tx = bid % ntx, rr = bid / ntx, ty = rr % nty, tz = rr / nty;
   const int x0 = tx * SW_TX - 4, y0 = ty * SW_TY - 2, z0 = tz * SW_TZ - 2;  // footprint origin, x0 % 4 == 0
   const int sy = a.d0, sz = a.d0 * a.d1;
   if (VEC) {
-    // every global load of the footprint is issued before the first LDS store
-    constexpr int NP = SW_Q * SW_PY * SW_PZ, IP = (NP + SW_NT - 1) / SW_NT;
-    constexpr int NBV = SW_Q * SW_BY * SW_BZ, IB = (NBV + SW_NT - 1) / SW_NT;
-    float4 pv[IP], bv[IB];
-    uint32_t cv[IP];
+    const RB rp = rb_of(pin, a.n * 4u), rcd = rb_of(nc, a.n), rbb = rb_of(b, a.n * 4u);
+    float4 pv[4], bv[3];
+    uint32_t cv[4];
+    {
+      const int q = t % SW_Q, r0 = t / SW_Q, ly = r0 % SW_PY, lz0 = r0 / SW_PY;
+      const int gx = x0 + 4 * q, gy = y0 + ly;
+      const bool okxy = t < SW_PT && (uint32_t)gx < (uint32_t)a.d0 && (uint32_t)gy < (uint32_t)a.d1;
+      const uint32_t gxy = (uint32_t)gx + (uint32_t)sy * (uint32_t)gy;
 #pragma unroll
-    for (int it = 0; it < IP; it++) {
-      const int s = threadIdx.x + it * SW_NT;
-      const int q = s % SW_Q, row = s / SW_Q, ly = row % SW_PY, lz = row / SW_PY;
-      const int gx = x0 + 4 * q, gy = y0 + ly, gz = z0 + lz;
-      pv[it] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      cv[it] = 0x40404040u;  // outside: never updated
-      if (s < NP && gx >= 0 && gx < a.d0 && gy >= 0 && gy < a.d1 && gz >= 0 && gz < a.d2) {
-        const int g = gx + sy * gy + sz * gz;
-        pv[it] = *reinterpret_cast<const float4 *>(pin + g);
-        cv[it] = *reinterpret_cast<const uint32_t *>(nc + g);
+      for (int it = 0; it < 4; it++) {
+        const int gz = z0 + lz0 + 2 * it;
+        const bool ok = okxy && (uint32_t)gz < (uint32_t)a.d2;
+        const uint32_t g = gxy + (uint32_t)sz * (uint32_t)gz;
+        pv[it] = rb_ld4(rp, ok ? 4u * g : RB_OOB);
+        const uint32_t c = rb_ld32(rcd, ok ? g : RB_OOB);
+        cv[it] = ok ? c : 0x40404040u;
       }
     }
+    {
+      const int q = t % SW_Q, r0 = t / SW_Q, ly = r0 % SW_BY, lz0 = r0 / SW_BY;
+      const int gx = x0 + 4 * q, gy = y0 + 1 + ly;
+      const bool okxy = t < SW_BT && (uint32_t)gx < (uint32_t)a.d0 && (uint32_t)gy < (uint32_t)a.d1;
+      const uint32_t gxy = (uint32_t)gx + (uint32_t)sy * (uint32_t)gy;
 #pragma unroll
-    for (int it = 0; it < IB; it++) {
-      const int s = threadIdx.x + it * SW_NT;
-      const int q = s % SW_Q, row = s / SW_Q, ly = row % SW_BY, lz = row / SW_BY;
-      const int gx = x0 + 4 * q, gy = y0 + 1 + ly, gz = z0 + 1 + lz;
-      bv[it] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      if (s < NBV && gx >= 0 && gx < a.d0 && gy >= 0 && gy < a.d1 && gz >= 0 && gz < a.d2)
-        bv[it] = *reinterpret_cast<const float4 *>(b + gx + sy * gy + sz * gz);
-    }
-#pragma unroll
-    for (int it = 0; it < IP; it++) {
-      const int s = threadIdx.x + it * SW_NT;
-      if (s >= NP) break;
-      float4 v = pv[it];
-      const uint32_t c = cv[it];
-      if (c & 0x80808080u) {
-        if (c & 0x80u) v.x = 0.0f;
-        if (c & 0x8000u) v.y = 0.0f;
-        if (c & 0x800000u) v.z = 0.0f;
-        if (c & 0x80000000u) v.w = 0.0f;
+      for (int it = 0; it < 3; it++) {
+        const int gz = z0 + 1 + lz0 + 2 * it;
+        const bool ok = okxy && (uint32_t)gz < (uint32_t)a.d2;
+        bv[it] = rb_ld4(rbb, ok ? 4u * (gxy + (uint32_t)sz * (uint32_t)gz) : RB_OOB);
       }
-      *reinterpret_cast<float4 *>(sp + 4 * s) = v;
-      *reinterpret_cast<uint32_t *>(sc + 4 * s) = c;
     }
+    if (t < SW_PT) {
 #pragma unroll
-    for (int it = 0; it < IB; it++) {
-      const int s = threadIdx.x + it * SW_NT;
-      if (s >= NBV) break;
-      float4 v = bv[it];
-      v.x = a.hh * v.x; v.y = a.hh * v.y; v.z = a.hh * v.z; v.w = a.hh * v.w;
-      *reinterpret_cast<float4 *>(sb + 4 * s) = v;
+      for (int it = 0; it < 4; it++) {
+        float4 v = pv[it];
+        const uint32_t c = cv[it];
+        if (c & 0x80808080u) {  // solids read 0 in the neighbour sums
+          if (c & 0x80u) v.x = 0.0f;
+          if (c & 0x8000u) v.y = 0.0f;
+          if (c & 0x800000u) v.z = 0.0f;
+          if (c & 0x80000000u) v.w = 0.0f;
+        }
+        *reinterpret_cast<float4 *>(sp + 4 * (t + SW_PT * it)) = v;
+        *reinterpret_cast<uint32_t *>(sc + 4 * (t + SW_PT * it)) = c;
+      }
+    }
+    if (t < SW_BT) {
+#pragma unroll
+      for (int it = 0; it < 3; it++) {
+        float4 v = bv[it];
+        v.x = a.hh * v.x; v.y = a.hh * v.y; v.z = a.hh * v.z; v.w = a.hh * v.w;
+        *reinterpret_cast<float4 *>(sb + 4 * (t + SW_BT * it)) = v;
+      }
     }
   } else {
-    for (int i = threadIdx.x; i < SW_PN; i += SW_NT) {
+    for (int i = t; i < SW_PN; i += SW_NT) {
       const int lx = i % SW_PX, row = i / SW_PX, ly = row % SW_PY, lz = row / SW_PY;
       const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
       float pv = 0.0f;
@@ -370,7 +424,7 @@ __global__ __launch_bounds__(SW_NT) void k_mg_sweep(LvArgs a, int ntx, int nty, 
       sp[i] = pv;
       sc[i] = cv;
     }
-    for (int i = threadIdx.x; i < SW_BN; i += SW_NT) {
+    for (int i = t; i < SW_BN; i += SW_NT) {
       const int lx = i % SW_PX, row = i / SW_PX, ly = row % SW_BY, lz = row / SW_BY;
       const int gx = x0 + lx, gy = y0 + 1 + ly, gz = z0 + 1 + lz;
       float bv = 0.0f;
@@ -379,40 +433,47 @@ __global__ __launch_bounds__(SW_NT) void k_mg_sweep(LvArgs a, int ntx, int nty, 
     }
   }
   __syncthreads();
-  // colour 0 on the tile + ring 1 (x in [3, 37)): one thread per x-pair
-  constexpr int RP = (SW_TX + 2) / 2, RN = RP * SW_BY * SW_BZ;
+  // Colour of (gx, gy, gz) = (gx + gy + gz) & 1 with gx = x0 + lx etc.; x0 is
+  // a multiple of 4, so the colour-c cell of an x-pair starting at an odd
+  // lx is lx + ((K + ly + lz + c) & 1) with K = y0 + z0 + 1 (ly, lz counted
+  // from the box the pass walks).
+  const int K = y0 + z0 + 1;
+  // colour 0 on the tile + ring 1: x-pairs from lx = 3 (17 of them), rows
+  // (ly, lz) of the 10 x 6 ring-1 box (LDS offset 1 in y and z); thread
+  // t < 255 takes x-pair t % 17 of rows r0 + 15 it, r0 = t / 17, it = 0..3
+  static_assert((SW_TX + 2) / 2 == 17 && SW_BY * SW_BZ == 60, "colour-0 map");
+  if (t < 255) {
+    const int px = t % 17, r0 = t / 17;
 #pragma unroll
-  for (int it = 0; it < (RN + SW_NT - 1) / SW_NT; it++) {
-    const int i = threadIdx.x + it * SW_NT;
-    if (i >= RN) break;
-    const int px = i % RP, row = i / RP, ly = row % SW_BY, lz = row / SW_BY;
-    const int lx = 3 + 2 * px + ((3 + 2 * px + y0 + ly + 1 + z0 + lz + 1) & 1);  // (gx+gy+gz) even
-    const int pi = ((lz + 1) * SW_PY + (ly + 1)) * SW_PX + lx;
-    const uint8_t code = sc[pi];
-    if (code & (NC_SOLID | NC_NONE)) continue;
-    sp[pi] = sweep_update(sp, pi, code, sb[(lz * SW_BY + ly) * SW_PX + lx], omega, a.wdir);
+    for (int it = 0; it < 4; it++) {
+      const int r = r0 + 15 * it, lz = r / SW_BY, ly = r - SW_BY * lz;
+      const int lx = 3 + 2 * px + ((K + ly + lz) & 1);
+      const int pi = ((lz + 1) * SW_PY + (ly + 1)) * SW_PX + lx;
+      sweep_cell<ICNT>(sp, sc, sb, ct, pi, r * SW_PX + lx, omega, a.wdir);
+    }
   }
   __syncthreads();
-  // colour 1 on the tile (x in [4, 36))
-  constexpr int BP = SW_TX / 2, BN = BP * SW_TY * SW_TZ;
+  // colour 1 on the tile: x-pairs from lx = 4 (16), rows of the 8 x 4 tile
+  // (LDS offset 2); thread t takes x-pair t % 16 of rows t / 16 and t / 16 + 16
+  static_assert(SW_TX / 2 == 16 && SW_TY == 8 && SW_TZ == 4, "colour-1 map");
+  {
+    const int px = t % 16, r0 = t / 16;
 #pragma unroll
-  for (int it = 0; it < (BN + SW_NT - 1) / SW_NT; it++) {
-    const int i = threadIdx.x + it * SW_NT;
-    if (i >= BN) break;
-    const int px = i % BP, row = i / BP, ly = row % SW_TY, lz = row / SW_TY;
-    const int lx = 4 + 2 * px + ((4 + 2 * px + y0 + ly + 2 + z0 + lz + 2 + 1) & 1);  // (gx+gy+gz) odd
-    const int pi = ((lz + 2) * SW_PY + (ly + 2)) * SW_PX + lx;
-    const uint8_t code = sc[pi];
-    if (code & (NC_SOLID | NC_NONE)) continue;
-    sp[pi] = sweep_update(sp, pi, code, sb[((lz + 1) * SW_BY + (ly + 1)) * SW_PX + lx], omega, a.wdir);
+    for (int it = 0; it < 2; it++) {
+      const int r = r0 + 16 * it, ly = r % SW_TY, lz = r / SW_TY;
+      const int lx = 4 + 2 * px + ((K + ly + lz) & 1);
+      const int pi = ((lz + 2) * SW_PY + (ly + 2)) * SW_PX + lx;
+      const int bi = ((lz + 1) * SW_BY + (ly + 1)) * SW_PX + lx;
+      sweep_cell<ICNT>(sp, sc, sb, ct, pi, bi, omega, a.wdir);
+    }
   }
   __syncthreads();
   if (VEC) {
     constexpr int WN = (SW_TX / 4) * SW_TY * SW_TZ;
-    for (int s = threadIdx.x; s < WN; s += SW_NT) {
-      const int q = s % (SW_TX / 4), row = s / (SW_TX / 4), ly = row % SW_TY, lz = row / SW_TY;
-      const int gx = x0 + 4 + 4 * q, gy = y0 + 2 + ly, gz = z0 + 2 + lz;
-      if (gx >= a.d0 || gy >= a.d1 || gz >= a.d2) continue;
+    static_assert(WN == SW_NT, "one float4 of the tile per thread");
+    const int q = t % (SW_TX / 4), row = t / (SW_TX / 4), ly = row % SW_TY, lz = row / SW_TY;
+    const int gx = x0 + 4 + 4 * q, gy = y0 + 2 + ly, gz = z0 + 2 + lz;
+    if (gx < a.d0 && gy < a.d1 && gz < a.d2) {
       const int g = gx + sy * gy + sz * gz;
       const int pi = ((lz + 2) * SW_PY + (ly + 2)) * SW_PX + 4 + 4 * q;
       float4 v = *reinterpret_cast<const float4 *>(sp + pi);
@@ -428,7 +489,7 @@ __global__ __launch_bounds__(SW_NT) void k_mg_sweep(LvArgs a, int ntx, int nty, 
     }
   } else {
     constexpr int TN = SW_TX * SW_TY * SW_TZ;
-    for (int i = threadIdx.x; i < TN; i += SW_NT) {
+    for (int i = t; i < TN; i += SW_NT) {
       const int lx = i % SW_TX, row = i / SW_TX, ly = row % SW_TY, lz = row / SW_TY;
       const int gx = x0 + 4 + lx, gy = y0 + 2 + ly, gz = z0 + 2 + lz;
       if (gx >= a.d0 || gy >= a.d1 || gz >= a.d2) continue;
@@ -443,13 +504,23 @@ void launch_mg_sweep(const MgLevel &L, const float *pin, float *pout, float omeg
   LvArgs a = lv_args(L);
   int ntx = (L.d[0] + SW_TX - 1) / SW_TX, nty = (L.d[1] + SW_TY - 1) / SW_TY, ntz = (L.d[2] + SW_TZ - 1) / SW_TZ;
   int nb = ntx * nty * ntz;
-  bool vec = (L.d[0] % 4) == 0;
-  if (fine && vec)
-    k_mg_sweep<true, true><<<nb, SW_NT, 0, s>>>(a, ntx, nty, L.b, L.nc, pin, pout, omega);
-  else if (vec)
-    k_mg_sweep<false, true><<<nb, SW_NT, 0, s>>>(a, ntx, nty, L.b, L.nc, pin, pout, omega);
-  else
-    k_mg_sweep<false, false><<<nb, SW_NT, 0, s>>>(a, ntx, nty, L.b, L.nc, pin, pout, omega);
+  // VEC: whole float4 rows, buffer loads with 32-bit byte offsets below 2 GiB
+  bool vec = (L.d[0] % 4) == 0 && (uint64_t)L.n * 4u < (1ull << 31);
+  if (L.wdir == 1.0f) {
+    if (fine && vec)
+      k_mg_sweep<true, true, true><<<nb, SW_NT, 0, s>>>(a, ntx, nty, L.b, L.nc, pin, pout, omega);
+    else if (vec)
+      k_mg_sweep<false, true, true><<<nb, SW_NT, 0, s>>>(a, ntx, nty, L.b, L.nc, pin, pout, omega);
+    else
+      k_mg_sweep<false, false, true><<<nb, SW_NT, 0, s>>>(a, ntx, nty, L.b, L.nc, pin, pout, omega);
+  } else {
+    if (fine && vec)
+      k_mg_sweep<true, true, false><<<nb, SW_NT, 0, s>>>(a, ntx, nty, L.b, L.nc, pin, pout, omega);
+    else if (vec)
+      k_mg_sweep<false, true, false><<<nb, SW_NT, 0, s>>>(a, ntx, nty, L.b, L.nc, pin, pout, omega);
+    else
+      k_mg_sweep<false, false, false><<<nb, SW_NT, 0, s>>>(a, ntx, nty, L.b, L.nc, pin, pout, omega);
+  }
 }
 
 // ---------------------------------------------------------------- coarse
@@ -548,6 +619,114 @@ __global__ __launch_bounds__(CO_NT) void k_mg_coarse(LvArgs a, const float *__re
   }
 }
 
+// The same solve with one thread per (x, y) column and the column's phi in
+// registers, for an outside weight of 1 (integer counts 1..6, div_count).
+// LDS holds phi z-fastest (rows of PZ = d2 + 2 rounded up to odd, so the 64
+// lanes of a wave, at consecutive x, fall in distinct banks); a cell reads
+// its four x/y neighbours there and its two z neighbours from registers.
+// Register slot j of a column holds z = j - ((x + y) & 1), so the cells of
+// colour c are the slots j = c, c + 2, ... on every lane; slots outside the
+// column hold phi = 0 and count 0 (never updated), which gives the Dirichlet
+// phi = 0 above the top and below the bottom as the ghost ring does.
+template <int ZJ>
+__global__ __launch_bounds__(CO_NT) void k_mg_coarse_z(LvArgs a, const float *__restrict__ b,
+                                                        const uint8_t *__restrict__ f,
+                                                        const uint8_t *__restrict__ nc, float *__restrict__ p,
+                                                        int iters, float omega) {
+  __shared__ float sp[CO_PAD_MAX];
+  const int PZ = (a.d2 + 2) | 1, PX = a.d0 + 2, PY = a.d1 + 2;
+  const int SY = PX * PZ, PN = PY * SY;
+  const uint32_t sy = (uint32_t)a.d0, sz = (uint32_t)a.d0 * (uint32_t)a.d1;
+  for (int i = threadIdx.x; i < PN; i += CO_NT) sp[i] = 0.0f;
+  const int col = threadIdx.x;
+  const bool has = col < a.d0 * a.d1;
+  const int x = has ? col % a.d0 : 0, y = has ? col / a.d0 : 0;
+  const int par = (x + y) & 1;
+  // LDS index of slot j is B + j
+  const int B = ((y + 1) * PX + (x + 1)) * PZ + 1 - par;
+  float ph[ZJ + 2], hb[ZJ], rc[ZJ];
+  uint32_t cp[(ZJ + 3) / 4];
+  ph[0] = 0.0f;
+  ph[ZJ + 1] = 0.0f;
+#pragma unroll
+  for (int k = 0; k < (ZJ + 3) / 4; k++) cp[k] = 0u;
+#pragma unroll
+  for (int j = 0; j < ZJ; j++) {
+    const int z = j - par;
+    float v = 0.0f, h = 0.0f, r = 1.0f;
+    uint32_t cnt = 0u;
+    if (has && z >= 0 && z < a.d2) {
+      const uint32_t g = (uint32_t)x + sy * (uint32_t)y + sz * (uint32_t)z;
+      const uint8_t code = nc[g];
+      if (!(code & NC_SOLID)) v = p[g];
+      if (!(code & (NC_SOLID | NC_NONE))) {
+        cnt = (code & 7u) + ((code >> 3) & 7u);
+        h = a.hh * b[g];
+        r = 1.0f / (float)cnt;
+      }
+    }
+    ph[j + 1] = v;
+    hb[j] = h;
+    rc[j] = r;
+    cp[j >> 2] |= cnt << (8 * (j & 3));
+  }
+  __syncthreads();  // the zero fill is complete
+  if (has) {
+#pragma unroll
+    for (int j = 0; j < ZJ; j++) {
+      const int z = j - par;
+      if (z >= 0 && z < a.d2) sp[B + j] = ph[j + 1];
+    }
+  }
+  __syncthreads();
+  const float *sxp = sp + B + PZ, *sxm = sp + B - PZ, *syp = sp + B + SY, *sym = sp + B - SY;
+  for (int it = 0; it < iters; it++) {
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+#pragma unroll
+      for (int k = 0; k < (ZJ + 3) / 4; k++) asm volatile("" : "+v"(cp[k]));  // conversions stay in the loop
+      // every slot of the colour is computed (reads stay inside sp: the
+      // host keeps PN + ZJ within it) and kept only where the count is not
+      // 0; the stores follow all reads of the pass, so no read waits on them
+      // the next slot's four reads are issued before this slot's arithmetic
+      float nb[4] = {sxp[c], sxm[c], syp[c], sym[c]};
+#pragma unroll
+      for (int j = c; j < ZJ; j += 2) {
+        float nx[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (j + 2 < ZJ) {
+          nx[0] = sxp[j + 2]; nx[1] = sxm[j + 2]; nx[2] = syp[j + 2]; nx[3] = sym[j + 2];
+        }
+        const float cn = (float)((cp[j >> 2] >> (8 * (j & 3))) & 0xffu);
+        float sum = 0.0f;
+        sum += nb[0];
+        sum += nb[1];
+        sum += nb[2];
+        sum += nb[3];
+        sum += ph[j + 2];
+        sum += ph[j];
+        const float nv = mixf_(ph[j + 1], div_count(sum - hb[j], cn, rc[j]), omega);
+        ph[j + 1] = cn != 0.0f ? nv : ph[j + 1];
+#pragma unroll
+        for (int k = 0; k < 4; k++) nb[k] = nx[k];
+      }
+#pragma unroll
+      for (int j = c; j < ZJ; j += 2)
+        if ((cp[j >> 2] >> (8 * (j & 3))) & 0xffu) sp[B + j] = ph[j + 1];
+      __syncthreads();
+    }
+  }
+  if (has) {
+#pragma unroll
+    for (int j = 0; j < ZJ; j++) {
+      const int z = j - par;
+      if (z >= 0 && z < a.d2) {
+        const uint32_t g = (uint32_t)x + sy * (uint32_t)y + sz * (uint32_t)z;
+        if (f[g] != 1) p[g] = ph[j + 1];
+      }
+    }
+  }
+}
+
 // Same schedule for a coarsest level larger than LDS: one workgroup, phi in
 // global memory (L2-resident), a workgroup barrier between colour passes.
 __global__ __launch_bounds__(1024) void k_mg_coarse_global(LvArgs a, const float *__restrict__ b,
@@ -582,6 +761,17 @@ bool mg_coarse_fits_lds(const MgLevel &L) {
 
 void launch_mg_coarse(const MgLevel &L, int iters, float omega, hipStream_t s) {
   LvArgs a = lv_args(L);
+  const uint32_t pz = (uint32_t)((L.d[2] + 2) | 1);
+  const bool colz = L.wdir == 1.0f && L.d[0] * L.d[1] <= CO_NT &&
+                    pz * (uint32_t)(L.d[0] + 2) * (uint32_t)(L.d[1] + 2) + 24u <= (uint32_t)CO_PAD_MAX;
+  if (colz && L.d[2] + 1 <= 12) {
+    k_mg_coarse_z<12><<<1, CO_NT, 0, s>>>(a, L.b, L.f, L.nc, L.phi, iters, omega);
+    return;
+  }
+  if (colz && L.d[2] + 1 <= 22) {
+    k_mg_coarse_z<22><<<1, CO_NT, 0, s>>>(a, L.b, L.f, L.nc, L.phi, iters, omega);
+    return;
+  }
   if (mg_coarse_fits_lds(L))
     k_mg_coarse<<<1, CO_NT, 0, s>>>(a, L.b, L.f, L.nc, L.phi, iters, omega);
   else
