@@ -129,7 +129,26 @@ struct Geo {
   int vx, vy;              // velocity padded dims x, y (nx+1, ny+1)
   uint32_t n, nv;          // cell / padded-face counts
   FastDiv dnx, dny, dvx, dvy;
+  // brick grids (BR_X x BR_Y x BR_Z) over the cells and the padded faces
+  uint32_t cbricks, fbricks;
+  FastDiv dcbx, dcby, dfbx, dfby;
 };
+
+// Brick order for the gather-heavy advection kernels: block b (after
+// xcd_tile) takes brick b of the brick grid (x fastest, then y, then z) and
+// thread t its cell (t % 16, (t / 16) % 2, t / 32).  A block's trilinear
+// samples touch the rows and planes around its brick, which the blocks in
+// flight on the same XCD share: consecutive blocks walk x, then y, and a
+// plane is fetched again only at the edges of an 8-plane brick layer.
+constexpr int BR_X = 16, BR_Y = 2, BR_Z = 8;
+__device__ __forceinline__ void brick_cell(uint32_t b, uint32_t t, const FastDiv &dbx, const FastDiv &dby, int &x,
+                                           int &y, int &z) {
+  const uint32_t r = fdiv(b, dbx), bx = b - r * dbx.d;
+  const uint32_t bz = fdiv(r, dby), by = r - bz * dby.d;
+  x = (int)(bx * BR_X + (t & (BR_X - 1)));
+  y = (int)(by * BR_Y + ((t / BR_X) & (BR_Y - 1)));
+  z = (int)(bz * BR_Z + t / (BR_X * BR_Y));
+}
 
 __device__ __forceinline__ bool inb(const Geo &g, int x, int y, int z) {
   return x >= 0 && y >= 0 && z >= 0 && x < g.nx && y < g.ny && z < g.nz;

@@ -16,11 +16,18 @@ struct VelPtrs {
   const float *__restrict__ w;
 };
 
+static_assert(BR_X * BR_Y * BR_Z == NT, "one brick per block");
+
 // Read-only arrays are gathered through buffer resources (RB, common.hpp):
 // a sample's eight loads share one 32-bit byte offset (+4 and the row/plane
 // strides), instead of eight 64-bit address computations.
 __device__ __forceinline__ float ldf(const RB &b, uint32_t idx) {
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(b.r, idx * 4u, 0, 0));
+}
+// Elements idx and idx + 1 in one 8-byte load (4-byte alignment is all a
+// multi-dword buffer load needs): the x-pair of a trilinear sample.
+__device__ __forceinline__ float2 ld2(const RB &b, uint32_t idx) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(b.r, idx * 4u, 0, 0));
 }
 __device__ __forceinline__ float4 ld4(const RB &b, uint32_t idx) {
   return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(b.r, idx * 16u, 0, 0));
@@ -45,10 +52,11 @@ __device__ __forceinline__ float triC(const Geo &g, const RB &X, float gx0, floa
   float fx = gx - (float)bx, fy = gy - (float)by, fz = gz - (float)bz;
   uint32_t sy = (uint32_t)g.vx, sz = (uint32_t)g.vx * (uint32_t)g.vy;
   uint32_t i = vidx(g, bx, by, bz);
-  float c00 = mixf_(ldf(X, i), ldf(X, i + 1), fx);
-  float c10 = mixf_(ldf(X, i + sy), ldf(X, i + sy + 1), fx);
-  float c01 = mixf_(ldf(X, i + sz), ldf(X, i + sz + 1), fx);
-  float c11 = mixf_(ldf(X, i + sy + sz), ldf(X, i + sy + sz + 1), fx);
+  const float2 p00 = ld2(X, i), p10 = ld2(X, i + sy), p01 = ld2(X, i + sz), p11 = ld2(X, i + sy + sz);
+  float c00 = mixf_(p00.x, p00.y, fx);
+  float c10 = mixf_(p10.x, p10.y, fx);
+  float c01 = mixf_(p01.x, p01.y, fx);
+  float c11 = mixf_(p11.x, p11.y, fx);
   return mixf_(mixf_(c00, c10, fy), mixf_(c01, c11, fy), fz);
 }
 // 8-corner min/max at the sample's base cell (tmpl_comp.wgsl:16-25).
@@ -61,13 +69,15 @@ __device__ __forceinline__ void mmC(const Geo &g, const RB &X, float gx0, float 
   bz = bz > hz - 1 ? hz - 1 : bz;
   uint32_t sy = (uint32_t)g.vx, sz = (uint32_t)g.vx * (uint32_t)g.vy;
   uint32_t i = vidx(g, bx, by, bz);
-  const uint32_t o[8] = {0, 1, sy, sy + 1, sz, sz + 1, sy + sz, sy + sz + 1};
+  const uint32_t o[4] = {0, sy, sz, sy + sz};
   float l = 1e30f, h = -1e30f;
 #pragma unroll
-  for (int j = 0; j < 8; j++) {
-    float v = ldf(X, i + o[j]);
-    l = fmin_(l, v);
-    h = fmax_(h, v);
+  for (int j = 0; j < 4; j++) {
+    const float2 v = ld2(X, i + o[j]);
+    l = fmin_(l, v.x);
+    h = fmax_(h, v.x);
+    l = fmin_(l, v.y);
+    h = fmax_(h, v.y);
   }
   lo = l;
   hi = h;
@@ -110,11 +120,11 @@ __device__ __forceinline__ void trace2(const Geo &g, const VelR &V, float k, flo
 __global__ __launch_bounds__(NT) void k_adv_vel1(const Params *__restrict__ P, Geo g, VelPtrs VP,
                                                   float *__restrict__ hu, float *__restrict__ hv,
                                                   float *__restrict__ hw) {
-  uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
-  if (i >= g.nv) return;
-  const VelR V = velr_of(g, VP);
   int x, y, z;
-  decode_face(g, i, x, y, z);
+  brick_cell((uint32_t)xcd_tile(blockIdx.x, gridDim.x), threadIdx.x, g.dfbx, g.dfby, x, y, z);
+  if (x >= g.vx || y >= g.vy || z > g.nz) return;
+  const uint32_t i = vidx(g, x, y, z);
+  const VelR V = velr_of(g, VP);
   const float k = P->dt / P->dx;
   float bx, by, bz;
   if (x <= g.nx && y <= g.ny - 1 && z <= g.nz - 1) {
@@ -132,7 +142,7 @@ __global__ __launch_bounds__(NT) void k_adv_vel1(const Params *__restrict__ P, G
 }
 void launch_adv_vel1(const Fields &f, hipStream_t s) {
   VelPtrs V{f.va[0], f.va[1], f.va[2]};
-  k_adv_vel1<<<nblocks(f.g.nv), NT, 0, s>>>(f.P, f.g, V, f.hat[0], f.hat[1], f.hat[2]);
+  k_adv_vel1<<<f.g.fbricks, NT, 0, s>>>(f.P, f.g, V, f.hat[0], f.hat[1], f.hat[2]);
 }
 
 // ---------------------------------------------------------------- adv_vel2
@@ -145,12 +155,12 @@ __global__ __launch_bounds__(NT) void k_adv_vel2(const Params *__restrict__ P, G
                                                   const float *__restrict__ hw, float *__restrict__ ou,
                                                   float *__restrict__ ov, float *__restrict__ ow,
                                                   const uint8_t *__restrict__ fl) {
-  uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
-  if (i >= g.nv) return;
+  int x, y, z;
+  brick_cell((uint32_t)xcd_tile(blockIdx.x, gridDim.x), threadIdx.x, g.dfbx, g.dfby, x, y, z);
+  if (x >= g.vx || y >= g.vy || z > g.nz) return;
+  const uint32_t i = vidx(g, x, y, z);
   const VelR V = velr_of(g, VP);
   const RB H[3] = {rb_of(hu, g.nv * 4u), rb_of(hv, g.nv * 4u), rb_of(hw, g.nv * 4u)};
-  int x, y, z;
-  decode_face(g, i, x, y, z);
   const float k = P->dt / P->dx;
   const bool sc = solidC(g, fl, x, y, z);
 #pragma unroll 1
@@ -178,7 +188,7 @@ __global__ __launch_bounds__(NT) void k_adv_vel2(const Params *__restrict__ P, G
 }
 void launch_adv_vel2(const Fields &f, hipStream_t s) {
   VelPtrs V{f.va[0], f.va[1], f.va[2]};
-  k_adv_vel2<<<nblocks(f.g.nv), NT, 0, s>>>(f.P, f.g, V, f.hat[0], f.hat[1], f.hat[2], f.vb[0], f.vb[1], f.vb[2],
+  k_adv_vel2<<<f.g.fbricks, NT, 0, s>>>(f.P, f.g, V, f.hat[0], f.hat[1], f.hat[2], f.vb[0], f.vb[1], f.vb[2],
                                             f.flags);
 }
 
@@ -235,12 +245,12 @@ __device__ __forceinline__ float outside(const Geo &g, float x, float y, float z
 __global__ __launch_bounds__(NT) void k_adv_s1(const Params *__restrict__ P, Geo g, VelPtrs VP,
                                                 const float4 *__restrict__ s0p, const float4 *__restrict__ s1p,
                                                 float4 *__restrict__ h0, float4 *__restrict__ h1) {
-  uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
-  if (i >= g.n) return;
+  int x, y, z;
+  brick_cell((uint32_t)xcd_tile(blockIdx.x, gridDim.x), threadIdx.x, g.dcbx, g.dcby, x, y, z);
+  if (x >= g.nx || y >= g.ny || z >= g.nz) return;
+  const uint32_t i = sidx(g, x, y, z);
   const VelR V = velr_of(g, VP);
   const RB s0 = rb_of(s0p, g.n * 16u), s1 = rb_of(s1p, g.n * 16u);
-  int x, y, z;
-  decode_cell(g, i, x, y, z);
   const float k = P->dt / P->dx;
   float bx, by, bz;
   trace<-1>(g, V, k, (float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f, bx, by, bz);
@@ -254,7 +264,7 @@ __global__ __launch_bounds__(NT) void k_adv_s1(const Params *__restrict__ P, Geo
 }
 void launch_adv_s1(const Fields &f, hipStream_t s) {
   VelPtrs V{f.va[0], f.va[1], f.va[2]};
-  k_adv_s1<<<nblocks(f.g.n), NT, 0, s>>>(f.P, f.g, V, f.s0, f.s1, f.h0, f.h1);
+  k_adv_s1<<<f.g.cbricks, NT, 0, s>>>(f.P, f.g, V, f.s0, f.s1, f.h0, f.h1);
 }
 
 __device__ __forceinline__ float mcl(float r, float s, float t, float lo, float hi) {
@@ -265,13 +275,13 @@ __global__ __launch_bounds__(NT) void k_adv_s2(const Params *__restrict__ P, Geo
                                                 const float4 *__restrict__ s0p, const float4 *__restrict__ s1p,
                                                 const float4 *__restrict__ h0p, const float4 *__restrict__ h1p,
                                                 float4 *__restrict__ o0, float4 *__restrict__ o1) {
-  uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
-  if (i >= g.n) return;
+  int x, y, z;
+  brick_cell((uint32_t)xcd_tile(blockIdx.x, gridDim.x), threadIdx.x, g.dcbx, g.dcby, x, y, z);
+  if (x >= g.nx || y >= g.ny || z >= g.nz) return;
+  const uint32_t i = sidx(g, x, y, z);
   const VelR V = velr_of(g, VP);
   const RB s0 = rb_of(s0p, g.n * 16u), s1 = rb_of(s1p, g.n * 16u), h0 = rb_of(h0p, g.n * 16u),
            h1 = rb_of(h1p, g.n * 16u);
-  int x, y, z;
-  decode_cell(g, i, x, y, z);
   const float k = P->dt / P->dx;
   float px = (float)x + 0.5f, py = (float)y + 0.5f, pz = (float)z + 0.5f;
   float bx, by, bz, fx, fy, fz;
@@ -296,7 +306,7 @@ __global__ __launch_bounds__(NT) void k_adv_s2(const Params *__restrict__ P, Geo
 }
 void launch_adv_s2(const Fields &f, hipStream_t s) {
   VelPtrs V{f.va[0], f.va[1], f.va[2]};
-  k_adv_s2<<<nblocks(f.g.n), NT, 0, s>>>(f.P, f.g, V, f.s0, f.s1, f.h0, f.h1, f.s0b, f.s1b);
+  k_adv_s2<<<f.g.cbricks, NT, 0, s>>>(f.P, f.g, V, f.s0, f.s1, f.h0, f.h1, f.s0b, f.s1b);
 }
 
 }  // namespace pyro

@@ -237,6 +237,15 @@ int pyro_sim_create(const void *params, size_t params_bytes, const float *sol_in
   g.n = (uint32_t)n64; g.nv = (uint32_t)nv64;
   g.dnx = make_fastdiv((uint32_t)g.nx); g.dny = make_fastdiv((uint32_t)g.ny);
   g.dvx = make_fastdiv((uint32_t)g.vx); g.dvy = make_fastdiv((uint32_t)g.vy);
+  {
+    auto cdiv = [](uint32_t a, uint32_t b) { return (a + b - 1) / b; };
+    const uint32_t cbx = cdiv(g.nx, BR_X), cby = cdiv(g.ny, BR_Y), cbz = cdiv(g.nz, BR_Z);
+    const uint32_t fbx = cdiv(g.vx, BR_X), fby = cdiv(g.vy, BR_Y), fbz = cdiv(g.nz + 1, BR_Z);
+    g.cbricks = cbx * cby * cbz;
+    g.fbricks = fbx * fby * fbz;
+    g.dcbx = make_fastdiv(cbx); g.dcby = make_fastdiv(cby);
+    g.dfbx = make_fastdiv(fbx); g.dfby = make_fastdiv(fby);
+  }
   s->cycles = pressure_cycles;
   s->ign_until = ign_until;
   s->use_graph = !(cflags & PYRO_CREATE_NO_GRAPH);
