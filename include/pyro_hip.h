@@ -163,8 +163,23 @@ int pyro_poisson_levels(const PyroPoisson *p, int *ref_levels, int *pc_levels);
 int pyro_poisson_set(PyroPoisson *p, const double *b, const double *phi);
 int pyro_poisson_get(PyroPoisson *p, double *phi);
 int pyro_poisson_residual(PyroPoisson *p, double *rel);
-/* Iterate until the relative residual is <= tol or max_iters outer iterations. */
+/* Iterate until the relative residual is <= tol or max_iters outer iterations.
+ * Sealed fluid components (pure-Neumann pockets: face-connected fluid cells
+ * none of which lies on a side or the top) make the system singular and, with
+ * a right-hand side that does not sum to zero over them, inconsistent: no
+ * method can then reduce the residual below that part of b.  Unless disabled
+ * (pyro_poisson_set_sealed_projection), a solve first subtracts the mean of b
+ * over each sealed component, once per right-hand side; the residual is
+ * relative to that b.  The components are found on the device when the solid
+ * flags change. */
 int pyro_poisson_solve(PyroPoisson *p, int method, double tol, int max_iters, PyroSolveInfo *info);
+/* mode 0: never project; 1 (default): for every method but PYRO_SOLVER_VCYCLE
+ * (which stays the reference's); 2: for every method. */
+int pyro_poisson_set_sealed_projection(PyroPoisson *p, int mode);
+/* Number of sealed fluid components and of their cells under the current flags. */
+int pyro_poisson_sealed_info(const PyroPoisson *p, uint32_t *components, uint32_t *cells);
+/* The right-hand side as the solver holds it (after any projection). */
+int pyro_poisson_get_rhs(PyroPoisson *p, double *b);
 
 /* Pressure problem of a simulation (same grid): solid flags from its level-0
  * flags; b = its current level-0 right-hand side (set by the divergence

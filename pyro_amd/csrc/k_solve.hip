@@ -170,6 +170,10 @@ __global__ __launch_bounds__(NT) void k_zero64(uint32_t n, double *__restrict__ 
   if (i < n) a[i] = 0.0;
 }
 void launch_zero64(uint32_t n, double *a, hipStream_t s) { k_zero64<<<nblocks(n), NT, 0, s>>>(n, a); }
+__global__ __launch_bounds__(NT) void k_zero_u8(uint32_t n, uint8_t *a) {
+  const uint32_t i = blockIdx.x * NT + threadIdx.x;
+  if (i < n) a[i] = 0;
+}
 
 // ---------------------------------------------------------------- fused CG steps
 // Per-block FP64 partial sums go to part[k * nb + block] (nb = blocks of the
@@ -308,6 +312,154 @@ void launch_to_f64_dot(uint32_t n, const float *phi, const double *q, double *z,
 }
 void launch_xpby_dev(uint32_t n, const double *z, const double *dev, double *p, hipStream_t s) {
   k_xpby_dev<<<nblocks(n), NT, 0, s>>>(n, z, dev, p);
+}
+
+
+// ---------------------------------------------------------------- sealed fluid components
+// Connected components of the fluid cells (6-neighbour) by union-find:
+// parent[i] starts at i; every fluid cell unites itself with its fluid +x,
+// +y, +z neighbours, always hanging the larger root under the smaller by a
+// compare-and-swap on the root.  A link only ever points a cell at a smaller
+// index, so a find walks strictly decreasing indices and a failed
+// compare-and-swap restarts from a strictly smaller root: both loops end.
+// Afterwards every path is compressed; the root of a component is then its
+// smallest cell index, whatever the order of the links.  A component is
+// open when one of its cells lies on a side (x = 0 or d0-1, z = 0 or d2-1)
+// or at the top (y = d1-1), where phi = 0 lies beyond the face, and sealed
+// otherwise: a pure-Neumann pocket, consistent only when b sums to zero.
+__device__ __forceinline__ uint32_t cc_load(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t cc_find(uint32_t *parent, uint32_t x) {
+  uint32_t p = cc_load(parent + x);
+  while (p != x) {  // path halving: parent[x] is replaced by an ancestor
+    const uint32_t gp = cc_load(parent + p);
+    if (gp == p) return p;
+    __hip_atomic_store(parent + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    x = gp;
+    p = cc_load(parent + x);
+  }
+  return x;
+}
+// without path halving: in the compression pass the only store to
+// parent[i] must be thread i's own (the root), or a halving store of another
+// thread passing through i could leave an intermediate ancestor there
+__device__ __forceinline__ uint32_t cc_find_ro(const uint32_t *parent, uint32_t x) {
+  uint32_t p = cc_load(parent + x);
+  while (p != x) {
+    x = p;
+    p = cc_load(parent + x);
+  }
+  return x;
+}
+__device__ __forceinline__ void cc_unite(uint32_t *parent, uint32_t a, uint32_t b) {
+  uint32_t ra = cc_find(parent, a), rb = cc_find(parent, b);
+  while (ra != rb) {
+    if (ra < rb) {
+      const uint32_t t = ra;
+      ra = rb;
+      rb = t;
+    }
+    const uint32_t old = atomicCAS(parent + ra, ra, rb);
+    if (old == ra) return;
+    ra = cc_find(parent, old);  // ra was linked meanwhile: go on from its root
+    rb = cc_find(parent, rb);
+  }
+}
+struct CcDims {
+  int d0, d1, d2;
+  uint32_t n;
+  FastDiv dd0, dd1;
+};
+CcDims cc_dims(const MgLevel &L) {
+  CcDims c;
+  c.d0 = L.d[0]; c.d1 = L.d[1]; c.d2 = L.d[2];
+  c.n = L.n;
+  c.dd0 = make_fastdiv((uint32_t)L.d[0]);
+  c.dd1 = make_fastdiv((uint32_t)L.d[1]);
+  return c;
+}
+__global__ __launch_bounds__(NT) void k_cc_init(uint32_t n, uint32_t *parent) {
+  const uint32_t i = blockIdx.x * NT + threadIdx.x;
+  if (i < n) parent[i] = i;
+}
+__global__ __launch_bounds__(NT) void k_cc_link(CcDims c, const uint8_t *__restrict__ f, uint32_t *parent) {
+  const uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
+  if (i >= c.n || f[i] == 1) return;
+  const uint32_t r = fdiv(i, c.dd0), x = i - r * c.dd0.d, z = fdiv(r, c.dd1), y = r - z * c.dd1.d;
+  const uint32_t sy = (uint32_t)c.d0, sz = (uint32_t)c.d0 * (uint32_t)c.d1;
+  if (x + 1 < (uint32_t)c.d0 && f[i + 1] != 1) cc_unite(parent, i, i + 1);
+  if (y + 1 < (uint32_t)c.d1 && f[i + sy] != 1) cc_unite(parent, i, i + sy);
+  if (z + 1 < (uint32_t)c.d2 && f[i + sz] != 1) cc_unite(parent, i, i + sz);
+}
+// compress to the roots; mark the roots of components with an open face
+__global__ __launch_bounds__(NT) void k_cc_root_open(CcDims c, const uint8_t *__restrict__ f, uint32_t *parent,
+                                                     uint8_t *__restrict__ open) {
+  const uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
+  if (i >= c.n || f[i] == 1) return;
+  const uint32_t root = cc_find_ro(parent, i);
+  __hip_atomic_store(parent + i, root, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t r = fdiv(i, c.dd0), x = i - r * c.dd0.d, z = fdiv(r, c.dd1), y = r - z * c.dd1.d;
+  if (x == 0 || x + 1 == (uint32_t)c.d0 || z == 0 || z + 1 == (uint32_t)c.d2 || y + 1 == (uint32_t)c.d1)
+    open[root] = 1;
+}
+// (root, cell) of every fluid cell in a sealed component, in any order
+__global__ __launch_bounds__(NT) void k_cc_sealed(uint32_t n, const uint8_t *__restrict__ f,
+                                                  const uint32_t *__restrict__ parent,
+                                                  const uint8_t *__restrict__ open, uint2 *out,
+                                                  unsigned int *count) {
+  const uint32_t i = blockIdx.x * NT + threadIdx.x;
+  if (i >= n || f[i] == 1) return;
+  const uint32_t root = parent[i];
+  if (open[root]) return;
+  const unsigned int k = atomicAdd(count, 1u);
+  out[k] = make_uint2(root, i);
+}
+void launch_cc_label(const MgLevel &L, uint32_t *parent, uint8_t *open, hipStream_t s) {
+  const CcDims c = cc_dims(L);
+  k_cc_init<<<nblocks(L.n), NT, 0, s>>>(L.n, parent);
+  k_zero_u8<<<nblocks(L.n), NT, 0, s>>>(L.n, open);
+  k_cc_link<<<nblocks(L.n), NT, 0, s>>>(c, L.f, parent);
+  k_cc_root_open<<<nblocks(L.n), NT, 0, s>>>(c, L.f, parent, open);
+}
+void launch_cc_sealed(const MgLevel &L, const uint32_t *parent, const uint8_t *open, uint2 *out,
+                      unsigned int *count, hipStream_t s) {
+  k_cc_sealed<<<nblocks(L.n), NT, 0, s>>>(L.n, L.f, parent, open, out, count);
+}
+
+// b -= mean of b over each sealed component (segment k of idx: cells
+// off[k] .. off[k+1]-1, ascending).  One block per component; the sum is
+// formed in a fixed order, so the result is bit-reproducible.
+__global__ __launch_bounds__(NT) void k_seal_project(const uint32_t *__restrict__ off,
+                                                     const uint32_t *__restrict__ idx, double *b) {
+  __shared__ double sh[NT / 64];
+  __shared__ double mean_s;
+  const uint32_t a = off[blockIdx.x], e = off[blockIdx.x + 1];
+  double acc = 0.0;
+  for (uint32_t j = a + threadIdx.x; j < e; j += NT) acc += b[idx[j]];
+  const double sum = block_sum<NT>(acc, sh);  // valid in thread 0
+  if (threadIdx.x == 0) mean_s = sum / (double)(e - a);
+  __syncthreads();
+  const double mean = mean_s;
+  for (uint32_t j = a + threadIdx.x; j < e; j += NT) b[idx[j]] -= mean;
+}
+void launch_seal_project(uint32_t ncomp, const uint32_t *off, const uint32_t *idx, double *b, hipStream_t s) {
+  if (ncomp) k_seal_project<<<ncomp, NT, 0, s>>>(off, idx, b);
+}
+
+// changed |= (f != prev); prev = f
+__global__ __launch_bounds__(NT) void k_flags_changed(uint32_t n, const uint8_t *__restrict__ f, uint8_t *prev,
+                                                      unsigned int *changed) {
+  const uint32_t i = blockIdx.x * NT + threadIdx.x;
+  if (i >= n) return;
+  const uint8_t v = f[i];
+  if (prev[i] != v) {
+    prev[i] = v;
+    *changed = 1u;
+  }
+}
+void launch_flags_changed(uint32_t n, const uint8_t *f, uint8_t *prev, unsigned int *changed, hipStream_t s) {
+  k_flags_changed<<<nblocks(n), NT, 0, s>>>(n, f, prev, changed);
 }
 
 }  // namespace pyro

@@ -99,6 +99,15 @@ void launch_to_f64_dot(uint32_t n, const float *phi, const double *q, double *z,
                        hipStream_t s);
 void launch_xpby_dev(uint32_t n, const double *z, const double *dev, double *p, hipStream_t s);
 // r = b - A x, dev[2] = r.r
+// Sealed fluid components (k_solve.hip): union-find labels with every
+// fluid cell's parent at its component's smallest index, open[root] = 1 for
+// components touching a side or the top; (root, cell) pairs of the sealed
+// ones; b -= mean over each sealed component; a flag-change detector.
+void launch_cc_label(const MgLevel &L, uint32_t *parent, uint8_t *open, hipStream_t s);
+void launch_cc_sealed(const MgLevel &L, const uint32_t *parent, const uint8_t *open, uint2 *out,
+                      unsigned int *count, hipStream_t s);
+void launch_seal_project(uint32_t ncomp, const uint32_t *off, const uint32_t *idx, double *b, hipStream_t s);
+void launch_flags_changed(uint32_t n, const uint8_t *f, uint8_t *prev, unsigned int *changed, hipStream_t s);
 void launch_resid_rr(const MgLevel &L, const double *x, const double *b, double *r, double *part, double *dev,
                      hipStream_t s);
 

@@ -14,6 +14,7 @@
 //    (on level 0 it is half a fine cell beyond the wall).  The reference
 //    weight 1 moves that point outward on each coarser level, which is what
 //    slows its V-cycle to ~0.55 per cycle at 32^3 and stalls it at 64^3.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -57,6 +58,16 @@ struct PyroPoisson {
   int coarse_iters_pc = 80;
   PyroNet *net = nullptr;
   int learned_mode = 0;  // 0: network in every preconditioner application, 1: first only
+  // sealed fluid components (pure-Neumann pockets): b is made consistent by
+  // subtracting its mean over each (segment k of seal_idx: cells
+  // seal_off[k] .. seal_off[k+1]-1), once per right-hand side
+  uint8_t *fprev = nullptr;          // level-0 flags at the last labelling
+  uint8_t *cc_open = nullptr;        // open[root]
+  unsigned int *cc_flag = nullptr;   // [0] sealed-cell counter, [1] flags changed
+  uint32_t *seal_idx = nullptr, *seal_off = nullptr;
+  uint32_t seal_cap = 0, seal_comp = 0, seal_cells = 0;
+  int seal_mode = 1;                 // 0 never, 1 all methods but VCYCLE, 2 all
+  bool labelled = false, b_fresh = true;
 };
 
 namespace {
@@ -152,17 +163,73 @@ __global__ __launch_bounds__(256) void k_count_fluid(uint32_t n, const uint8_t *
   if (threadIdx.x == 0) atomicAdd(cnt, sh[0] + sh[1] + sh[2] + sh[3]);
 }
 
+// Sealed components of the level-0 fluid cells (launch_cc_label), sorted on
+// the host into per-component segments of ascending cell index.  The union-
+// find parents and the (root, cell) pairs use the learned preconditioner's
+// buffers w and t, which hold nothing between solves.
+int relabel(PyroPoisson *P) {
+  MgLevel &L0 = P->pc[0];
+  uint32_t *parent = reinterpret_cast<uint32_t *>(P->w);
+  uint2 *pairs = reinterpret_cast<uint2 *>(P->t);
+  launch_cc_label(L0, parent, P->cc_open, P->stream);
+  PCHK(hipMemsetAsync(P->cc_flag, 0, sizeof(unsigned int), P->stream));
+  launch_cc_sealed(L0, parent, P->cc_open, pairs, P->cc_flag, P->stream);
+  unsigned int cnt = 0;
+  PCHK(hipMemcpyAsync(&cnt, P->cc_flag, sizeof(cnt), hipMemcpyDeviceToHost, P->stream));
+  int rc = sync(P);
+  if (rc) return rc;
+  std::vector<uint2> h(cnt);
+  if (cnt) {
+    PCHK(hipMemcpyAsync(h.data(), pairs, cnt * sizeof(uint2), hipMemcpyDeviceToHost, P->stream));
+    rc = sync(P);
+    if (rc) return rc;
+  }
+  std::sort(h.begin(), h.end(), [](const uint2 &a, const uint2 &b) { return a.x != b.x ? a.x < b.x : a.y < b.y; });
+  std::vector<uint32_t> idx(cnt), off;
+  for (uint32_t k = 0; k < cnt; k++) {
+    if (k == 0 || h[k].x != h[k - 1].x) off.push_back(k);
+    idx[k] = h[k].y;
+  }
+  off.push_back(cnt);
+  if (cnt + 1 > P->seal_cap) {
+    if (P->seal_idx) (void)hipFree(P->seal_idx);
+    if (P->seal_off) (void)hipFree(P->seal_off);
+    P->seal_idx = P->seal_off = nullptr;
+    P->seal_cap = 0;
+    const uint32_t cap = cnt + 1 > 1024 ? cnt + 1 : 1024;
+    if (hipMalloc(&P->seal_idx, cap * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&P->seal_off, cap * sizeof(uint32_t)) != hipSuccess)
+      return pfail(PYRO_E_OOM, "hipMalloc failed (sealed components)");
+    P->seal_cap = cap;
+  }
+  if (cnt) PCHK(hipMemcpyAsync(P->seal_idx, idx.data(), cnt * sizeof(uint32_t), hipMemcpyHostToDevice, P->stream));
+  PCHK(hipMemcpyAsync(P->seal_off, off.data(), off.size() * sizeof(uint32_t), hipMemcpyHostToDevice, P->stream));
+  rc = sync(P);
+  if (rc) return rc;
+  P->seal_comp = (uint32_t)off.size() - 1;
+  P->seal_cells = cnt;
+  P->labelled = true;
+  P->b_fresh = true;
+  return 0;
+}
+
 int set_flags_device(PyroPoisson *P, const uint8_t *sim_flags) {
   k_flags_from_sim<<<nblocks(P->n), 256, 0, P->stream>>>(P->n, sim_flags, P->pc[0].f);
   launch_mg_setup(P->pc, P->nl_pc, P->stream);
-  PCHK(hipMemsetAsync(P->count, 0, sizeof(unsigned int), P->stream));
+  PCHK(hipMemsetAsync(P->count, 0, 2 * sizeof(unsigned int), P->stream));
   k_count_fluid<<<COUNT_BLOCKS, 256, 0, P->stream>>>(P->n, P->pc[0].f, P->count);
-  unsigned int c = 0;
-  PCHK(hipMemcpyAsync(&c, P->count, sizeof(c), hipMemcpyDeviceToHost, P->stream));
+  launch_flags_changed(P->n, P->pc[0].f, P->fprev, P->count + 1, P->stream);
+  unsigned int c[2] = {0, 0};
+  PCHK(hipMemcpyAsync(c, P->count, sizeof(c), hipMemcpyDeviceToHost, P->stream));
   int rc = sync(P);
   if (rc) return rc;
-  P->nfluid = (double)c;
+  P->nfluid = (double)c[0];
+  if (c[1] || !P->labelled) return relabel(P);
   return 0;
+}
+
+bool projects(const PyroPoisson *P, int method) {
+  return P->seal_mode == 2 || (P->seal_mode == 1 && method != PYRO_SOLVER_VCYCLE);
 }
 }  // namespace
 
@@ -222,7 +289,10 @@ int pyro_poisson_create(uint32_t nx, uint32_t ny, uint32_t nz, float h, const ui
   if (!rc) rc |= palloc(P, &P->z, P->n);
   if (!rc) rc |= palloc(P, &P->w, P->n);
   if (!rc) rc |= palloc(P, &P->t, P->n);
-  if (!rc) rc |= palloc(P, &P->count, 4);
+  if (!rc) rc |= palloc(P, &P->count, 8);
+  if (!rc) rc |= palloc(P, &P->cc_flag, 8);
+  if (!rc) rc |= palloc(P, &P->fprev, P->n);
+  if (!rc) rc |= palloc(P, &P->cc_open, P->n);
   if (!rc) rc |= palloc(P, &P->cgpart, 2 * (size_t)cg_part_blocks(P->n));
   if (!rc) rc |= palloc(P, &P->dev, 4);
   if (!rc) rc |= palloc(P, &P->part, DOT_PART);
@@ -233,10 +303,12 @@ int pyro_poisson_create(uint32_t nx, uint32_t ny, uint32_t nz, float h, const ui
     f8[i] = flags[i] == 1u ? 1 : 0;
     P->nfluid += f8[i] ? 0.0 : 1.0;
   }
-  if (hipMemcpy(P->pc[0].f, f8.data(), P->n, hipMemcpyHostToDevice) != hipSuccess)
+  if (hipMemcpy(P->pc[0].f, f8.data(), P->n, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(P->fprev, f8.data(), P->n, hipMemcpyHostToDevice) != hipSuccess)
     return bail(pfail(PYRO_E_HIP, "upload failed"));
   launch_mg_setup(P->pc, P->nl_pc, P->stream);
   if (sync(P)) return bail(PYRO_E_HIP);
+  if ((rc = relabel(P))) return bail(rc);
   *out = P;
   return 0;
 }
@@ -245,6 +317,8 @@ int pyro_poisson_destroy(PyroPoisson *P) {
   if (!P) return 0;
   if (P->stream) (void)hipStreamSynchronize(P->stream);
   for (void *v : P->allocs) (void)hipFree(v);
+  if (P->seal_idx) (void)hipFree(P->seal_idx);
+  if (P->seal_off) (void)hipFree(P->seal_off);
   if (P->stream) (void)hipStreamDestroy(P->stream);
   delete P;
   return 0;
@@ -260,6 +334,7 @@ int pyro_poisson_levels(const PyroPoisson *P, int *ref_levels, int *pc_levels) {
 int pyro_poisson_set(PyroPoisson *P, const double *b, const double *phi) {
   if (!P) return pfail(PYRO_E_INVALID, "null handle");
   if (b) PCHK(hipMemcpy(P->b, b, P->n * sizeof(double), hipMemcpyHostToDevice));
+  if (b) P->b_fresh = true;
   if (phi) PCHK(hipMemcpy(P->x, phi, P->n * sizeof(double), hipMemcpyHostToDevice));
   return 0;
 }
@@ -285,6 +360,10 @@ int pyro_poisson_solve(PyroPoisson *P, int method, double tol, int max_iters, Py
   PCHK(hipEventCreate(&e0));
   PCHK(hipEventCreate(&e1));
   PCHK(hipEventRecord(e0, P->stream));
+  if (P->b_fresh && P->seal_comp && projects(P, method)) {
+    launch_seal_project(P->seal_comp, P->seal_off, P->seal_idx, P->b, P->stream);
+    P->b_fresh = false;
+  }
   int it = 0, cycles = 0, rc = 0, fallbacks = 0;
   double rel = 0.0;
   const uint32_t n = P->n;
@@ -400,6 +479,26 @@ int pyro_poisson_solve(PyroPoisson *P, int method, double tol, int max_iters, Py
   return 0;
 }
 
+int pyro_poisson_set_sealed_projection(PyroPoisson *P, int mode) {
+  if (!P || mode < 0 || mode > 2) return pfail(PYRO_E_INVALID, "bad argument");
+  P->seal_mode = mode;
+  return 0;
+}
+
+int pyro_poisson_sealed_info(const PyroPoisson *P, uint32_t *components, uint32_t *cells) {
+  if (!P) return pfail(PYRO_E_INVALID, "null handle");
+  if (components) *components = P->seal_comp;
+  if (cells) *cells = P->seal_cells;
+  return 0;
+}
+
+int pyro_poisson_get_rhs(PyroPoisson *P, double *b) {
+  if (!P || !b) return pfail(PYRO_E_INVALID, "null argument");
+  PCHK(hipStreamSynchronize(P->stream));
+  PCHK(hipMemcpy(b, P->b, P->n * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
 int pyro_poisson_attach_net(PyroPoisson *P, PyroNet *net, int mode) {
   if (!P || (mode != 0 && mode != 1)) return pfail(PYRO_E_INVALID, "bad argument");
   P->net = net;
@@ -428,6 +527,7 @@ int pyro_poisson_set_from_sim(PyroPoisson *P, PyroSim *sim, int warm) {
     return pfail(PYRO_E_INVALID, std::string("simulation: ") + pyro_last_error());
   if (nb != (size_t)P->n * 4) return pfail(PYRO_E_INVALID, "grid mismatch");
   launch_to_f64(P->n, static_cast<const float *>(bp), P->b, P->stream);
+  P->b_fresh = true;
   if (warm)
     launch_to_f64(P->n, static_cast<const float *>(pp), P->x, P->stream);
   else

@@ -52,6 +52,9 @@ def _lib():
         lib.pyro_poisson_residual.argtypes = [P, C.POINTER(C.c_double)]
         lib.pyro_poisson_solve.argtypes = [P, C.c_int, C.c_double, C.c_int, C.POINTER(_Info)]
         lib.pyro_poisson_attach_net.argtypes = [P, P, C.c_int]
+        lib.pyro_poisson_set_sealed_projection.argtypes = [P, C.c_int]
+        lib.pyro_poisson_sealed_info.argtypes = [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        lib.pyro_poisson_get_rhs.argtypes = [P, P]
         lib.pyro_poisson_set_flags_from_sim.argtypes = [P, P]
         lib.pyro_poisson_set_from_sim.argtypes = [P, P, C.c_int]
         lib.pyro_sim_step_solve.argtypes = [P, P, C.c_int, C.c_double, C.c_int, C.c_float, C.c_int,
@@ -110,6 +113,24 @@ class Poisson:
         out = np.zeros(self.shape, np.float64)
         _check(_lib().pyro_poisson_get(self._h, out.ctypes.data))
         return out
+
+    def get_rhs(self) -> np.ndarray:
+        """b as the solver holds it (after the sealed-component projection)."""
+        out = np.zeros(self.shape, np.float64)
+        _check(_lib().pyro_poisson_get_rhs(self._h, out.ctypes.data))
+        return out
+
+    def sealed_info(self) -> tuple[int, int]:
+        """(components, cells) of the sealed fluid components (pure-Neumann pockets)."""
+        k, c = C.c_uint32(), C.c_uint32()
+        _check(_lib().pyro_poisson_sealed_info(self._h, C.byref(k), C.byref(c)))
+        return k.value, c.value
+
+    def set_sealed_projection(self, mode: str = "solvers") -> None:
+        """Subtract b's mean over each sealed component before a solve:
+        "never", "solvers" (every method but the reference V-cycles; default)
+        or "all"."""
+        _check(_lib().pyro_poisson_set_sealed_projection(self._h, {"never": 0, "solvers": 1, "all": 2}[mode]))
 
     def residual(self) -> float:
         r = C.c_double()

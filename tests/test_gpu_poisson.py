@@ -81,3 +81,92 @@ def test_mgpcg_on_enclosed_pockets():
     P.set(b=b)
     info = P.solve("mgpcg", tol=1e-7, max_iters=300)
     assert info.rel_resid <= 1e-7, info
+
+
+def _projected(flags, b):
+    """b minus its mean over every sealed fluid component (scipy labels)."""
+    from pyro_amd.poisson import fluid_components
+
+    lab, n, open_ = fluid_components(flags)
+    out = b.copy()
+    sealed = 0
+    cells = 0
+    for k in range(1, n + 1):
+        if not open_[k]:
+            m = lab == k
+            out[m] -= b[m].mean()
+            sealed += 1
+            cells += int(m.sum())
+    return out, sealed, cells
+
+
+def _pocket_geometry():
+    nz, ny, nx = 36, 32, 40
+    f = np.zeros((nz, ny, nx), np.uint32)
+    f[4:16, 0:12, 4:18] = 1
+    f[5:15, 0:11, 5:17] = 0          # hollow box on the ground: sealed pocket (ground is Neumann)
+    f[20:30, 10:20, 20:32] = 1
+    f[21:29, 11:19, 21:31] = 0       # floating hollow box: sealed
+    f[24:26, 13:15, 24:26] = 1       # a solid inside it (pocket stays one component)
+    f[2:5, 20:23, 30:33] = 1
+    f[3, 21, 31] = 0                 # one trapped fluid cell
+    f[10:14, 20:24, 0:6] = 1
+    f[11:13, 21:23, 0:5] = 0         # box open to the x = 0 side: open
+    return f
+
+
+def test_sealed_components_match_scipy_and_make_b_consistent():
+    flags = _pocket_geometry()
+    rng = np.random.default_rng(11)
+    b = rng.normal(size=flags.shape)
+    b[flags == 1] = 0.0
+    ref, n_sealed, n_cells = _projected(flags, b)
+    assert n_sealed == 3
+    P = Poisson(flags, 0.05)
+    assert P.sealed_info() == (n_sealed, n_cells)
+    P.set(b=b)
+    info = P.solve("mgpcg", tol=1e-9, max_iters=300)
+    assert info.rel_resid <= 1e-9, info
+    got = P.get_rhs()
+    fl = flags == 0
+    assert np.abs(got - ref)[fl].max() <= 1e-12 * np.abs(ref).max()
+    # a second solve of the same right-hand side does not project again
+    P.solve("mgpcg", tol=1e-9, max_iters=300)
+    assert np.array_equal(P.get_rhs(), got)
+
+
+def test_reference_vcycles_keep_b_and_projection_modes():
+    flags = _pocket_geometry()
+    rng = np.random.default_rng(12)
+    b = rng.normal(size=flags.shape)
+    b[flags == 1] = 0.0
+    P = Poisson(flags, 0.05)
+    P.set(b=b)
+    P.solve("vcycle", tol=0.0, max_iters=2)
+    assert np.array_equal(P.get_rhs(), b)  # the reference method solves b as given
+    P.set_sealed_projection("never")
+    P.set(b=b)
+    info = P.solve("mgpcg", tol=1e-9, max_iters=30)
+    assert np.array_equal(P.get_rhs(), b)
+    assert info.rel_resid > 1e-3  # inconsistent: the sealed part of b cannot be removed
+    P.set_sealed_projection("all")
+    P.set(b=b)
+    P.solve("vcycle", tol=0.0, max_iters=1)
+    assert not np.array_equal(P.get_rhs(), b)
+
+
+@pytest.mark.parametrize("frac", [0.3, 0.55, 0.62])
+def test_sealed_labels_on_random_solids(frac):
+    # near the fluid percolation threshold: hundreds of pockets of up to ~25 cells
+    rng = np.random.default_rng(13)
+    flags = (rng.random((30, 34, 38)) < frac).astype(np.uint32)
+    b = rng.normal(size=flags.shape)
+    b[flags == 1] = 0.0
+    ref, n_sealed, n_cells = _projected(flags, b)
+    assert n_sealed >= 20
+    P = Poisson(flags, 0.05)
+    assert P.sealed_info() == (n_sealed, n_cells)
+    P.set(b=b)
+    P.solve("mgpcg", tol=0.0, max_iters=1)  # the projection happens at the start of a solve
+    fl = flags == 0
+    assert np.abs(P.get_rhs() - ref)[fl].max() <= 1e-12 * np.abs(ref).max()
