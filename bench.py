@@ -115,6 +115,33 @@ def gpu_run(args):
     copy_ms = pa.sim.debug_copy(64 << 20, 20)
     copy_gbs = 2 * (64 << 20) * 4 / (copy_ms * 1e-3) / 1e9
     log(f"[bench] plain copy: {copy_gbs:.0f} GB/s")
+    burning = None
+    if args.burn_time > 0:
+        # the same run continued to t = burn_time (the pool burns from about
+        # 1.2 s), then timed again: state-dependent work (pyrolysis, the
+        # solid pass, combustion) at full fire
+        t0 = time.perf_counter()
+        n_pre = 0
+        while sim.time < args.burn_time:
+            sim.step(args.dt, False)
+            n_pre += 1
+            if n_pre % 500 == 0:
+                sim.sync()
+                log(f"[bench] pool at t = {sim.time:.2f} s ({time.perf_counter() - t0:.1f} s)")
+        sim.step(args.dt, True)
+        sim.sync()
+        t1 = time.perf_counter()
+        for k in range(args.steps):
+            sim.step(args.dt, k % 10 == 9)
+        sim.sync()
+        wall_b = time.perf_counter() - t1
+        sb = sim.read_stats()
+        burning = {"t_start_s": round(float(sim.time) - args.steps * args.dt, 3), "steps": args.steps,
+                   "ms_per_step": 1e3 * wall_b / args.steps, "cell_updates_per_s": cells * args.steps / wall_b,
+                   "hrr_W": sb.hrr, "pyrolysis_kg_s": sb.pyro, "max_T": sb.max_t, "max_u": sb.max_u,
+                   "step_resid": sb.resid}
+        log(f"[bench] pool from t = {burning['t_start_s']} s: {burning['ms_per_step']:.3f} ms/step, "
+            f"HRR {sb.hrr / 1e3:.0f} kW")
     net, learn = (None, None) if args.no_pressure else learned_net(args)
     frame = None if args.no_pressure else pressure_frame(sim, args, p["dx"], net)
     if learn is not None:
@@ -124,7 +151,7 @@ def gpu_run(args):
             f"{learn['forward_tflops_pool_2cm']:.0f} TFLOP/s")
     return dict(cells=cells, wall=wall, stats=st, stage_ms=stage_ms, smooth_ms=smooth_ms,
                 coarse_ms=coarse_ms, achieved=achieved, algo_bytes=algo_bytes, dims=(nx, ny, nz),
-                copy_gbs=copy_gbs, frame=frame, learned=learn, net=net)
+                copy_gbs=copy_gbs, frame=frame, learned=learn, net=net, burning=burning)
 
 
 def _info(i):
@@ -146,8 +173,9 @@ def learned_net(args):
 
 def pressure_frame(sim, args, dx, net=None):
     """Time to a 1e-4 relative residual on the pressure problem of the next
-    step of the benchmark run: its right-hand side, solid flags and the warm
-    start the step itself would use (phi of the previous step).  Methods:
+    step of the benchmark run (the burning pool at t = --burn-time unless that
+    is 0): its right-hand side, solid flags and the warm start the step
+    itself would use (phi of the previous step).  Methods:
     the reference V-cycles (plain MG as the reference runs it), FP64-residual
     refinement (plain MG iteration on the consistent hierarchy), MGPCG, and
     MGPCG with the learned network (every preconditioner application, or
@@ -160,7 +188,8 @@ def pressure_frame(sim, args, dx, net=None):
         sim.run_stage(stg)
     P = Poisson(sim.read_field(pa.sim.level_field(0, "flags")), dx)
     P.set_from_sim(sim, warm=True)
-    out = {"tol": 1e-4, "start": "warm (phi of the previous step)", "initial_rel_resid": P.residual()}
+    out = {"tol": 1e-4, "start": "warm (phi of the previous step)", "t_s": round(float(sim.time), 3),
+           "initial_rel_resid": P.residual()}
     runs = [("vcycle", None, 400), ("refine", None, 100), ("mgpcg", None, 100)]
     if net is not None:
         runs += [("learned", "every", 100), ("learned", "first", 100)]
@@ -243,9 +272,34 @@ def cpu_baseline(args, dims):
         if el > args.cpu_seconds or n >= 50:
             break
     cells = dims[0] * dims[1] * dims[2]
-    return dict(value=cells * n / el, unit="cell-updates/s", cores=threads, kind="port",
-                sample=f"{n} full steps (after 1 warm step) of the same {dims[0]}x{dims[1]}x{dims[2]} pool "
-                       f"workload on the oracle (OpenMP, {threads} threads)")
+    out = dict(value=cells * n / el, unit="cell-updates/s", cores=threads, kind="port",
+               sample=f"{n} full steps (after 1 warm step) of the same {dims[0]}x{dims[1]}x{dims[2]} pool "
+                      f"workload on the oracle (OpenMP, {threads} threads)")
+    # the small pool (4 cm) on all threads and on one
+    del o
+    c1 = make_pool(0.04)
+    d1 = c1.grid().dims
+    n1cells = d1[0] * d1[1] * d1[2]
+    extra = {}
+    for th, budget in ((threads, 0.3 * args.cpu_seconds), (1, 0.5 * args.cpu_seconds)):
+        oracle.set_threads(th)
+        o = oracle.OSim(c1.params(), c1.voxelize(), None, 4, c1.get("ignition.until"))
+        o.add_velocity_noise(1e-3, 0)
+        o.step(args.dt, False)
+        k = 0
+        t0 = time.perf_counter()
+        while True:
+            o.step(args.dt, False)
+            k += 1
+            e = time.perf_counter() - t0
+            if e > budget or k >= 50:
+                break
+        log(f"[bench] cpu baseline {d1[0]}x{d1[1]}x{d1[2]}, {th} thread(s): {k} steps in {e:.1f} s")
+        extra[f"pool_4cm_{th}_threads"] = {"value": n1cells * k / e, "steps": k, "threads": th}
+        del o
+    oracle.set_threads(threads)
+    out["pool_4cm"] = extra
+    return out
 
 
 def bonfire_c5(args, net):
@@ -253,8 +307,11 @@ def bonfire_c5(args, net):
     160), pre-rolled with the reference step, checkpointed, then `c5_steps`
     steps per pressure method from that same state: the reference step
     (fixed 4 V-cycles), and pressure solves to 1e-4 by the reference V-cycles
-    (at most 50), refinement, MGPCG and learned MGPCG; wall time per step,
-    solver iterations and the FP64 statistics (HRR, pyrolysis, solid mass)."""
+    (at most 50) on b as the step forms it and on the consistent b (mean
+    removed on sealed pockets inside the log pile, where heat release makes
+    the system inconsistent), refinement, MGPCG and learned MGPCG (consistent
+    b); wall time per step, solver iterations and the FP64 statistics (HRR,
+    pyrolysis, solid mass)."""
     import tempfile
 
     import pyro_amd as pa
@@ -276,12 +333,16 @@ def bonfire_c5(args, net):
     out = {"grid": [nx, ny, nz], "cells": nx * ny * nz, "dt": dt, "preroll_steps": args.c5_preroll,
            "steps": args.c5_steps, "tol": 1e-4}
     P = Poisson(sim.read_field(pa.level_field(0, "flags")), s.grid().dx)
-    runs = [("reference_step", None, None), ("vcycle", "vcycle", 50), ("refine", "refine", 100),
-            ("mgpcg", "mgpcg", 100)]
+    runs = [("reference_step", None, None), ("vcycle", "vcycle", 50), ("vcycle_consistent", "vcycle", 50),
+            ("refine", "refine", 100), ("mgpcg", "mgpcg", 100)]
     if net is not None:
         runs.append(("learned_first", "learned", 100))
         P.attach_net(net, "first")
     for key, method, cap in runs:
+        # every method but the reference V-cycles solves the consistent
+        # problem (b minus its mean on sealed pockets); vcycle_consistent is
+        # the reference V-cycle on that same problem
+        P.set_sealed_projection("all" if key == "vcycle_consistent" else "solvers")
         sim.load(ck)
         its, worst, reached = [], 0.0, 0
         sim.sync()
@@ -302,7 +363,7 @@ def bonfire_c5(args, net):
              "solid_kg": st.solid, "max_T": st.max_t, "step_resid": st.resid}
         if its:
             d.update(mean_iterations=float(np.mean(its)), worst_rel_resid=worst,
-                     steps_reaching_tol=int(reached))
+                     steps_reaching_tol=int(reached), sealed_components=P.sealed_info()[0])
         out[key] = d
         log(f"[bench] bonfire {key}: {d}")
     P.close()
@@ -328,6 +389,8 @@ def main():
     ap.add_argument("--c5-preroll", type=int, default=3600,
                     help="bonfire steps before the comparison (3600 x 5 ms: the logs burn from about 12 s)")
     ap.add_argument("--c5-dt", type=float, default=0.005)
+    ap.add_argument("--burn-time", type=float, default=4.0,
+                    help="continue the C2 run to this time and time it again (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -384,6 +447,7 @@ def main():
         "coarse_solve_ms": r["coarse_ms"],
         "pressure": {"pool_frame": r["frame"], "standalone": c3, "learned_training": r["learned"],
                      "bonfire_10cm": c5},
+        "pool_burning": r["burning"],
         "copy_kernel_GBs": r["copy_gbs"],
         "last_stats": {"hrr_W": r["stats"].hrr, "max_T": r["stats"].max_t, "resid": r["stats"].resid},
     }
