@@ -308,8 +308,8 @@ constexpr int SW_Q = SW_PX / 4;  // float4 per footprint row
 // RN(1/count)) for div_count (code 0x40 / 0x80 reads (0, 0): a finite,
 // discarded result).
 template <bool ICNT>
-__device__ __forceinline__ void sweep_cell(float *sp, const uint8_t *sc, const float *sb, const float2 *ct, int pi,
-                                           int bi, float omega, float wdir) {
+__device__ __forceinline__ void sweep_cell(float *sp, const uint8_t *sc, float hb, const float2 *ct, int pi,
+                                           float omega, float wdir) {
   const uint32_t code = sc[pi];
   float sum = 0.0f;
   sum += sp[pi + 1];
@@ -321,30 +321,40 @@ __device__ __forceinline__ void sweep_cell(float *sp, const uint8_t *sc, const f
   float q;
   if (ICNT) {
     const float2 cr = ct[code & 63u];
-    q = div_count(sum - sb[bi], cr.x, cr.y);
+    q = div_count(sum - hb, cr.x, cr.y);
   } else {
-    q = (sum - sb[bi]) / nc_count(code, wdir);
+    q = (sum - hb) / nc_count(code, wdir);
   }
   const float nv = mixf_(sp[pi], q, omega);
   if (!(code & (NC_SOLID | NC_NONE))) sp[pi] = nv;
 }
 
+// h^2 b of the cell at grid (gx, gy, gz), read from global memory (0 outside
+// the grid: the cell is then never updated).  Not staged in LDS, which
+// keeps a block at 19.7 KB (8 blocks per CU).  VEC: through the buffer
+// resource (arrays below 2 GiB); otherwise a bounds-checked load.
+template <bool VEC>
+__device__ __forceinline__ float sweep_hb(const LvArgs &a, const RB &rbb, const float *__restrict__ b, int gx, int gy,
+                                          int gz) {
+  const bool ok = (uint32_t)gx < (uint32_t)a.d0 && (uint32_t)gy < (uint32_t)a.d1 && (uint32_t)gz < (uint32_t)a.d2;
+  const uint32_t g = (uint32_t)gx + (uint32_t)a.d0 * ((uint32_t)gy + (uint32_t)a.d1 * (uint32_t)gz);
+  if (VEC) return a.hh * __builtin_bit_cast(float, rb_ld32(rbb, ok ? 4u * g : RB_OOB));
+  return ok ? a.hh * b[g] : 0.0f;
+}
+
 // Staging of the VEC path: thread t < 240 loads x-group q = t % 10 of
 // footprint row ly = (t / 10) % 12 in planes (t / 120) + 2 it, it = 0..3
-// (LDS float4 index t + 240 it); h^2 b: thread t < 200, x-group t % 10, row
-// (t / 10) % 10 of the ring-1 box, planes (t / 100) + 2 it, it = 0..2 (LDS
-// index t + 200 it).  Loads outside the grid take the out-of-range offset
-// and read 0 (codes: 0x40, never updated).  The VEC path needs the level's
-// arrays below 2 GiB (launch_mg_sweep checks).
-constexpr int SW_PT = SW_Q * SW_PY * 2, SW_BT = SW_Q * SW_BY * 2;
-static_assert(SW_PZ == 8 && SW_BZ == 6 && SW_PT <= SW_NT && SW_BT <= SW_NT, "staging map");
+// (LDS float4 index t + 240 it).  Loads outside the grid take the
+// out-of-range offset and read 0 (codes: 0x40, never updated).  The VEC path
+// needs the level's arrays below 2 GiB (launch_mg_sweep checks).
+constexpr int SW_PT = SW_Q * SW_PY * 2;
+static_assert(SW_PZ == 8 && SW_PT <= SW_NT, "staging map");
 
 template <bool FINE, bool VEC, bool ICNT>
 __global__ __launch_bounds__(SW_NT) void k_mg_sweep(LvArgs a, int ntx, int nty, const float *__restrict__ b,
                                                      const uint8_t *__restrict__ nc, const float *__restrict__ pin,
                                                      float *__restrict__ pout, float omega) {
   __shared__ __align__(16) float sp[SW_PN];
-  __shared__ __align__(16) float sb[SW_BN];
   __shared__ __align__(16) uint8_t sc[SW_PN];
   __shared__ float2 ct[64];
   const int t = threadIdx.x;
@@ -356,9 +366,10 @@ __global__ __launch_bounds__(SW_NT) void k_mg_sweep(LvArgs a, int ntx, int nty, 
   const int tx = bid % ntx, rr = bid / ntx, ty = rr % nty, tz = rr / nty;
   const int x0 = tx * SW_TX - 4, y0 = ty * SW_TY - 2, z0 = tz * SW_TZ - 2;  // footprint origin, x0 % 4 == 0
   const int sy = a.d0, sz = a.d0 * a.d1;
+  const RB rbb = rb_of(b, VEC ? a.n * 4u : 0u);
   if (VEC) {
-    const RB rp = rb_of(pin, a.n * 4u), rcd = rb_of(nc, a.n), rbb = rb_of(b, a.n * 4u);
-    float4 pv[4], bv[3];
+    const RB rp = rb_of(pin, a.n * 4u), rcd = rb_of(nc, a.n);
+    float4 pv[4];
     uint32_t cv[4];
     {
       const int q = t % SW_Q, r0 = t / SW_Q, ly = r0 % SW_PY, lz0 = r0 / SW_PY;
@@ -373,18 +384,6 @@ __global__ __launch_bounds__(SW_NT) void k_mg_sweep(LvArgs a, int ntx, int nty, 
         pv[it] = rb_ld4(rp, ok ? 4u * g : RB_OOB);
         const uint32_t c = rb_ld32(rcd, ok ? g : RB_OOB);
         cv[it] = ok ? c : 0x40404040u;
-      }
-    }
-    {
-      const int q = t % SW_Q, r0 = t / SW_Q, ly = r0 % SW_BY, lz0 = r0 / SW_BY;
-      const int gx = x0 + 4 * q, gy = y0 + 1 + ly;
-      const bool okxy = t < SW_BT && (uint32_t)gx < (uint32_t)a.d0 && (uint32_t)gy < (uint32_t)a.d1;
-      const uint32_t gxy = (uint32_t)gx + (uint32_t)sy * (uint32_t)gy;
-#pragma unroll
-      for (int it = 0; it < 3; it++) {
-        const int gz = z0 + 1 + lz0 + 2 * it;
-        const bool ok = okxy && (uint32_t)gz < (uint32_t)a.d2;
-        bv[it] = rb_ld4(rbb, ok ? 4u * (gxy + (uint32_t)sz * (uint32_t)gz) : RB_OOB);
       }
     }
     if (t < SW_PT) {
@@ -402,14 +401,6 @@ __global__ __launch_bounds__(SW_NT) void k_mg_sweep(LvArgs a, int ntx, int nty, 
         *reinterpret_cast<uint32_t *>(sc + 4 * (t + SW_PT * it)) = c;
       }
     }
-    if (t < SW_BT) {
-#pragma unroll
-      for (int it = 0; it < 3; it++) {
-        float4 v = bv[it];
-        v.x = a.hh * v.x; v.y = a.hh * v.y; v.z = a.hh * v.z; v.w = a.hh * v.w;
-        *reinterpret_cast<float4 *>(sb + 4 * (t + SW_BT * it)) = v;
-      }
-    }
   } else {
     for (int i = t; i < SW_PN; i += SW_NT) {
       const int lx = i % SW_PX, row = i / SW_PX, ly = row % SW_PY, lz = row / SW_PY;
@@ -423,13 +414,6 @@ __global__ __launch_bounds__(SW_NT) void k_mg_sweep(LvArgs a, int ntx, int nty, 
       }
       sp[i] = pv;
       sc[i] = cv;
-    }
-    for (int i = t; i < SW_BN; i += SW_NT) {
-      const int lx = i % SW_PX, row = i / SW_PX, ly = row % SW_BY, lz = row / SW_BY;
-      const int gx = x0 + lx, gy = y0 + 1 + ly, gz = z0 + 1 + lz;
-      float bv = 0.0f;
-      if (gx >= 0 && gx < a.d0 && gy >= 0 && gy < a.d1 && gz >= 0 && gz < a.d2) bv = a.hh * b[gx + sy * gy + sz * gz];
-      sb[i] = bv;
     }
   }
   __syncthreads();
@@ -449,7 +433,8 @@ __global__ __launch_bounds__(SW_NT) void k_mg_sweep(LvArgs a, int ntx, int nty, 
       const int r = r0 + 15 * it, lz = r / SW_BY, ly = r - SW_BY * lz;
       const int lx = 3 + 2 * px + ((K + ly + lz) & 1);
       const int pi = ((lz + 1) * SW_PY + (ly + 1)) * SW_PX + lx;
-      sweep_cell<ICNT>(sp, sc, sb, ct, pi, r * SW_PX + lx, omega, a.wdir);
+      sweep_cell<ICNT>(sp, sc, sweep_hb<VEC>(a, rbb, b, x0 + lx, y0 + 1 + ly, z0 + 1 + lz), ct, pi, omega,
+                       a.wdir);
     }
   }
   __syncthreads();
@@ -463,8 +448,8 @@ __global__ __launch_bounds__(SW_NT) void k_mg_sweep(LvArgs a, int ntx, int nty, 
       const int r = r0 + 16 * it, ly = r % SW_TY, lz = r / SW_TY;
       const int lx = 4 + 2 * px + ((K + ly + lz) & 1);
       const int pi = ((lz + 2) * SW_PY + (ly + 2)) * SW_PX + lx;
-      const int bi = ((lz + 1) * SW_BY + (ly + 1)) * SW_PX + lx;
-      sweep_cell<ICNT>(sp, sc, sb, ct, pi, bi, omega, a.wdir);
+      sweep_cell<ICNT>(sp, sc, sweep_hb<VEC>(a, rbb, b, x0 + lx, y0 + 2 + ly, z0 + 2 + lz), ct, pi, omega,
+                       a.wdir);
     }
   }
   __syncthreads();
