@@ -99,6 +99,14 @@ def slice_image(a: np.ndarray, b: np.ndarray) -> np.ndarray:
 
 
 def main(argv=None) -> int:
+    run_scene(argv)
+    return 0
+
+
+def run_scene(argv=None) -> dict:
+    """Run the frame loop for the command line `argv`; returns the summary
+    printed at the end (frames, seconds, and when averaged: hrr_kw, h50, h90,
+    h99, heskestad_m, puffing_hz, burning_rate, pyrolysis_kg_s, radiative)."""
     import pyro_amd as pa
 
     ap = argparse.ArgumentParser(prog="pyro_amd.run", description="simulation-only frame loop of main.rs")
@@ -202,14 +210,15 @@ def main(argv=None) -> int:
               f"pyrolysis {pyro_avg / n_avg:.4f} kg/s  burning area {area_avg / n_avg:.3f} m2  -> "
               f"{pyro_avg / max(area_avg, 1e-9):.4f} kg/m2s")
         print(f"cumulative heat release 50% height {h50:.2f} m  90% {h90:.2f} m  99% {h99:.2f} m")
-        summary.update(hrr_kw=q / 1e3, h50=h50, h90=h90, h99=h99)
+        summary.update(hrr_kw=q / 1e3, h50=h50, h90=h90, h99=h99, pyrolysis_kg_s=pyro_avg / n_avg,
+                       burning_rate=pyro_avg / max(area_avg, 1e-9), radiative=rad_avg / max(hrr_avg, 1.0))
         for i in range(int(scene.get("objects"))):
             if scene.get(f"object.{i}.pool") > 0.5:
                 L, fq = heskestad(q, scene.get(f"object.{i}.radius"))
                 print(f"Heskestad mean flame height (at this HRR) {L:.2f} m, puffing frequency {fq:.2f} Hz")
                 summary.update(heskestad_m=L, puffing_hz=fq)
     print(summary, flush=True)
-    return 0
+    return summary
 
 
 if __name__ == "__main__":
