@@ -321,7 +321,7 @@ def bonfire_c5(args, net):
     s.set("domain.cell", 0.1)
     nx, ny, nz = s.grid().dims
     dt = args.c5_dt
-    sim = pa.Sim(s.params(), s.voxelize(), None, 4, s.get("ignition.until"))
+    sim = pa.Sim.new(s, device_voxelize=True)
     sim.add_velocity_noise(1e-3, 0)
     t0 = time.perf_counter()
     for k in range(args.c5_preroll):
@@ -371,6 +371,44 @@ def bonfire_c5(args, net):
     return out
 
 
+def voxelize_leg():
+    """Scene::voxelize (scene.rs:621-660) of the bonfire (57 shapes) on the
+    device (pyro_voxelize into device memory, shape upload and synchronisation
+    included) at 10 cm and 5 cm, and the host loop at 10 cm."""
+    import ctypes as C
+
+    import torch
+
+    import pyro_amd as pa
+    from pyro_amd import _native
+
+    out = {}
+    for cell in (0.1, 0.05):
+        s = pa.Scene.load(str(ROOT / "pyro_amd" / "scenes" / "bonfire.toml"))
+        s.set("domain.cell", cell)
+        sh = s.shapes()
+        dims, dx, t0, seed = s.voxel_spec()
+        n = dims[0] * dims[1] * dims[2]
+        dst = torch.empty(4 * n, dtype=torch.float32, device="cuda")
+        d = (C.c_uint32 * 3)(*dims)
+        ts = []
+        for _ in range(4):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            _native.check_hip(_native.hip().pyro_voxelize(sh.ctypes.data, sh.size, d, dx, t0, seed,
+                                                         dst.data_ptr(), 1))
+            ts.append(time.perf_counter() - t)
+        key = f"bonfire_{cell * 100:g}cm"
+        out[key] = {"grid": list(dims), "cells": n, "shapes": int(sh.size), "device_ms": 1e3 * min(ts[1:])}
+        if cell == 0.1:
+            t = time.perf_counter()
+            s.voxelize()
+            out[key]["host_ms"] = 1e3 * (time.perf_counter() - t)
+        del dst
+        log(f"[bench] voxelize {key}: {out[key]}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -401,6 +439,7 @@ def main():
     r = gpu_run(args)
     c3 = None if args.no_pressure else pressure_c3(args)
     c5 = None if (args.no_pressure or args.c5_steps <= 0) else bonfire_c5(args, r.get("net"))
+    vox = voxelize_leg()
     cpu = None if args.no_cpu else cpu_baseline(args, r["dims"])
     nx, ny, nz = r["dims"]
     value = r["cells"] * args.steps / r["wall"]
@@ -448,6 +487,7 @@ def main():
         "pressure": {"pool_frame": r["frame"], "standalone": c3, "learned_training": r["learned"],
                      "bonfire_10cm": c5},
         "pool_burning": r["burning"],
+        "scene_voxelize": vox,
         "copy_kernel_GBs": r["copy_gbs"],
         "last_stats": {"hrr_W": r["stats"].hrr, "max_T": r["stats"].max_t, "resid": r["stats"].resid},
     }

@@ -79,6 +79,18 @@ const char *pyro_last_error(void);
 int pyro_sim_create(const void *params, size_t params_bytes, const float *sol_init,
                     const float *s0_init, uint32_t pressure_cycles, float ign_until,
                     uint32_t flags, PyroSim **out);
+/* Scene::voxelize (scene.rs:621-660) on the device: SOL (N x vec4) from the
+ * expanded primitives of pyro_scene_shapes (pyro_scene.h), bit-identical to
+ * pyro_scene_voxelize; written to dst in device memory when dst_on_device,
+ * else copied to host memory.  No counterpart in the reference (its
+ * voxelization is a host loop); for grids too large for a host SOL array. */
+struct PyroShape;
+int pyro_voxelize(const struct PyroShape *shapes, size_t count, const uint32_t dims[3], float dx, float t0,
+                  uint32_t seed, float *dst, int dst_on_device);
+/* A simulation's SOL from primitives, on the device: both SOL buffers and the
+ * solid flags (as sol_init does in pyro_sim_create, which also accepts
+ * sol_init = NULL: no solids). */
+int pyro_sim_voxelize(PyroSim *sim, const struct PyroShape *shapes, size_t count, float t0, uint32_t seed);
 int pyro_sim_destroy(PyroSim *sim);
 
 /* Sim::step (sim.rs:246-423): one timestep; stats are produced when

@@ -45,6 +45,27 @@ int pyro_scene_params(const PyroScene *s, void *out, size_t bytes);
 /* scene.rs:621-660 Scene::voxelize -> N x vec4 SOL = (slot, mass, char, Ts). */
 int pyro_scene_voxelize(const PyroScene *s, float *out, size_t n_floats);
 
+/* The primitives Scene::voxelize tests each cell against (scene.rs:670-879,
+ * every object expanded), in order -- the last one containing a cell wins --
+ * with the material's SOL slot and mass rule, and the yaw's sine and cosine
+ * computed here, so that a device voxelizer (pyro_voxelize in pyro_hip.h)
+ * reproduces pyro_scene_voxelize bit for bit with plain f32 arithmetic. */
+typedef struct PyroShape {
+  uint32_t kind;   /* 0 box (yawed about y), 1 cylinder (segment a-b), 2 sphere, 3 pool */
+  uint32_t slot;   /* SOL slot of the material */
+  uint32_t inert;  /* 1: mass 1 (non-combustible material); else density x jitter */
+  float density;
+  float c[3], h[3];        /* box / sphere / pool centre, box half sizes */
+  float sin_yaw, cos_yaw;  /* box yaw */
+  float a[3], b[3], r;     /* cylinder ends and radius; sphere / pool radius in r */
+  float depth;             /* pool depth */
+} PyroShape;
+/* count = number of primitives; they are written to out when cap >= count. */
+int pyro_scene_shapes(const PyroScene *s, PyroShape *out, size_t cap, size_t *count);
+/* What voxelize needs besides the shapes: grid dims, cell size, ambient
+ * temperature, seed of the mass jitter. */
+int pyro_scene_voxel_spec(const PyroScene *s, uint32_t dims[3], float *dx, float *t0, uint32_t *seed);
+
 /* scene.rs:408-421 material(name): index into the 11-entry table, or error
  * listing the choices. */
 int pyro_material_index(const char *name);

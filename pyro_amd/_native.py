@@ -58,6 +58,8 @@ def hip() -> C.CDLL:
         lib.pyro_sim_info.argtypes = [P, P, P, P, P, P]
         lib.pyro_sim_level_dims.argtypes = [P, i32, P]
         lib.pyro_sim_sync.argtypes = [P]
+        lib.pyro_voxelize.argtypes = [P, sz, P, f32, f32, u32, P, i32]
+        lib.pyro_sim_voxelize.argtypes = [P, P, sz, f32, u32]
         lib.pyro_sim_device_ptr.argtypes = [P, i32, C.POINTER(P), C.POINTER(sz)]
         lib.pyro_sim_begin_step.argtypes = [P, f32, i32]
         lib.pyro_sim_run_stage.argtypes = [P, i32]
@@ -92,6 +94,8 @@ def host() -> C.CDLL:
         lib.pyro_material_index.argtypes = [C.c_char_p]
         lib.pyro_scene_hash.argtypes = [u32]
         lib.pyro_scene_hash.restype = f32
+        lib.pyro_scene_shapes.argtypes = [P, P, sz, C.POINTER(sz)]
+        lib.pyro_scene_voxel_spec.argtypes = [P, P, C.POINTER(f32), C.POINTER(f32), C.POINTER(u32)]
         _host = lib
     return _host
 

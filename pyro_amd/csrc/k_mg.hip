@@ -298,7 +298,7 @@ void launch_mg_setup(const MgLevel *L, int nl, hipStream_t s) {
 constexpr int SW_TX = 32, SW_TY = 8, SW_TZ = 4, SW_NT = 256;
 constexpr int SW_PX = SW_TX + 8, SW_PY = SW_TY + 4, SW_PZ = SW_TZ + 4;  // footprint
 constexpr int SW_BY = SW_TY + 2, SW_BZ = SW_TZ + 2;                      // ring-1 rows for b
-constexpr int SW_PN = SW_PX * SW_PY * SW_PZ, SW_BN = SW_PX * SW_BY * SW_BZ;
+constexpr int SW_PN = SW_PX * SW_PY * SW_PZ;
 constexpr int SW_Q = SW_PX / 4;  // float4 per footprint row
 
 // One cell of a colour pass at LDS index pi (h^2 b at bi), without branches:

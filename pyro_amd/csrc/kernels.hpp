@@ -4,6 +4,8 @@
 #pragma once
 #include "common.hpp"
 
+struct PyroShape;  // pyro_scene.h
+
 namespace pyro {
 
 struct Fields {
@@ -33,6 +35,10 @@ void launch_solid(const Fields &f, StatParts *st, hipStream_t s);
 void launch_curl(const Fields &f, StatParts *st, hipStream_t s);
 // flags[i] = 1 where sol[i] is solid (at creation and after SOL is written).
 void launch_init_flags(const Params *P, const Geo &g, const float4 *sol, uint8_t *flags, hipStream_t s);
+// Scene::voxelize on the device (voxel.hip): SOL of dims from the expanded
+// primitives (device copy), bit-identical to the host voxelizer.
+int launch_voxelize(const ::PyroShape *d_shapes, int count, const uint32_t dims[3], float dx, float t0,
+                    uint32_t seed, float4 *out, hipStream_t s);
 void launch_adv_vel1(const Fields &f, hipStream_t s);
 void launch_adv_vel2(const Fields &f, hipStream_t s);
 void launch_adv_s1(const Fields &f, hipStream_t s);

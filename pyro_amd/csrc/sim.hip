@@ -16,6 +16,7 @@
 
 #include "kernels.hpp"
 #include "pyro_hip.h"
+#include "pyro_scene.h"
 
 using namespace pyro;
 
@@ -216,7 +217,7 @@ const char *pyro_last_error(void) { return g_err.c_str(); }
 
 int pyro_sim_create(const void *params, size_t params_bytes, const float *sol_init, const float *s0_init,
                     uint32_t pressure_cycles, float ign_until, uint32_t cflags, PyroSim **out) {
-  if (!params || !sol_init || !out) return fail(PYRO_E_INVALID, "pyro_sim_create: null argument");
+  if (!params || !out) return fail(PYRO_E_INVALID, "pyro_sim_create: null argument");
   if (params_bytes != PYRO_PARAMS_BYTES) return fail(PYRO_E_INVALID, "pyro_sim_create: Params blob must be 576 bytes");
   *out = nullptr;
   int ndev = 0;
@@ -307,14 +308,62 @@ int pyro_sim_create(const void *params, size_t params_bytes, const float *sol_in
     s0src = amb.data();
   }
   if (hipMemcpy(s->dP, &s->hp, sizeof(Params), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(s->s0, s0src, (size_t)g.n * 16, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(s->sol[0], sol_init, (size_t)g.n * 16, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(s->sol[1], sol_init, (size_t)g.n * 16, hipMemcpyHostToDevice) != hipSuccess)
+      hipMemcpy(s->s0, s0src, (size_t)g.n * 16, hipMemcpyHostToDevice) != hipSuccess)
     return bail(fail(PYRO_E_HIP, "initial upload failed"));
+  if (sol_init) {
+    if (hipMemcpy(s->sol[0], sol_init, (size_t)g.n * 16, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->sol[1], sol_init, (size_t)g.n * 16, hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(PYRO_E_HIP, "initial upload failed"));
+  } else {
+    // no solids: SOL = (0, 0, 0, T0) everywhere (the voxelizer with no shapes)
+    const uint32_t dims[3] = {hp.nx, hp.ny, hp.nz};
+    if (launch_voxelize(nullptr, 0, dims, hp.dx, hp.t0, 0u, static_cast<float4 *>(s->sol[0]), s->stream) ||
+        hipMemcpyAsync(s->sol[1], s->sol[0], (size_t)g.n * 16, hipMemcpyDeviceToDevice, s->stream) != hipSuccess)
+      return bail(fail(PYRO_E_HIP, "initial SOL fill failed"));
+  }
   launch_init_flags(s->dP, s->g, s->sol[0], s->flags, s->stream);
   if (sync_check(s)) return bail(PYRO_E_HIP);
   *out = s;
   return 0;
+}
+
+int pyro_voxelize(const struct PyroShape *shapes, size_t count, const uint32_t dims[3], float dx, float t0,
+                  uint32_t seed, float *dst, int dst_on_device) {
+  if (!dims || !dst || (count && !shapes)) return fail(PYRO_E_INVALID, "pyro_voxelize: null argument");
+  const uint64_t n64 = (uint64_t)dims[0] * dims[1] * dims[2];
+  if (!n64 || n64 * 16ull >= (1ull << 32) || count > (1u << 20)) return fail(PYRO_E_INVALID, "pyro_voxelize: bad size");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(PYRO_E_NODEVICE, "no HIP device available");
+  PyroShape *dsh = nullptr;
+  float4 *dout = nullptr;
+  auto done = [&](int rc) {
+    if (dsh) (void)hipFree(dsh);
+    if (dout && !dst_on_device) (void)hipFree(dout);
+    return rc;
+  };
+  if (count && (hipMalloc(&dsh, count * sizeof(PyroShape)) != hipSuccess ||
+                hipMemcpy(dsh, shapes, count * sizeof(PyroShape), hipMemcpyHostToDevice) != hipSuccess))
+    return done(fail(PYRO_E_HIP, "pyro_voxelize: shape upload failed"));
+  if (dst_on_device) dout = reinterpret_cast<float4 *>(dst);
+  else if (hipMalloc(&dout, (size_t)n64 * 16) != hipSuccess) return done(fail(PYRO_E_OOM, "pyro_voxelize: hipMalloc failed"));
+  if (launch_voxelize(dsh, (int)count, dims, dx, t0, seed, dout, nullptr) || hipDeviceSynchronize() != hipSuccess)
+    return done(fail(PYRO_E_HIP, "pyro_voxelize: kernel failed"));
+  if (!dst_on_device && hipMemcpy(dst, dout, (size_t)n64 * 16, hipMemcpyDeviceToHost) != hipSuccess)
+    return done(fail(PYRO_E_HIP, "pyro_voxelize: download failed"));
+  return done(0);
+}
+
+int pyro_sim_voxelize(PyroSim *s, const struct PyroShape *shapes, size_t count, float t0, uint32_t seed) {
+  if (!s || (count && !shapes)) return fail(PYRO_E_INVALID, "pyro_sim_voxelize: null argument");
+  int rc = sync_check(s);
+  if (rc) return rc;
+  const uint32_t dims[3] = {s->hp.nx, s->hp.ny, s->hp.nz};
+  rc = pyro_voxelize(shapes, count, dims, s->hp.dx, t0, seed, reinterpret_cast<float *>(s->sol[s->sol_cur]), 1);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(s->sol[1 - s->sol_cur], s->sol[s->sol_cur], (size_t)s->g.n * 16, hipMemcpyDeviceToDevice,
+                        s->stream));
+  launch_init_flags(s->dP, s->g, static_cast<const float4 *>(s->sol[s->sol_cur]), s->flags, s->stream);
+  return sync_check(s);
 }
 
 int pyro_sim_destroy(PyroSim *s) {

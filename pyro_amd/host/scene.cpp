@@ -861,4 +861,46 @@ int pyro_material_index(const char *name) {
 
 float pyro_scene_hash(uint32_t v) { return hash(v); }
 
+int pyro_scene_shapes(const PyroScene *sc, PyroShape *out, size_t cap, size_t *count) {
+  if (!sc || !count) { g_err = "null argument"; return PYRO_SCENE_E_ARG; }
+  return guard([&] {
+    const Scene &s = sc->s;
+    uint32_t dims[3];
+    float dx;
+    grid(s, dims, &dx);
+    std::vector<int> used = used_materials(s);
+    std::vector<PyroShape> v;
+    for (const Object &o : s.objects) {
+      const int mi = material(o.material);
+      int slot = 0;
+      for (size_t u = 0; u < used.size(); u++)
+        if (used[u] == mi) { slot = (int)u; break; }
+      const Material &m = MATERIALS[used[slot]];
+      for (const Shape &sh : expand(o, dx)) {
+        PyroShape p{};
+        p.kind = (uint32_t)sh.kind;
+        p.slot = (uint32_t)slot;
+        p.inert = m.kind == 2 ? 1u : 0u;
+        p.density = m.density;
+        for (int q = 0; q < 3; q++) { p.c[q] = sh.c[q]; p.h[q] = sh.h[q]; p.a[q] = sh.a[q]; p.b[q] = sh.b[q]; }
+        sincos_(sh.yaw, &p.sin_yaw, &p.cos_yaw);
+        p.r = sh.r;
+        p.depth = sh.depth;
+        v.push_back(p);
+      }
+    }
+    *count = v.size();
+    if (out && cap >= v.size() && !v.empty()) std::memcpy(out, v.data(), v.size() * sizeof(PyroShape));
+  });
+}
+
+int pyro_scene_voxel_spec(const PyroScene *sc, uint32_t dims[3], float *dx, float *t0, uint32_t *seed) {
+  if (!sc || !dims || !dx || !t0 || !seed) { g_err = "null argument"; return PYRO_SCENE_E_ARG; }
+  return guard([&] {
+    grid(sc->s, dims, dx);
+    *t0 = sc->s.ambient_temp;
+    *seed = sc->s.seed;
+  });
+}
+
 }  // extern "C"

@@ -135,8 +135,9 @@ def run_scene(argv=None) -> dict:
     grid = scene.grid()
     nx, ny, nz = grid.dims
     print(f"grid {nx}x{ny}x{nz} ({nx * ny * nz / 1e6:.1f} M cells, {grid.dx * 100:.1f} cm)", flush=True)
-    sim = pa.Sim(scene.params(), scene.voxelize(), None, int(scene.get("sim.pressure_cycles")),
-                 scene.get("ignition.until"))
+    # voxelized on the device: the same bits as Scene::voxelize, without the
+    # O(cells x shapes) host loop
+    sim = pa.Sim.new(scene, device_voxelize=True)
     if args.load_state:
         sim.load(args.load_state)
     P = None

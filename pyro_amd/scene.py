@@ -62,6 +62,14 @@ class Grid:
     dx: float
 
 
+# PyroShape (pyro_scene.h): kind, slot, inert, density, c[3], h[3], sin_yaw,
+# cos_yaw, a[3], b[3], r, depth
+SHAPE_DTYPE = np.dtype([("kind", np.uint32), ("slot", np.uint32), ("inert", np.uint32), ("density", np.float32),
+                        ("c", np.float32, 3), ("h", np.float32, 3), ("sin_yaw", np.float32),
+                        ("cos_yaw", np.float32), ("a", np.float32, 3), ("b", np.float32, 3), ("r", np.float32),
+                        ("depth", np.float32)])
+
+
 class Scene:
     """A parsed scene (scene.rs:8-25)."""
 
@@ -117,6 +125,33 @@ class Scene:
         _native.check_host(
             _native.host().pyro_scene_voxelize(self._h, out.ctypes.data, out.size)
         )
+        return out
+
+    def shapes(self) -> np.ndarray:
+        """The expanded voxelization primitives (PyroShape records, pyro_scene.h)."""
+        cnt = C.c_size_t()
+        _native.check_host(_native.host().pyro_scene_shapes(self._h, None, 0, C.byref(cnt)))
+        out = np.zeros(cnt.value, dtype=SHAPE_DTYPE)
+        if cnt.value:
+            _native.check_host(_native.host().pyro_scene_shapes(self._h, out.ctypes.data, cnt.value, C.byref(cnt)))
+        return out
+
+    def voxel_spec(self):
+        """(dims, dx, t0, seed) of Scene::voxelize."""
+        d = (C.c_uint32 * 3)()
+        dx, t0, seed = C.c_float(), C.c_float(), C.c_uint32()
+        _native.check_host(_native.host().pyro_scene_voxel_spec(self._h, d, C.byref(dx), C.byref(t0), C.byref(seed)))
+        return (d[0], d[1], d[2]), dx.value, t0.value, seed.value
+
+    def voxelize_device(self) -> np.ndarray:
+        """Scene::voxelize computed on the GPU (pyro_voxelize), copied to the host."""
+        sh = self.shapes()
+        dims, dx, t0, seed = self.voxel_spec()
+        nx, ny, nz = dims
+        out = np.zeros((nz, ny, nx, 4), dtype=np.float32)
+        d = (C.c_uint32 * 3)(*dims)
+        _native.check_hip(_native.hip().pyro_voxelize(sh.ctypes.data if sh.size else None, sh.size, d, dx, t0, seed,
+                                                     out.ctypes.data, 0))
         return out
 
 

@@ -84,9 +84,9 @@ class Sim:
         self.p = params
         self.dims = (params["nx"], params["ny"], params["nz"])
         self.dx = params["dx"]
-        sol = np.ascontiguousarray(sol_init, dtype=np.float32)
         n = self.dims[0] * self.dims[1] * self.dims[2]
-        if sol.size != 4 * n:
+        sol = None if sol_init is None else np.ascontiguousarray(sol_init, dtype=np.float32)
+        if sol is not None and sol.size != 4 * n:
             raise ValueError("sol_init must hold N x 4 floats")
         s0p = None
         if s0_init is not None:
@@ -96,7 +96,8 @@ class Sim:
             s0p = s0.ctypes.data
         h = C.c_void_p()
         blob = (C.c_uint8 * 576).from_buffer_copy(params.bytes())
-        _native.check_hip(lib.pyro_sim_create(blob, 576, sol.ctypes.data, s0p, int(pressure_cycles),
+        _native.check_hip(lib.pyro_sim_create(blob, 576, None if sol is None else sol.ctypes.data, s0p,
+                                              int(pressure_cycles),
                                               float(ign_until), 0 if use_graph else CREATE_NO_GRAPH,
                                               C.byref(h)))
         self._h = h
@@ -104,12 +105,27 @@ class Sim:
         self.nv = (self.dims[0] + 1) * (self.dims[1] + 1) * (self.dims[2] + 1)
 
     @classmethod
-    def new(cls, scene: Scene, use_graph: bool = True) -> "Sim":
-        """Sim::new(gpu, scene) (sim.rs:108-232)."""
+    def new(cls, scene: Scene, use_graph: bool = True, device_voxelize: bool = False) -> "Sim":
+        """Sim::new(gpu, scene) (sim.rs:108-232).  device_voxelize: SOL is
+        voxelized on the GPU (pyro_sim_voxelize; the same bits) instead of on
+        the host and uploaded."""
         p = scene.params()
-        sol = scene.voxelize()
-        return cls(p, sol, None, int(scene.get("sim.pressure_cycles")), scene.get("ignition.until"),
-                   use_graph=use_graph)
+        if not device_voxelize:
+            return cls(p, scene.voxelize(), None, int(scene.get("sim.pressure_cycles")),
+                       scene.get("ignition.until"), use_graph=use_graph)
+        sim = cls(p, None, None, int(scene.get("sim.pressure_cycles")), scene.get("ignition.until"),
+                  use_graph=use_graph)
+        sim.voxelize(scene)
+        return sim
+
+    def voxelize(self, scene: Scene) -> None:
+        """SOL of `scene` (same grid) voxelized on the device (pyro_sim_voxelize)."""
+        sh = scene.shapes()
+        dims, _, t0, seed = scene.voxel_spec()
+        if tuple(dims) != tuple(self.dims):
+            raise ValueError("scene grid differs from the simulation's")
+        _native.check_hip(_native.hip().pyro_sim_voxelize(self._h, sh.ctypes.data if sh.size else None, sh.size,
+                                                          t0, seed))
 
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and self._h.value:
