@@ -49,6 +49,8 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 SWEEP_BYTES_PER_CELL = 16.0  # read phi 4 + b 4 + flags 4, write phi 4 (reference layout)
+# renderer pack: read s0, s1, sol 48 + flags 4 + three velocity faces 12, write three half4 24
+RENDER_PACK_BYTES_PER_CELL = 88.0
 TRAFFIC_FILE = ROOT / "profiles" / "mg_sweep_l0_pool_2cm_traffic.json"
 # U-Net FLOPs per level-0 voxel: 2*27*cin*cout per conv at its level's
 # resolution (1/8 per halving) plus the 1x1x1 head: level 0 2*27*(2*16 + 16*16
@@ -142,6 +144,7 @@ def gpu_run(args):
                    "step_resid": sb.resid}
         log(f"[bench] pool from t = {burning['t_start_s']} s: {burning['ms_per_step']:.3f} ms/step, "
             f"HRR {sb.hrr / 1e3:.0f} kW")
+    handoff = render_handoff(sim, args) if args.burn_time > 0 else None
     net, learn = (None, None) if args.no_pressure else learned_net(args)
     frame = None if args.no_pressure else pressure_frame(sim, args, p["dx"], net)
     if learn is not None:
@@ -151,7 +154,78 @@ def gpu_run(args):
             f"{learn['forward_tflops_pool_2cm']:.0f} TFLOP/s")
     return dict(cells=cells, wall=wall, stats=st, stage_ms=stage_ms, smooth_ms=smooth_ms,
                 coarse_ms=coarse_ms, achieved=achieved, algo_bytes=algo_bytes, dims=(nx, ny, nz),
-                copy_gbs=copy_gbs, frame=frame, learned=learn, net=net, burning=burning)
+                copy_gbs=copy_gbs, frame=frame, learned=learn, net=net, burning=burning, handoff=handoff)
+
+
+def render_handoff(sim, args):
+    """The renderer hand-off on the burning pool (render.rs:255-277,
+    render.wgsl:16-37): the pack kernel against its compulsory bytes (s0, s1,
+    sol 48 B + flag 4 B + 3 velocity faces 12 B read, three half4 volumes
+    24 B written per cell, reference layout); one blocking pack + copy into
+    pinned memory; the
+    raw host staging of the seven fields the reference binds
+    (pyro_sim_read_field, pageable); and steps with a hand-off every 21 steps
+    (one 24 fps frame at dt = 2 ms) against the same steps without, the copy
+    of frame k running beside the steps of frame k+1; both from the same
+    checkpointed state, alternated twice."""
+    import tempfile
+
+    import pyro_amd as pa
+
+    nx, ny, nz = sim.dims
+    cells = nx * ny * nz
+    bufs = [pa.PinnedBuffer(sim.dims), pa.PinnedBuffer(sim.dims)]
+    sim.render_pack()
+    sim.time_kernel("render_pack", 0, 5)
+    pack_ms = sim.time_kernel("render_pack", 0, 50)
+    pack_gbs = RENDER_PACK_BYTES_PER_CELL * cells / (pack_ms * 1e-3) / 1e9
+    ts = []
+    for k in range(5):
+        sim.sync()
+        t = time.perf_counter()
+        sim.render_pack(bufs[0])
+        sim.render_wait()
+        ts.append(time.perf_counter() - t)
+    handoff_ms = 1e3 * min(ts)
+    raw = ["s0", "s1", "sol", "flags", "u", "v", "w"]
+    t = time.perf_counter()
+    raw_bytes = sum(sim.read_field(f).nbytes for f in raw)
+    raw_ms = 1e3 * (time.perf_counter() - t)
+    frames, per = 6, 21
+
+    def run(with_handoff):
+        sim.sync()
+        t0 = time.perf_counter()
+        for f in range(frames):
+            for k in range(per):
+                sim.step(args.dt, False)
+            if with_handoff:
+                sim.render_wait()  # frame f-1 has landed: the renderer may take it
+                sim.render_pack(bufs[f % 2])
+        sim.sync()
+        sim.render_wait()
+        return 1e3 * (time.perf_counter() - t0) / (frames * per)
+
+    ck = Path(tempfile.mkdtemp()) / "handoff.ckpt"
+    sim.save(ck)
+    times = {False: [], True: []}
+    for rep in range(2):
+        for mode in (False, True):
+            sim.load(ck)
+            times[mode].append(run(mode))
+    sim.load(ck)
+    ck.unlink()
+    plain, overlapped = float(np.mean(times[False])), float(np.mean(times[True]))
+    for bf in bufs:
+        bf.close()
+    out = {"volumes_MB": 24 * cells / 1e6, "pack_kernel_us": pack_ms * 1e3, "pack_kernel_GBs": pack_gbs,
+           "pack_kernel_hbm_frac": pack_gbs / HBM_PEAK_GBS, "pack_and_pinned_copy_ms": handoff_ms,
+           "pcie_GBs": 24 * cells / (handoff_ms * 1e-3) / 1e9, "raw_staging_MB": raw_bytes / 1e6,
+           "raw_staging_ms": raw_ms, "steps_per_frame": per, "ms_per_step_plain": plain,
+           "ms_per_step_with_handoff": overlapped,
+           "handoff_cost_per_frame_ms": (overlapped - plain) * per}
+    log(f"[bench] render hand-off: {out}")
+    return out
 
 
 def _info(i):
@@ -487,6 +561,7 @@ def main():
         "pressure": {"pool_frame": r["frame"], "standalone": c3, "learned_training": r["learned"],
                      "bonfire_10cm": c5},
         "pool_burning": r["burning"],
+        "render_handoff": r["handoff"],
         "scene_voxelize": vox,
         "copy_kernel_GBs": r["copy_gbs"],
         "last_stats": {"hrr_W": r["stats"].hrr, "max_T": r["stats"].max_t, "resid": r["stats"].resid},

@@ -53,6 +53,7 @@ enum PyroField {
   PYRO_FIELD_U = 11, PYRO_FIELD_V = 12, PYRO_FIELD_W = 13,    /* vel[cur] */
   PYRO_FIELD_UB = 14, PYRO_FIELD_VB = 15, PYRO_FIELD_WB = 16, /* vel[1-cur] */
   PYRO_FIELD_HU = 17, PYRO_FIELD_HV = 18, PYRO_FIELD_HW = 19, /* predictor */
+  PYRO_FIELD_RENDER = 20,  /* last pyro_sim_render_pack: vol | sol | vel, 3 x N x half4 */
   PYRO_FIELD_LEVEL_BASE = 32 /* + 4*level + {0 phi, 1 b, 2 flags(u32)} */
 };
 
@@ -100,8 +101,11 @@ int pyro_sim_step(PyroSim *sim, float dt, int want_stats);
  * in FP64 (slot map of stats.wgsl:2-3; 10 = sum r^2, 11 = sum b^2). */
 int pyro_sim_read_stats(PyroSim *sim, double *out, size_t n);
 /* Sim::read_scalars (sim.rs:494-508) and every other field, blocking.
- * Writing SOL (or SOL_OUT) sets both SOL buffers and recomputes the solid
- * flags, as Sim::new does from one voxelized array (sim.rs:136-140). */
+ * Writing SOL (or SOL_OUT) sets both SOL buffers, as Sim::new does from one
+ * voxelized array (sim.rs:136-140), and recomputes the solid flags.  Flags
+ * follow SOL from creation on; the reference leaves them zero until the
+ * first step's solid pass rewrites every cell (combust.wgsl:131), so they
+ * differ only when read before the first step. */
 int pyro_sim_read_field(PyroSim *sim, int field, void *dst, size_t bytes);
 int pyro_sim_write_field(PyroSim *sim, int field, const void *src, size_t bytes);
 /* Sim::cells (sim.rs:234), Sim::time / dims / cycles / levels. */
@@ -111,6 +115,26 @@ int pyro_sim_level_dims(const PyroSim *sim, int level, uint32_t dims[3]);
 int pyro_sim_sync(PyroSim *sim);
 /* Device pointer of a field for interop (renderer hand-off); *bytes = size. */
 int pyro_sim_device_ptr(PyroSim *sim, int field, void **dptr, size_t *bytes);
+
+/* Renderer hand-off (replaces the pack pass that starts Renderer::render,
+ * render.rs:255-277 binding s0, s1, sol[sol_cur], flags, vel[cur];
+ * render.wgsl:16-37).  Packs the current state into the renderer's three
+ * rgba16float volumes -- vol = (soot ppm, T, smoke g/m3, HRR kW/m3),
+ * sol = (occupancy, Ts, char fraction, slot) premultiplied by occupancy,
+ * vel = (cell-centre u, v, w, 0) -- each N x half4, x fastest, vol then sol
+ * then vel: 24 bytes per cell, the rows queue.write_texture takes.
+ * Stream-ordered after the steps already enqueued.  With host_dst (bytes =
+ * 24 N) the volumes are then copied to host_dst on a second stream, which
+ * overlaps the steps enqueued next; pyro_sim_render_wait blocks until that
+ * copy has landed.  host_dst should be pinned (pyro_host_alloc) for the copy
+ * to run asynchronously; pageable memory works but blocks the caller.  With
+ * host_dst NULL only the device buffer (PYRO_FIELD_RENDER) is written.
+ * host_dst must stay valid and untouched until pyro_sim_render_wait. */
+int pyro_sim_render_pack(PyroSim *sim, void *host_dst, size_t bytes);
+int pyro_sim_render_wait(PyroSim *sim);
+/* Pinned host memory for the hand-off (hipHostMalloc / hipHostFree). */
+int pyro_host_alloc(size_t bytes, void **out);
+int pyro_host_free(void *p);
 
 /* Checkpoint (no reference counterpart): everything a step reads -- vel,
  * s0, s1, sol, the level-0 pressure warm start, Params, time, frame -- in one
@@ -133,6 +157,7 @@ int pyro_sim_time_stage(PyroSim *sim, int stage, int reps, float *avg_ms);
 #define PYRO_KERNEL_MG_RESTRICT 1 /* residual + restriction level -> level+1 */
 #define PYRO_KERNEL_MG_PROLONG 2  /* prolongation level+1 -> level */
 #define PYRO_KERNEL_MG_COARSE 3   /* all 80 SOR iterations of the coarsest level */
+#define PYRO_KERNEL_RENDER_PACK 4 /* the renderer pack (after one pyro_sim_render_pack) */
 /* Time `reps` launches of one multigrid kernel (HIP events on the sim
  * stream); the pressure state is modified. */
 int pyro_sim_time_kernel(PyroSim *sim, int kernel, int level, int reps, float *avg_ms);

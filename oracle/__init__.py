@@ -53,6 +53,7 @@ def lib() -> C.CDLL:
         L.osim_time.restype = C.c_double
         L.osim_add_velocity_noise.argtypes = [V, C.c_float, C.c_uint32]
         L.osim_set_threads.argtypes = [C.c_int]
+        L.osim_render_pack.argtypes = [V, V]
         L.oracle_expf.argtypes = [C.c_float]
         L.oracle_expf.restype = C.c_float
         L.oracle_noise.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
@@ -175,6 +176,14 @@ class OSim:
     def read_stats_raw(self):
         out = np.zeros(1040, dtype=np.float64)
         lib().osim_read_stats(self._h, out.ctypes.data)
+        return out
+
+    def render_pack(self):
+        """render.wgsl:16-37: (vol, sol, vel), each (nz, ny, nx, 4) f32, the
+        values before the rgba16float store."""
+        nx, ny, nz = self.dims
+        out = np.zeros((3, nz, ny, nx, 4), np.float32)
+        lib().osim_render_pack(self._h, out.ctypes.data)
         return out
 
     def add_velocity_noise(self, amp, seed):

@@ -1202,6 +1202,45 @@ int osim_write_field(OSim *s, int field, const void *src, size_t bytes) {
   return 0;
 }
 
+/* render.wgsl:16-37 (pack): the renderer's three volumes, as the f32
+ * values handed to textureStore before the rgba16float conversion.
+ * out = vol | sol | vel, each N x vec4, x fastest:
+ *   vol = (soot ppm, T, smoke g/m3, heat release kW/m3), (0, T, 0, 0) in solids
+ *   sol = (occupancy, Ts*occ, char fraction*occ, slot*occ)
+ *   vel = (cell-centre u, v, w, 0) */
+void osim_render_pack(const OSim *s, float *out) {
+  const size_t n = s->n;
+#pragma omp parallel for schedule(static)
+  for (int z = 0; z < NZ; z++)
+    for (int y = 0; y < NY; y++)
+      for (int x = 0; x < NX; x++) {
+        const size_t i = sidx(s, x, y, z);
+        const v4 a = s->s0[i], b = s->s1[i], so = s->sol[s->sol_cur][i];
+        const float rho = rhoOf(s, a.x);
+        const int solid = s->flags[i] == 1u;
+        float *vol = out + 4 * i, *sl = out + 4 * (n + i), *ve = out + 4 * (2 * n + i);
+        if (solid) {
+          vol[0] = 0.0f; vol[1] = a.x; vol[2] = 0.0f; vol[3] = 0.0f;
+        } else {
+          vol[0] = rho * a.w / RHO_SOOT * 1e6f;
+          vol[1] = a.x;
+          vol[2] = rho * b.x * 1e3f;
+          vol[3] = b.y * 1e-3f;
+        }
+        const float tot = so.y + so.z;
+        const float oc = solid ? 1.0f : 0.0f;
+        sl[0] = oc;
+        sl[1] = so.w * oc;
+        sl[2] = so.z / fmax_(tot, 1e-6f) * oc;
+        sl[3] = so.x * oc;
+        const float *U = s->vel[s->cur][0], *V = s->vel[s->cur][1], *W = s->vel[s->cur][2];
+        ve[0] = 0.5f * (U[vidx(s, x, y, z)] + U[vidx(s, x + 1, y, z)]);
+        ve[1] = 0.5f * (V[vidx(s, x, y, z)] + V[vidx(s, x, y + 1, z)]);
+        ve[2] = 0.5f * (W[vidx(s, x, y, z)] + W[vidx(s, x, y, z + 1)]);
+        ve[3] = 0.0f;
+      }
+}
+
 void osim_add_velocity_noise(OSim *s, float amp, uint32_t seed) {
   for (int k = 0; k < 3; k++) {
     float *V = s->vel[s->cur][k];

@@ -55,6 +55,8 @@ void osim_read_stats(const OSim *s, double *out /* 1040 */);
 double osim_time(const OSim *s);
 /* C1 harness noise: u,v,w += amp*U[-1,1] on every valid face of vel[cur]. */
 void osim_add_velocity_noise(OSim *s, float amp, uint32_t seed);
+/* render.wgsl:16-37: vol | sol | vel (3 x N x vec4 f32, before the f16 store) */
+void osim_render_pack(const OSim *s, float *out);
 void osim_set_threads(int n);
 
 /* Building blocks exposed for closed-form tests. */

@@ -25,7 +25,7 @@ HOST = PKG / "host"
 INCLUDE = ROOT / "include"
 
 HIP_SOURCES = ["k_fluid.hip", "k_advect.hip", "k_combust.hip", "k_mg.hip", "k_solve.hip", "k_net.hip", "sim.hip", "poisson.hip",
-               "net.hip", "voxel.hip"]
+               "net.hip", "voxel.hip", "k_render.hip"]
 HIP_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",

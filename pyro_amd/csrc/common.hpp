@@ -90,6 +90,10 @@ __device__ __forceinline__ float smoothstep_(float e0, float e1, float x) {
 __device__ __forceinline__ float len3(float x, float y, float z) { return sqrtf(x * x + y * y + z * z); }
 __device__ __forceinline__ float pow4(float x) { float y = x * x; return y * y; }
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+// common.wgsl:45 rhoOf: ideal-gas density at temperature T (floor 200 K)
+__device__ __forceinline__ float rhoOf(const Params *__restrict__ P, float T) {
+  return P->rho0 * P->t0 / fmax_(T, 200.0f);
+}
 
 // exp(x): Cody-Waite reduction x = k*ln2 + r with ln2 split into a 16-bit
 // head (0.693145751953125) and a tail, |r| <= ln2/2, then the degree-7 Taylor

@@ -8,9 +8,6 @@
 
 namespace pyro {
 
-__device__ __forceinline__ float rhoOf(const Params *__restrict__ P, float T) {
-  return P->rho0 * P->t0 / fmax_(T, 200.0f);
-}
 __device__ __forceinline__ float kapP(const Params *__restrict__ P, float T, float yo, float ys) {
   float fv = rhoOf(P, T) * fmax_(ys, 0.0f) / RHO_SOOT;
   return 1862.0f * fv * T + P->kap_gas * clampf_(1.0f - yo / P->yo2, 0.0f, 1.0f);

@@ -37,6 +37,8 @@ void launch_curl(const Fields &f, StatParts *st, hipStream_t s);
 void launch_init_flags(const Params *P, const Geo &g, const float4 *sol, uint8_t *flags, hipStream_t s);
 // Scene::voxelize on the device (voxel.hip): SOL of dims from the expanded
 // primitives (device copy), bit-identical to the host voxelizer.
+// render.wgsl:16-37: vol | sol | vel as three N x half4 volumes into `out`
+void launch_render_pack(const Fields &f, void *out, hipStream_t s);
 int launch_voxelize(const ::PyroShape *d_shapes, int count, const uint32_t dims[3], float dx, float t0,
                     uint32_t seed, float4 *out, hipStream_t s);
 void launch_adv_vel1(const Fields &f, hipStream_t s);

@@ -58,6 +58,12 @@ struct PyroSim {
   bool in_step = false, step_stats = false;
   std::vector<void *> allocs;
   int *d_flag = nullptr;
+  // renderer hand-off: packed volumes, the stream that copies them out and
+  // the events that order the copy against the steps
+  void *render = nullptr;
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t ev_packed = nullptr, ev_copied = nullptr;
+  bool copy_issued = false;
 };
 
 namespace {
@@ -184,6 +190,9 @@ FieldRef field_ref(PyroSim *s, int field) {
     case PYRO_FIELD_U: case PYRO_FIELD_V: case PYRO_FIELD_W: vf(s->vel[s->cur][field - PYRO_FIELD_U]); return r;
     case PYRO_FIELD_UB: case PYRO_FIELD_VB: case PYRO_FIELD_WB: vf(s->vel[1 - s->cur][field - PYRO_FIELD_UB]); return r;
     case PYRO_FIELD_HU: case PYRO_FIELD_HV: case PYRO_FIELD_HW: vf(s->hat[field - PYRO_FIELD_HU]); return r;
+    case PYRO_FIELD_RENDER:
+      if (s->render) { r.ptr = s->render; r.dev_bytes = r.host_bytes = (size_t)s->g.n * 24; }
+      return r;
     default: break;
   }
   if (field >= PYRO_FIELD_LEVEL_BASE) {
@@ -366,9 +375,64 @@ int pyro_sim_voxelize(PyroSim *s, const struct PyroShape *shapes, size_t count, 
   return sync_check(s);
 }
 
+int pyro_sim_render_pack(PyroSim *s, void *host_dst, size_t bytes) {
+  if (!s) return fail(PYRO_E_INVALID, "null sim");
+  if (s->in_step) return fail(PYRO_E_STATE, "render_pack inside a step");
+  const size_t need = (size_t)s->g.n * 24;
+  if (host_dst && bytes != need)
+    return fail(PYRO_E_INVALID, "pyro_sim_render_pack: destination must hold 3 x N x 8 bytes (" +
+                                    std::to_string(need) + ")");
+  if (!s->render) {
+    int rc = dalloc(s, &s->render, need);
+    if (rc) return rc;
+  }
+  if (!s->copy_stream) HIPCHK(hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking));
+  if (!s->ev_packed) HIPCHK(hipEventCreateWithFlags(&s->ev_packed, hipEventDisableTiming));
+  if (!s->ev_copied) HIPCHK(hipEventCreateWithFlags(&s->ev_copied, hipEventDisableTiming));
+  // the previous hand-off's copy must have read the buffer before it is
+  // packed again
+  if (s->copy_issued) HIPCHK(hipStreamWaitEvent(s->stream, s->ev_copied, 0));
+  launch_render_pack(fields_of(s), s->render, s->stream);
+  HIPCHK(hipGetLastError());
+  if (!host_dst) return 0;
+  HIPCHK(hipEventRecord(s->ev_packed, s->stream));
+  HIPCHK(hipStreamWaitEvent(s->copy_stream, s->ev_packed, 0));
+  HIPCHK(hipMemcpyAsync(host_dst, s->render, need, hipMemcpyDeviceToHost, s->copy_stream));
+  HIPCHK(hipEventRecord(s->ev_copied, s->copy_stream));
+  s->copy_issued = true;
+  return 0;
+}
+
+int pyro_sim_render_wait(PyroSim *s) {
+  if (!s) return fail(PYRO_E_INVALID, "null sim");
+  if (!s->copy_issued) return 0;
+  hipError_t e = hipEventSynchronize(s->ev_copied);
+  if (e != hipSuccess) return fail(PYRO_E_HIP, std::string("render hand-off copy: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int pyro_host_alloc(size_t bytes, void **out) {
+  if (!out || !bytes) return fail(PYRO_E_INVALID, "pyro_host_alloc: bad argument");
+  *out = nullptr;
+  hipError_t e = hipHostMalloc(out, bytes, hipHostMallocDefault);
+  if (e != hipSuccess) return fail(PYRO_E_OOM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int pyro_host_free(void *p) {
+  if (!p) return 0;
+  hipError_t e = hipHostFree(p);
+  if (e != hipSuccess) return fail(PYRO_E_HIP, std::string("hipHostFree: ") + hipGetErrorString(e));
+  return 0;
+}
+
 int pyro_sim_destroy(PyroSim *s) {
   if (!s) return 0;
+  if (s->copy_stream) (void)hipStreamSynchronize(s->copy_stream);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
+  if (s->ev_packed) (void)hipEventDestroy(s->ev_packed);
+  if (s->ev_copied) (void)hipEventDestroy(s->ev_copied);
+  if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
   for (auto &a : s->gexec)
     for (auto &b : a)
       for (auto &c : b)
@@ -645,7 +709,9 @@ int pyro_sim_time_kernel(PyroSim *s, int kernel, int level, int reps, float *avg
   if (level < 0 || level >= s->nl) return fail(PYRO_E_INVALID, "bad level");
   if ((kernel == PYRO_KERNEL_MG_RESTRICT || kernel == PYRO_KERNEL_MG_PROLONG) && level >= s->nl - 1)
     return fail(PYRO_E_INVALID, "restrict/prolong need a coarser level");
-  if (kernel < 0 || kernel > PYRO_KERNEL_MG_COARSE) return fail(PYRO_E_INVALID, "bad kernel id");
+  if (kernel < 0 || kernel > PYRO_KERNEL_RENDER_PACK) return fail(PYRO_E_INVALID, "bad kernel id");
+  if (kernel == PYRO_KERNEL_RENDER_PACK && !s->render)
+    return fail(PYRO_E_STATE, "render_pack timing needs a pyro_sim_render_pack call first");
   if (kernel == PYRO_KERNEL_MG_SMOOTH && (reps & 1)) reps++;  // even: phi ends in its own buffer
   hipEvent_t a, b;
   HIPCHK(hipEventCreate(&a));
@@ -661,6 +727,7 @@ int pyro_sim_time_kernel(PyroSim *s, int kernel, int level, int reps, float *avg
       }
       case PYRO_KERNEL_MG_RESTRICT: launch_mg_restrict(s->lv[level], s->lv[level + 1], s->stream); break;
       case PYRO_KERNEL_MG_PROLONG: launch_mg_prolong(s->lv[level + 1], s->lv[level], s->stream); break;
+      case PYRO_KERNEL_RENDER_PACK: launch_render_pack(fields_of(s), s->render, s->stream); break;
       default: launch_mg_coarse(s->lv[s->nl - 1], COARSE_ITERS, 1.6f, s->stream); break;
     }
   }

@@ -8,6 +8,7 @@ meaning follow the reference's ``src/sim.rs``: ``Sim.new`` (:108),
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -21,7 +22,7 @@ STATS_N = 1040
 FIELDS = {
     "s0": 0, "s1": 1, "s0b": 2, "s1b": 3, "h0": 4, "h1": 5, "sol": 6, "sol_out": 7,
     "pyr": 8, "flags": 9, "curl": 10, "u": 11, "v": 12, "w": 13, "ub": 14, "vb": 15,
-    "wb": 16, "hu": 17, "hv": 18, "hw": 19,
+    "wb": 16, "hu": 17, "hv": 18, "hw": 19, "render": 20,
 }
 LEVEL_BASE = 32
 STAGES = [
@@ -180,6 +181,8 @@ class Sim:
             return (nz, ny, nx), np.uint32
         if 11 <= fid <= 19:
             return (nz + 1, ny + 1, nx + 1), np.float32
+        if fid == 20:
+            return (3, nz, ny, nx, 4), np.float16
         l, q = divmod(fid - LEVEL_BASE, 4)
         d = self.level_dims(l)
         return (d[2], d[1], d[0]), (np.uint32 if q == 2 else np.float32)
@@ -225,7 +228,7 @@ class Sim:
         return ms.value
 
     def time_kernel(self, kernel: str, level: int, reps: int) -> float:
-        kid = {"mg_smooth": 0, "mg_restrict": 1, "mg_prolong": 2, "mg_coarse": 3}[kernel]
+        kid = {"mg_smooth": 0, "mg_restrict": 1, "mg_prolong": 2, "mg_coarse": 3, "render_pack": 4}[kernel]
         ms = C.c_float()
         _native.check_hip(_native.hip().pyro_sim_time_kernel(self._h, kid, int(level), int(reps), C.byref(ms)))
         return ms.value
@@ -238,6 +241,57 @@ class Sim:
 
     def sync(self) -> None:
         _native.check_hip(_native.hip().pyro_sim_sync(self._h))
+
+    # -- renderer hand-off (render.rs:255-277, render.wgsl:16-37) ----------
+    def render_pack(self, dst: "PinnedBuffer | None" = None):
+        """Pack the current state into the renderer's volumes (vol, sol,
+        vel: (3, nz, ny, nx, 4) float16).  Without `dst` the call waits and
+        returns a new array; with a PinnedBuffer the copy is asynchronous and
+        overlaps the steps enqueued next: call render_wait() before reading
+        dst.array."""
+        nx, ny, nz = self.dims
+        nbytes = 24 * nx * ny * nz
+        lib = _native.hip()
+        if dst is None:
+            out = np.zeros((3, nz, ny, nx, 4), np.float16)
+            _native.check_hip(lib.pyro_sim_render_pack(self._h, out.ctypes.data, nbytes))
+            _native.check_hip(lib.pyro_sim_render_wait(self._h))
+            return out
+        if dst.ptr is None:
+            raise ValueError("render_pack: the buffer is closed")
+        if dst.nbytes != nbytes:
+            raise ValueError(f"render_pack: the buffer must hold {nbytes} bytes")
+        _native.check_hip(lib.pyro_sim_render_pack(self._h, dst.ptr, nbytes))
+        return dst
+
+    def render_wait(self) -> None:
+        _native.check_hip(_native.hip().pyro_sim_render_wait(self._h))
+
+
+class PinnedBuffer:
+    """Page-locked host memory (pyro_host_alloc) for the renderer hand-off,
+    viewed as the (3, nz, ny, nx, 4) float16 volumes of one simulation.  The
+    memory is released when the last array viewing it is gone, so a view
+    kept past close() stays valid.  Call Sim.render_wait() before reading
+    the array or dropping the buffer while a copy into it may be running."""
+
+    def __init__(self, dims):
+        nx, ny, nz = dims
+        self.nbytes = 24 * nx * ny * nz
+        p = C.c_void_p()
+        _native.check_hip(_native.hip().pyro_host_alloc(self.nbytes, C.byref(p)))
+        raw = (C.c_uint8 * self.nbytes).from_address(p.value)
+        weakref.finalize(raw, _host_free, p.value)
+        self.ptr = p
+        self.array = np.frombuffer(raw, dtype=np.float16).reshape(3, nz, ny, nx, 4)
+
+    def close(self) -> None:
+        self.array = None
+        self.ptr = None
+
+
+def _host_free(addr: int) -> None:
+    _native.hip().pyro_host_free(C.c_void_p(addr))
 
 
 def debug_math(fn: int, x: np.ndarray, y: np.ndarray | None = None) -> np.ndarray:

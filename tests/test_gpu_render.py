@@ -1,0 +1,104 @@
+"""Renderer hand-off (pyro_sim_render_pack; render.rs:255-277, render.wgsl
+pack :16-37) against the oracle's pack: the f32 values of the oracle rounded
+to float16 by numpy (round to nearest even) equal the device's half4 volumes
+bit for bit, including f16 overflow to infinity, subnormal results and
+values that are not representable; the asynchronous copy into pinned memory
+holds the state at the pack while the steps enqueued after it run.
+
+Packs are compared after at least one step: before it the reference's flags
+are still zero (sim.rs:130; the solid pass fills them, combust.wgsl:131),
+while this library derives them from SOL at creation."""
+
+import numpy as np
+import pytest
+
+import pyro_amd as pa
+from pyro_amd import _native
+from harness import mini_scene, twin
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits16(a):
+    return np.ascontiguousarray(a).view(np.uint16)
+
+
+def _assert_pack_equal(o_pack_f32, g_pack):
+    with np.errstate(over="ignore"):  # beyond 65504 rounds to infinity on both sides
+        want = o_pack_f32.astype(np.float16)
+    bad = _bits16(want) != _bits16(g_pack)
+    assert not bad.any(), (f"{int(bad.sum())} half values differ, first at {np.argwhere(bad)[0]}: "
+                           f"{want[tuple(np.argwhere(bad)[0])]} vs {g_pack[tuple(np.argwhere(bad)[0])]}")
+
+
+def test_pack_bitwise_after_steps():
+    o, g = twin(mini_scene(), seed=11)
+    for k in range(3):
+        o.step(0.004, False)
+        g.step(0.004, False)
+    pack = g.render_pack()
+    print(f"[render] mini dims={g.dims} solid={int(pack[1, ..., 0].astype(bool).sum())}", flush=True)
+    assert pack[1, ..., 0].astype(bool).sum() > 50  # solid cells present
+    _assert_pack_equal(o.render_pack(), pack)
+    assert np.array_equal(_bits16(g.read_field("render")), _bits16(pack))
+
+
+def test_pack_bitwise_over_the_half_range():
+    """Fields spread log-uniformly over 1e-12..1e9 (and signs): f16 overflow,
+    subnormals and rounding of every kind on the conversion."""
+    o, g = twin(mini_scene(), seed=12)
+    o.step(0.004, False)
+    g.step(0.004, False)
+    nx, ny, nz = g.dims
+    rng = np.random.default_rng(13)
+
+    def wide(shape):
+        return (np.sign(rng.uniform(-1, 1, shape)) * 10.0 ** rng.uniform(-12, 9, shape)).astype(np.float32)
+
+    s0 = o.read_field("s0")
+    s0[..., 3] = np.abs(wide((nz, ny, nx)))
+    s1 = wide((nz, ny, nx, 4))
+    vel = {f: wide((nz + 1, ny + 1, nx + 1)) for f in ("u", "v", "w")}
+    for sim in (o, g):
+        sim.write_field("s0", s0)
+        sim.write_field("s1", s1)
+        for f, a in vel.items():
+            sim.write_field(f, a)
+    pack = g.render_pack()
+    assert np.isinf(pack[0, ..., 3]).any() and (np.abs(pack[0, ..., 3]) < 6.1e-5).any()
+    _assert_pack_equal(o.render_pack(), pack)
+
+
+def test_async_handoff_overlaps_steps():
+    o, g = twin(mini_scene(), seed=14)
+    o.step(0.004, False)
+    g.step(0.004, False)
+    buf_a, buf_b = pa.PinnedBuffer(g.dims), pa.PinnedBuffer(g.dims)
+    try:
+        want_a = o.render_pack()
+        g.render_pack(buf_a)  # returns at once; the copy runs beside the steps
+        for k in range(3):
+            o.step(0.004, False)
+            g.step(0.004, False)
+        want_b = o.render_pack()
+        g.render_pack(buf_b)
+        g.step(0.004, False)
+        g.render_wait()
+        _assert_pack_equal(want_a, buf_a.array)
+        _assert_pack_equal(want_b, buf_b.array)
+        o.step(0.004, False)
+        _assert_pack_equal(o.render_pack(), g.render_pack())
+    finally:
+        g.render_wait()
+        buf_a.close()
+        buf_b.close()
+
+
+def test_pack_rejects_wrong_size():
+    g = pa.Sim.new(mini_scene())
+    n = g.dims[0] * g.dims[1] * g.dims[2]
+    out = np.zeros(24 * n, np.uint8)
+    rc = _native.hip().pyro_sim_render_pack(g._h, out.ctypes.data, 24 * n - 8)
+    assert rc == -1 and b"3 x N x 8" in _native.hip().pyro_last_error()
+    with pytest.raises(ValueError):
+        g.render_pack(type("Fake", (), {"nbytes": 8, "ptr": None})())

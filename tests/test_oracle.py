@@ -531,3 +531,61 @@ def test_noise_is_deterministic_and_bounded():
     v = np.array([oracle.noise(k, i, 0) for k in range(3) for i in range(3000)])
     assert v.min() >= -1.0 and v.max() < 1.0 and abs(v.mean()) < 0.05
     assert oracle.noise(1, 17, 0) != oracle.noise(1, 17, 1)
+
+
+# ------------------------------------------------------------------ renderer pack
+def test_render_pack_closed_forms():
+    """render.wgsl:16-37 against its physical meaning, in float64: soot volume
+    fraction f_v = rho Y_s / rho_soot (ppm), smoke rho Y_m (g/m3), heat
+    release q (kW/m3) with rho = rho0 T0 / max(T, 200); solids carry only T;
+    the solid volume is occupancy, surface temperature, char / (fuel + char)
+    and the material slot; linear face velocities average to their exact
+    value at the cell centre (which also pins the x-fastest layout)."""
+    sc = pa.Scene.parse(
+        "[domain]\nsize = [0.8, 0.8, 0.8]\ncell = 0.05\n"
+        '[[object]]\nkind = "box"\nmaterial = "wood"\ncenter = [0.4, 0.2, 0.4]\nsize = [0.3, 0.2, 0.3]\n'
+        '[[object]]\nkind = "box"\nmaterial = "steel"\ncenter = [0.2, 0.6, 0.2]\nsize = [0.15, 0.1, 0.15]\n')
+    o, p = make(sc)
+    nx, ny, nz = sc.grid().dims
+    rng = np.random.default_rng(5)
+    s0 = np.stack([rng.uniform(150, 2200, (nz, ny, nx)), rng.uniform(0, 0.1, (nz, ny, nx)),
+                   rng.uniform(0, 0.23, (nz, ny, nx)), rng.uniform(0, 0.02, (nz, ny, nx))], -1).astype(F32)
+    s1 = np.stack([rng.uniform(0, 0.05, (nz, ny, nx)), rng.uniform(0, 5e6, (nz, ny, nx)),
+                   rng.uniform(-1, 1, (nz, ny, nx)), np.zeros((nz, ny, nx))], -1).astype(F32)
+    sol = o.read_field("sol")
+    wood = sol[..., 0] == 1.0
+    sol[..., 2] = np.where(wood, rng.uniform(0, 0.5, wood.shape), 0.0)
+    sol[..., 3] = np.where(sol[..., 1] > 0, rng.uniform(300, 900, wood.shape), sol[..., 3])
+    o.write_field("s0", s0)
+    o.write_field("s1", s1)
+    o.write_field("sol", sol)
+    # flags come from the first solid pass (Sim::new marks solids with it)
+    o.begin_step(0.002, False)
+    o.run_stage("solid")
+    o.end_step()
+    z, y, x = np.meshgrid(np.arange(nz + 1), np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
+    o.write_field("u", (3 * x - 2 * y + z).astype(F32))
+    o.write_field("v", (x + 5 * y - 4 * z).astype(F32))
+    o.write_field("w", (-x + y + 7 * z).astype(F32))
+    s0, s1, sol = o.read_field("s0"), o.read_field("s1"), o.read_field("sol")
+    solid = o.read_field("flags") == 1
+    assert solid.sum() > 50 and (~solid).sum() > 1000
+    vol, sl, vel = o.render_pack().astype(np.float64)
+    T = s0[..., 0].astype(np.float64)
+    rho = p["rho0"] * p["t0"] / np.maximum(T, 200.0)
+    fl = ~solid
+    np.testing.assert_allclose(vol[fl, 0], rho[fl] * s0[fl, 3] / 1800.0 * 1e6, rtol=3e-6, atol=1e-30)
+    np.testing.assert_allclose(vol[fl, 2], rho[fl] * s1[fl, 0] * 1e3, rtol=3e-6, atol=1e-30)
+    np.testing.assert_allclose(vol[fl, 3], s1[fl, 1] / 1e3, rtol=2e-7)
+    assert np.array_equal(vol[..., 1], T)
+    assert not vol[solid][:, [0, 2, 3]].any()
+    np.testing.assert_array_equal(sl[..., 0], solid.astype(np.float64))
+    assert np.array_equal(sl[solid, 1], sol[solid, 3]) and np.array_equal(sl[solid, 3], sol[solid, 0])
+    tot = sol[solid, 1].astype(np.float64) + sol[solid, 2]
+    np.testing.assert_allclose(sl[solid, 2], sol[solid, 2] / np.maximum(tot, 1e-6), rtol=3e-7, atol=1e-30)
+    assert not sl[fl].any()
+    zc, yc, xc = np.meshgrid(np.arange(nz) + 0.5, np.arange(ny) + 0.5, np.arange(nx) + 0.5, indexing="ij")
+    np.testing.assert_array_equal(vel[..., 0], 3 * xc - 2 * (yc - 0.5) + (zc - 0.5))
+    np.testing.assert_array_equal(vel[..., 1], (xc - 0.5) + 5 * yc - 4 * (zc - 0.5))
+    np.testing.assert_array_equal(vel[..., 2], -(xc - 0.5) + (yc - 0.5) + 7 * zc)
+    assert not vel[..., 3].any()
