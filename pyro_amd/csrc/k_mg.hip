@@ -622,7 +622,7 @@ __global__ __launch_bounds__(CO_NT) void k_mg_coarse_z(LvArgs a, const float *__
   const int PZ = (a.d2 + 2) | 1, PX = a.d0 + 2, PY = a.d1 + 2;
   const int SY = PX * PZ, PN = PY * SY;
   const uint32_t sy = (uint32_t)a.d0, sz = (uint32_t)a.d0 * (uint32_t)a.d1;
-  for (int i = threadIdx.x; i < PN; i += CO_NT) sp[i] = 0.0f;
+  for (int i = threadIdx.x; i < PN; i += (int)blockDim.x) sp[i] = 0.0f;
   const int col = threadIdx.x;
   const bool has = col < a.d0 * a.d1;
   const int x = has ? col % a.d0 : 0, y = has ? col / a.d0 : 0;
@@ -673,26 +673,84 @@ __global__ __launch_bounds__(CO_NT) void k_mg_coarse_z(LvArgs a, const float *__
       // every slot of the colour is computed (reads stay inside sp: the
       // host keeps PN + ZJ within it) and kept only where the count is not
       // 0; the stores follow all reads of the pass, so no read waits on them
-      // the next slot's four reads are issued before this slot's arithmetic
-      float nb[4] = {sxp[c], sxm[c], syp[c], sym[c]};
+      if constexpr (ZJ <= 12) {
+        // Short columns (one wave per SIMD on small levels): the slots of a
+        // colour are independent (red-black), so all of the pass's LDS reads
+        // are issued first, the fast quotients of every slot are formed
+        // branch-free, and one branch per pass redoes by IEEE division any
+        // quotient that came out subnormal (div_count); the slots' latencies
+        // overlap instead of queueing behind one another.
+        constexpr int NS = ZJ / 2;
+        float xs[NS], qs[NS], nb[4][NS];
+        bool redo = false;
 #pragma unroll
-      for (int j = c; j < ZJ; j += 2) {
-        float nx[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        if (j + 2 < ZJ) {
-          nx[0] = sxp[j + 2]; nx[1] = sxm[j + 2]; nx[2] = syp[j + 2]; nx[3] = sym[j + 2];
+        for (int t = 0; t < NS; t++) {
+          const int j = c + 2 * t;
+          nb[0][t] = sxp[j];
+          nb[1][t] = sxm[j];
+          nb[2][t] = syp[j];
+          nb[3][t] = sym[j];
         }
-        const float cn = (float)((cp[j >> 2] >> (8 * (j & 3))) & 0xffu);
-        float sum = 0.0f;
-        sum += nb[0];
-        sum += nb[1];
-        sum += nb[2];
-        sum += nb[3];
-        sum += ph[j + 2];
-        sum += ph[j];
-        const float nv = mixf_(ph[j + 1], div_count(sum - hb[j], cn, rc[j]), omega);
-        ph[j + 1] = cn != 0.0f ? nv : ph[j + 1];
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int k = 0; k < 4; k++) nb[k] = nx[k];
+        for (int t = 0; t < NS; t++) {
+          const int j = c + 2 * t;
+          const float cn = (float)((cp[j >> 2] >> (8 * (j & 3))) & 0xffu);
+          float sum = 0.0f;
+          sum += nb[0][t];
+          sum += nb[1][t];
+          sum += nb[2][t];
+          sum += nb[3][t];
+          sum += ph[j + 2];
+          sum += ph[j];
+          const float x = sum - hb[j];
+          const float q0 = x * rc[j];
+          const float e = __builtin_fmaf(-q0, cn, x);
+          xs[t] = x;
+          qs[t] = __builtin_fmaf(e, rc[j], q0);
+          redo |= __builtin_amdgcn_classf(qs[t], 0x90);
+        }
+        if (redo) {
+#pragma unroll
+          for (int t = 0; t < NS; t++) {
+            const int j = c + 2 * t;
+            if (__builtin_amdgcn_classf(qs[t], 0x90)) {
+              float x = xs[t];
+              asm volatile("" : "+v"(x));
+              qs[t] = x / (float)((cp[j >> 2] >> (8 * (j & 3))) & 0xffu);
+            }
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < NS; t++) {
+          const int j = c + 2 * t;
+          const float nv = mixf_(ph[j + 1], qs[t], omega);
+          ph[j + 1] = ((cp[j >> 2] >> (8 * (j & 3))) & 0xffu) ? nv : ph[j + 1];
+        }
+      } else {
+        // Long columns: many waves share each SIMD and hide the latency;
+        // registers are the limit (no per-pass arrays).  The next slot's
+        // four reads are issued before this slot's arithmetic.
+        float nb[4] = {sxp[c], sxm[c], syp[c], sym[c]};
+#pragma unroll
+        for (int j = c; j < ZJ; j += 2) {
+          float nx[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+          if (j + 2 < ZJ) {
+            nx[0] = sxp[j + 2]; nx[1] = sxm[j + 2]; nx[2] = syp[j + 2]; nx[3] = sym[j + 2];
+          }
+          const float cn = (float)((cp[j >> 2] >> (8 * (j & 3))) & 0xffu);
+          float sum = 0.0f;
+          sum += nb[0];
+          sum += nb[1];
+          sum += nb[2];
+          sum += nb[3];
+          sum += ph[j + 2];
+          sum += ph[j];
+          const float nv = mixf_(ph[j + 1], div_count(sum - hb[j], cn, rc[j]), omega);
+          ph[j + 1] = cn != 0.0f ? nv : ph[j + 1];
+#pragma unroll
+          for (int k = 0; k < 4; k++) nb[k] = nx[k];
+        }
       }
 #pragma unroll
       for (int j = c; j < ZJ; j += 2)
@@ -749,12 +807,15 @@ void launch_mg_coarse(const MgLevel &L, int iters, float omega, hipStream_t s) {
   const uint32_t pz = (uint32_t)((L.d[2] + 2) | 1);
   const bool colz = L.wdir == 1.0f && L.d[0] * L.d[1] <= CO_NT &&
                     pz * (uint32_t)(L.d[0] + 2) * (uint32_t)(L.d[1] + 2) + 24u <= (uint32_t)CO_PAD_MAX;
+  // one thread per column, whole waves only: a wave without columns would
+  // still run every colour pass's arithmetic on the CU
+  const int colt = ((L.d[0] * L.d[1] + 63) / 64) * 64;
   if (colz && L.d[2] + 1 <= 12) {
-    k_mg_coarse_z<12><<<1, CO_NT, 0, s>>>(a, L.b, L.f, L.nc, L.phi, iters, omega);
+    k_mg_coarse_z<12><<<1, colt, 0, s>>>(a, L.b, L.f, L.nc, L.phi, iters, omega);
     return;
   }
   if (colz && L.d[2] + 1 <= 22) {
-    k_mg_coarse_z<22><<<1, CO_NT, 0, s>>>(a, L.b, L.f, L.nc, L.phi, iters, omega);
+    k_mg_coarse_z<22><<<1, colt, 0, s>>>(a, L.b, L.f, L.nc, L.phi, iters, omega);
     return;
   }
   if (mg_coarse_fits_lds(L))
