@@ -59,30 +59,6 @@ __device__ __forceinline__ float triC(const Geo &g, const RB &X, float gx0, floa
   float c11 = mixf_(p11.x, p11.y, fx);
   return mixf_(mixf_(c00, c10, fy), mixf_(c01, c11, fy), fz);
 }
-// 8-corner min/max at the sample's base cell (tmpl_comp.wgsl:16-25).
-__device__ __forceinline__ void mmC(const Geo &g, const RB &X, float gx0, float gy0, float gz0,
-                                    int hx, int hy, int hz, float &lo, float &hi) {
-  float gx = clampf_(gx0, 0.0f, (float)hx), gy = clampf_(gy0, 0.0f, (float)hy), gz = clampf_(gz0, 0.0f, (float)hz);
-  int bx = (int)floorf(gx), by = (int)floorf(gy), bz = (int)floorf(gz);
-  bx = bx > hx - 1 ? hx - 1 : bx;
-  by = by > hy - 1 ? hy - 1 : by;
-  bz = bz > hz - 1 ? hz - 1 : bz;
-  uint32_t sy = (uint32_t)g.vx, sz = (uint32_t)g.vx * (uint32_t)g.vy;
-  uint32_t i = vidx(g, bx, by, bz);
-  const uint32_t o[4] = {0, sy, sz, sy + sz};
-  float l = 1e30f, h = -1e30f;
-#pragma unroll
-  for (int j = 0; j < 4; j++) {
-    const float2 v = ld2(X, i + o[j]);
-    l = fmin_(l, v.x);
-    h = fmax_(h, v.x);
-    l = fmin_(l, v.y);
-    h = fmax_(h, v.y);
-  }
-  lo = l;
-  hi = h;
-}
-
 // velAt / traceBack / traceFwd (vel.wgsl:2-19) in cell units.
 __device__ __forceinline__ void velAt(const Geo &g, const VelR &V, float px, float py, float pz, float &ux,
                                       float &uy, float &uz) {
@@ -104,15 +80,151 @@ __device__ __forceinline__ void trace(const Geo &g, const VelR &V, float k, floa
     ox = x + k * u2x; oy = y + k * u2y; oz = z + k * u2z;
   }
 }
-// traceBack and traceFwd from one point share their first sample u1 = velAt(x).
-__device__ __forceinline__ void trace2(const Geo &g, const VelR &V, float k, float x, float y, float z, float &bx,
-                                       float &by, float &bz, float &fx, float &fy, float &fz) {
+// ---------------------------------------------------------------- velocity tile
+// Each block stages the three padded velocity arrays over its brick plus a
+// halo of VT_H nodes in LDS.  A sample whose eight corners all lie in the
+// tile reads them there; any other (a trace longer than the halo) gathers
+// from global memory as before.  The arithmetic is the same either way, so
+// the results do not depend on where the corners came from.  A cell brick
+// and a face brick index the same padded grid (face (x, y, z) is the low
+// face of cell (x, y, z)), so one tile shape serves every kernel.  It
+// pays where a thread samples the velocity often (adv_vel1: 21 samples,
+// adv_vel2: 33, adv_s2: 9 plus the limiter's gathers); adv_s1 (6 samples)
+// is faster gathering directly, and so were all of them with a 3-node halo
+// (29.6 KB of LDS per block: fewer blocks in flight, longer staging).  Two
+// nodes cover traces up to about one cell per step, the fires' range; a
+// longer trace reads global memory.
+constexpr int VT_H = 2;
+constexpr int VT_X = BR_X + 2 * VT_H, VT_Y = BR_Y + 2 * VT_H, VT_Z = BR_Z + 2 * VT_H;
+constexpr int VT_N = VT_X * VT_Y * VT_Z;
+static_assert(VT_X * VT_Y + VT_X + 1 < 256, "corner offsets fit a ds_read2 offset");
+
+struct VTile {
+  const float *s;  // component c at s + c * VT_N, x fastest, then y, then z
+  int x0, y0, z0;  // global node index of the tile's first entry
+};
+
+__device__ __forceinline__ void brick_origin(uint32_t b, const FastDiv &dbx, const FastDiv &dby, int &x0, int &y0,
+                                             int &z0) {
+  const uint32_t r = fdiv(b, dbx), bx = b - r * dbx.d;
+  const uint32_t bz = fdiv(r, dby), by = r - bz * dby.d;
+  x0 = (int)(bx * BR_X);
+  y0 = (int)(by * BR_Y);
+  z0 = (int)(bz * BR_Z);
+}
+
+// Fill the tile (entries outside the padded arrays read 0 through the
+// buffer resource's range check; no sample reads them) and wait for it.
+__device__ __forceinline__ VTile stage_vtile(const Geo &g, const VelR &V, float *sm, int bx0, int by0, int bz0) {
+  const VTile T{sm, bx0 - VT_H, by0 - VT_H, bz0 - VT_H};
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const RB &X = c == 0 ? V.u : (c == 1 ? V.v : V.w);
+    for (int e = threadIdx.x; e < VT_N; e += NT) {
+      const int lz = e / (VT_X * VT_Y), r = e - lz * (VT_X * VT_Y);
+      const int ly = r / VT_X, lx = r - ly * VT_X;
+      const int gx = T.x0 + lx, gy = T.y0 + ly, gz = T.z0 + lz;
+      const bool in = (unsigned)gx < (unsigned)g.vx && (unsigned)gy < (unsigned)g.vy && (unsigned)gz <= (unsigned)g.nz;
+      const uint32_t off = in ? vidx(g, gx, gy, gz) * 4u : RB_OOB;
+      sm[c * VT_N + e] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(X.r, off, 0, 0));
+    }
+  }
+  __syncthreads();
+  return T;
+}
+
+// Base node and weights of a staggered sample, as in triC / mmC.
+struct CBase {
+  int bx, by, bz;
+  float fx, fy, fz;
+};
+__device__ __forceinline__ CBase cbase(float gx0, float gy0, float gz0, int hx, int hy, int hz) {
+  const float gx = clampf_(gx0, 0.0f, (float)hx), gy = clampf_(gy0, 0.0f, (float)hy),
+              gz = clampf_(gz0, 0.0f, (float)hz);
+  CBase b;
+  b.bx = (int)floorf(gx);
+  b.by = (int)floorf(gy);
+  b.bz = (int)floorf(gz);
+  b.bx = b.bx > hx - 1 ? hx - 1 : b.bx;
+  b.by = b.by > hy - 1 ? hy - 1 : b.by;
+  b.bz = b.bz > hz - 1 ? hz - 1 : b.bz;
+  b.fx = gx - (float)b.bx;
+  b.fy = gy - (float)b.by;
+  b.fz = gz - (float)b.bz;
+  return b;
+}
+// The four x-pairs of a sample's corners: from the tile, or gathered.
+__device__ __forceinline__ void corners(const Geo &g, const VTile &T, int c, const RB &X, const CBase &b,
+                                        float2 &p00, float2 &p10, float2 &p01, float2 &p11) {
+  const int lx = b.bx - T.x0, ly = b.by - T.y0, lz = b.bz - T.z0;
+  if ((unsigned)lx < (unsigned)(VT_X - 1) && (unsigned)ly < (unsigned)(VT_Y - 1) &&
+      (unsigned)lz < (unsigned)(VT_Z - 1)) {
+    const float *q = T.s + c * VT_N + (lz * VT_Y + ly) * VT_X + lx;
+    p00 = make_float2(q[0], q[1]);
+    p10 = make_float2(q[VT_X], q[VT_X + 1]);
+    p01 = make_float2(q[VT_X * VT_Y], q[VT_X * VT_Y + 1]);
+    p11 = make_float2(q[VT_X * VT_Y + VT_X], q[VT_X * VT_Y + VT_X + 1]);
+  } else {
+    const uint32_t sy = (uint32_t)g.vx, sz = (uint32_t)g.vx * (uint32_t)g.vy, i = vidx(g, b.bx, b.by, b.bz);
+    p00 = ld2(X, i);
+    p10 = ld2(X, i + sy);
+    p01 = ld2(X, i + sz);
+    p11 = ld2(X, i + sy + sz);
+  }
+}
+// triC (tmpl_comp.wgsl:2-15) of velocity component c through the tile.
+__device__ __forceinline__ float triCT(const Geo &g, const VTile &T, int c, const RB &X, float gx0, float gy0,
+                                       float gz0, int hx, int hy, int hz) {
+  const CBase b = cbase(gx0, gy0, gz0, hx, hy, hz);
+  float2 p00, p10, p01, p11;
+  corners(g, T, c, X, b, p00, p10, p01, p11);
+  const float c00 = mixf_(p00.x, p00.y, b.fx);
+  const float c10 = mixf_(p10.x, p10.y, b.fx);
+  const float c01 = mixf_(p01.x, p01.y, b.fx);
+  const float c11 = mixf_(p11.x, p11.y, b.fx);
+  return mixf_(mixf_(c00, c10, b.fy), mixf_(c01, c11, b.fy), b.fz);
+}
+// mmC (tmpl_comp.wgsl:16-25) through the tile.
+__device__ __forceinline__ void mmCT(const Geo &g, const VTile &T, int c, const RB &X, float gx0, float gy0,
+                                     float gz0, int hx, int hy, int hz, float &lo, float &hi) {
+  const CBase b = cbase(gx0, gy0, gz0, hx, hy, hz);
+  float2 p[4];
+  corners(g, T, c, X, b, p[0], p[1], p[2], p[3]);
+  float l = 1e30f, h = -1e30f;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    l = fmin_(l, p[j].x);
+    h = fmax_(h, p[j].x);
+    l = fmin_(l, p[j].y);
+    h = fmax_(h, p[j].y);
+  }
+  lo = l;
+  hi = h;
+}
+__device__ __forceinline__ void velAtT(const Geo &g, const VTile &T, const VelR &V, float px, float py, float pz,
+                                       float &ux, float &uy, float &uz) {
+  ux = triCT(g, T, 0, V.u, px, py - 0.5f, pz - 0.5f, g.nx, g.ny - 1, g.nz - 1);
+  uy = triCT(g, T, 1, V.v, px - 0.5f, py, pz - 0.5f, g.nx - 1, g.ny, g.nz - 1);
+  uz = triCT(g, T, 2, V.w, px - 0.5f, py - 0.5f, pz, g.nx - 1, g.ny - 1, g.nz);
+}
+__device__ __forceinline__ void traceBackT(const Geo &g, const VTile &T, const VelR &V, float k, float x, float y,
+                                           float z, float &ox, float &oy, float &oz) {
   float u1x, u1y, u1z, u2x, u2y, u2z;
-  velAt(g, V, x, y, z, u1x, u1y, u1z);
+  velAtT(g, T, V, x, y, z, u1x, u1y, u1z);
   const float hk = 0.5f * k;
-  velAt(g, V, x - hk * u1x, y - hk * u1y, z - hk * u1z, u2x, u2y, u2z);
+  velAtT(g, T, V, x - hk * u1x, y - hk * u1y, z - hk * u1z, u2x, u2y, u2z);
+  ox = x - k * u2x;
+  oy = y - k * u2y;
+  oz = z - k * u2z;
+}
+__device__ __forceinline__ void trace2T(const Geo &g, const VTile &T, const VelR &V, float k, float x, float y,
+                                        float z, float &bx, float &by, float &bz, float &fx, float &fy, float &fz) {
+  float u1x, u1y, u1z, u2x, u2y, u2z;
+  velAtT(g, T, V, x, y, z, u1x, u1y, u1z);
+  const float hk = 0.5f * k;
+  velAtT(g, T, V, x - hk * u1x, y - hk * u1y, z - hk * u1z, u2x, u2y, u2z);
   bx = x - k * u2x; by = y - k * u2y; bz = z - k * u2z;
-  velAt(g, V, x + hk * u1x, y + hk * u1y, z + hk * u1z, u2x, u2y, u2z);
+  velAtT(g, T, V, x + hk * u1x, y + hk * u1y, z + hk * u1z, u2x, u2y, u2z);
   fx = x + k * u2x; fy = y + k * u2y; fz = z + k * u2z;
 }
 
@@ -120,24 +232,28 @@ __device__ __forceinline__ void trace2(const Geo &g, const VelR &V, float k, flo
 __global__ __launch_bounds__(NT) void k_adv_vel1(const Params *__restrict__ P, Geo g, VelPtrs VP,
                                                   float *__restrict__ hu, float *__restrict__ hv,
                                                   float *__restrict__ hw) {
-  int x, y, z;
-  brick_cell((uint32_t)xcd_tile(blockIdx.x, gridDim.x), threadIdx.x, g.dfbx, g.dfby, x, y, z);
+  __shared__ float sm[3 * VT_N];
+  const uint32_t blk = (uint32_t)xcd_tile(blockIdx.x, gridDim.x);
+  int x, y, z, x0, y0, z0;
+  brick_cell(blk, threadIdx.x, g.dfbx, g.dfby, x, y, z);
+  brick_origin(blk, g.dfbx, g.dfby, x0, y0, z0);
+  const VelR V = velr_of(g, VP);
+  const VTile T = stage_vtile(g, V, sm, x0, y0, z0);
   if (x >= g.vx || y >= g.vy || z > g.nz) return;
   const uint32_t i = vidx(g, x, y, z);
-  const VelR V = velr_of(g, VP);
   const float k = P->dt / P->dx;
   float bx, by, bz;
   if (x <= g.nx && y <= g.ny - 1 && z <= g.nz - 1) {
-    trace<-1>(g, V, k, (float)x, (float)y + 0.5f, (float)z + 0.5f, bx, by, bz);
-    hu[i] = triC(g, V.u, bx, by - 0.5f, bz - 0.5f, g.nx, g.ny - 1, g.nz - 1);
+    traceBackT(g, T, V, k, (float)x, (float)y + 0.5f, (float)z + 0.5f, bx, by, bz);
+    hu[i] = triCT(g, T, 0, V.u, bx, by - 0.5f, bz - 0.5f, g.nx, g.ny - 1, g.nz - 1);
   }
   if (x <= g.nx - 1 && y <= g.ny && z <= g.nz - 1) {
-    trace<-1>(g, V, k, (float)x + 0.5f, (float)y, (float)z + 0.5f, bx, by, bz);
-    hv[i] = triC(g, V.v, bx - 0.5f, by, bz - 0.5f, g.nx - 1, g.ny, g.nz - 1);
+    traceBackT(g, T, V, k, (float)x + 0.5f, (float)y, (float)z + 0.5f, bx, by, bz);
+    hv[i] = triCT(g, T, 1, V.v, bx - 0.5f, by, bz - 0.5f, g.nx - 1, g.ny, g.nz - 1);
   }
   if (x <= g.nx - 1 && y <= g.ny - 1 && z <= g.nz) {
-    trace<-1>(g, V, k, (float)x + 0.5f, (float)y + 0.5f, (float)z, bx, by, bz);
-    hw[i] = triC(g, V.w, bx - 0.5f, by - 0.5f, bz, g.nx - 1, g.ny - 1, g.nz);
+    traceBackT(g, T, V, k, (float)x + 0.5f, (float)y + 0.5f, (float)z, bx, by, bz);
+    hw[i] = triCT(g, T, 2, V.w, bx - 0.5f, by - 0.5f, bz, g.nx - 1, g.ny - 1, g.nz);
   }
 }
 void launch_adv_vel1(const Fields &f, hipStream_t s) {
@@ -155,12 +271,16 @@ __global__ __launch_bounds__(NT) void k_adv_vel2(const Params *__restrict__ P, G
                                                   const float *__restrict__ hw, float *__restrict__ ou,
                                                   float *__restrict__ ov, float *__restrict__ ow,
                                                   const uint8_t *__restrict__ fl) {
-  int x, y, z;
-  brick_cell((uint32_t)xcd_tile(blockIdx.x, gridDim.x), threadIdx.x, g.dfbx, g.dfby, x, y, z);
+  __shared__ float sm[3 * VT_N];
+  const uint32_t blk = (uint32_t)xcd_tile(blockIdx.x, gridDim.x);
+  int x, y, z, x0, y0, z0;
+  brick_cell(blk, threadIdx.x, g.dfbx, g.dfby, x, y, z);
+  brick_origin(blk, g.dfbx, g.dfby, x0, y0, z0);
+  const VelR V = velr_of(g, VP);
+  const VTile T = stage_vtile(g, V, sm, x0, y0, z0);
   if (x >= g.vx || y >= g.vy || z > g.nz) return;
   const uint32_t i = vidx(g, x, y, z);
-  const VelR V = velr_of(g, VP);
-  const RB H[3] = {rb_of(hu, g.nv * 4u), rb_of(hv, g.nv * 4u), rb_of(hw, g.nv * 4u)};
+  const RB Hu = rb_of(hu, g.nv * 4u), Hv = rb_of(hv, g.nv * 4u), Hw = rb_of(hw, g.nv * 4u);
   const float k = P->dt / P->dx;
   const bool sc = solidC(g, fl, x, y, z);
 #pragma unroll 1
@@ -174,14 +294,14 @@ __global__ __launch_bounds__(NT) void k_adv_vel2(const Params *__restrict__ P, G
     }
     const float offx = c == 0 ? 0.0f : 0.5f, offy = c == 1 ? 0.0f : 0.5f, offz = c == 2 ? 0.0f : 0.5f;
     const RB &Vc = c == 0 ? V.u : (c == 1 ? V.v : V.w);
-    const RB &Hc = H[c];
+    const RB &Hc = c == 0 ? Hu : (c == 1 ? Hv : Hw);
     float px = (float)x + offx, py = (float)y + offy, pz = (float)z + offz;
     float bx, by, bz, fx, fy, fz;
-    trace2(g, V, k, px, py, pz, bx, by, bz, fx, fy, fz);
+    trace2T(g, T, V, k, px, py, pz, bx, by, bz, fx, fy, fz);
     bx = bx - offx; by = by - offy; bz = bz - offz;
     fx = fx - offx; fy = fy - offy; fz = fz - offz;
     float lo, hi;
-    mmC(g, Vc, bx, by, bz, hx, hy, hz, lo, hi);
+    mmCT(g, T, c, Vc, bx, by, bz, hx, hy, hz, lo, hi);
     float t = triC(g, Hc, fx, fy, fz, hx, hy, hz);
     O[i] = clampf_(ldf(Hc, i) + 0.5f * (ldf(Vc, i) - t), lo, hi);
   }
@@ -275,17 +395,21 @@ __global__ __launch_bounds__(NT) void k_adv_s2(const Params *__restrict__ P, Geo
                                                 const float4 *__restrict__ s0p, const float4 *__restrict__ s1p,
                                                 const float4 *__restrict__ h0p, const float4 *__restrict__ h1p,
                                                 float4 *__restrict__ o0, float4 *__restrict__ o1) {
-  int x, y, z;
-  brick_cell((uint32_t)xcd_tile(blockIdx.x, gridDim.x), threadIdx.x, g.dcbx, g.dcby, x, y, z);
+  __shared__ float sm[3 * VT_N];
+  const uint32_t blk = (uint32_t)xcd_tile(blockIdx.x, gridDim.x);
+  int x, y, z, x0, y0, z0;
+  brick_cell(blk, threadIdx.x, g.dcbx, g.dcby, x, y, z);
+  brick_origin(blk, g.dcbx, g.dcby, x0, y0, z0);
+  const VelR V = velr_of(g, VP);
+  const VTile T = stage_vtile(g, V, sm, x0, y0, z0);
   if (x >= g.nx || y >= g.ny || z >= g.nz) return;
   const uint32_t i = sidx(g, x, y, z);
-  const VelR V = velr_of(g, VP);
   const RB s0 = rb_of(s0p, g.n * 16u), s1 = rb_of(s1p, g.n * 16u), h0 = rb_of(h0p, g.n * 16u),
            h1 = rb_of(h1p, g.n * 16u);
   const float k = P->dt / P->dx;
   float px = (float)x + 0.5f, py = (float)y + 0.5f, pz = (float)z + 0.5f;
   float bx, by, bz, fx, fy, fz;
-  trace2(g, V, k, px, py, pz, bx, by, bz, fx, fy, fz);
+  trace2T(g, T, V, k, px, py, pz, bx, by, bz, fx, fy, fz);
   bx = bx - 0.5f; by = by - 0.5f; bz = bz - 0.5f;
   fx = fx - 0.5f; fy = fy - 0.5f; fz = fz - 0.5f;
   float4 r0 = ld4(h0, i), r1 = ld4(h1, i);
