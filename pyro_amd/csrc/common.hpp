@@ -140,11 +140,13 @@ struct Geo {
 
 // Brick order for the gather-heavy advection kernels: block b (after
 // xcd_tile) takes brick b of the brick grid (x fastest, then y, then z) and
-// thread t its cell (t % 16, (t / 16) % 2, t / 32).  A block's trilinear
+// thread t its cell (t % 16, (t / 16) % 4, t / 64).  A block's trilinear
 // samples touch the rows and planes around its brick, which the blocks in
-// flight on the same XCD share: consecutive blocks walk x, then y, and a
-// plane is fetched again only at the edges of an 8-plane brick layer.
-constexpr int BR_X = 16, BR_Y = 2, BR_Z = 8;
+// flight on the same XCD share: consecutive blocks walk x, then y.  The
+// kernels that stage the brick's velocity in LDS make the halo's share of
+// the tile the cost to minimise; 16x4x4 measured fastest of 16x2x8, 16x4x4,
+// 32x2x4 and 8x4x8.
+constexpr int BR_X = 16, BR_Y = 4, BR_Z = 4;
 __device__ __forceinline__ void brick_cell(uint32_t b, uint32_t t, const FastDiv &dbx, const FastDiv &dby, int &x,
                                            int &y, int &z) {
   const uint32_t r = fdiv(b, dbx), bx = b - r * dbx.d;

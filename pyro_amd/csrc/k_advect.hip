@@ -91,9 +91,9 @@ __device__ __forceinline__ void trace(const Geo &g, const VelR &V, float k, floa
 // pays where a thread samples the velocity often (adv_vel1: 21 samples,
 // adv_vel2: 33, adv_s2: 9 plus the limiter's gathers); adv_s1 (6 samples)
 // is faster gathering directly, and so were all of them with a 3-node halo
-// (29.6 KB of LDS per block: fewer blocks in flight, longer staging).  Two
-// nodes cover traces up to about one cell per step, the fires' range; a
-// longer trace reads global memory.
+// (29.6 KB of LDS per 16x2x8 block: fewer blocks in flight, longer
+// staging).  Two nodes (15 KB per 16x4x4 block) cover traces up to about
+// one cell per step, the fires' range; a longer trace reads global memory.
 constexpr int VT_H = 2;
 constexpr int VT_X = BR_X + 2 * VT_H, VT_Y = BR_Y + 2 * VT_H, VT_Z = BR_Z + 2 * VT_H;
 constexpr int VT_N = VT_X * VT_Y * VT_Z;
