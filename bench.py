@@ -447,38 +447,44 @@ def bonfire_c5(args, net):
 
 def voxelize_leg():
     """Scene::voxelize (scene.rs:621-660) of the bonfire (57 shapes) on the
-    device (pyro_voxelize into device memory, shape upload and synchronisation
-    included) at 10 cm and 5 cm, and the host loop at 10 cm."""
+    device at 10 cm and 5 cm: pyro_voxelize into a simulation's SOL buffer
+    (shape upload, kernel and synchronisation), and pyro_sim_voxelize (the
+    same plus the second SOL buffer and the solid flags); the host loop at
+    10 cm."""
     import ctypes as C
-
-    import torch
 
     import pyro_amd as pa
     from pyro_amd import _native
 
     out = {}
+    lib = _native.hip()
     for cell in (0.1, 0.05):
         s = pa.Scene.load(str(ROOT / "pyro_amd" / "scenes" / "bonfire.toml"))
         s.set("domain.cell", cell)
         sh = s.shapes()
         dims, dx, t0, seed = s.voxel_spec()
         n = dims[0] * dims[1] * dims[2]
-        dst = torch.empty(4 * n, dtype=torch.float32, device="cuda")
+        sim = pa.Sim(s.params(), None, None, 4, s.get("ignition.until"))
+        ptr, nbytes = C.c_void_p(), C.c_size_t()
+        _native.check_hip(lib.pyro_sim_device_ptr(sim._h, pa.FIELDS["sol"], C.byref(ptr), C.byref(nbytes)))
         d = (C.c_uint32 * 3)(*dims)
-        ts = []
+        ts, tsim = [], []
         for _ in range(4):
-            torch.cuda.synchronize()
+            sim.sync()
             t = time.perf_counter()
-            _native.check_hip(_native.hip().pyro_voxelize(sh.ctypes.data, sh.size, d, dx, t0, seed,
-                                                         dst.data_ptr(), 1))
+            _native.check_hip(lib.pyro_voxelize(sh.ctypes.data, sh.size, d, dx, t0, seed, ptr, 1))
             ts.append(time.perf_counter() - t)
+            t = time.perf_counter()
+            sim.voxelize(s)
+            tsim.append(time.perf_counter() - t)
         key = f"bonfire_{cell * 100:g}cm"
-        out[key] = {"grid": list(dims), "cells": n, "shapes": int(sh.size), "device_ms": 1e3 * min(ts[1:])}
+        out[key] = {"grid": list(dims), "cells": n, "shapes": int(sh.size), "device_ms": 1e3 * min(ts[1:]),
+                    "sim_voxelize_ms": 1e3 * min(tsim[1:])}
         if cell == 0.1:
             t = time.perf_counter()
             s.voxelize()
             out[key]["host_ms"] = 1e3 * (time.perf_counter() - t)
-        del dst
+        sim.close()
         log(f"[bench] voxelize {key}: {out[key]}")
     return out
 
