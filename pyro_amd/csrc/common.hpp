@@ -190,6 +190,37 @@ __device__ __forceinline__ int xcd_tile(int bid, int nb) {
 }
 
 // Deterministic block reductions (fixed pairing order) for FP64 stats.
+// v[first] + v[first + step] + ... (count terms), added in that order from
+// 0 with eight loads in flight: the bits of the plain loop, without a load
+// round trip per term.
+__device__ __forceinline__ double seq_sum(const double *__restrict__ v, int count, size_t first, size_t step) {
+  double a = 0.0;
+  for (int k0 = 0; k0 < count; k0 += 8) {
+    double x[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) x[u] = k0 + u < count ? v[first + (size_t)(k0 + u) * step] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (k0 + u < count) a += x[u];
+  }
+  return a;
+}
+// The same for a running maximum from 0 (compare-select, in order).
+__device__ __forceinline__ float seq_max(const float *__restrict__ v, int count, size_t first, size_t step) {
+  float m = 0.0f;
+  for (int k0 = 0; k0 < count; k0 += 8) {
+    float x[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) x[u] = k0 + u < count ? v[first + (size_t)(k0 + u) * step] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (k0 + u < count) m = fmax_(m, x[u]);
+  }
+  return m;
+}
+// Terms of thread t in a loop over b = t, t + nth, ... < n.
+__device__ __forceinline__ int strided_count(int t, int n, int nth) { return t < n ? (n - 1 - t) / nth + 1 : 0; }
+
 template <int NT>
 __device__ __forceinline__ double block_sum(double v, double *sh) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

@@ -266,10 +266,7 @@ __global__ __launch_bounds__(1024) void k_stats_finalize(Geo g, StatParts st, in
   double *out = st.out;
   // layer sums 16+y: one thread per layer, blocks in order
   for (int y = t; y < 1024; y += 1024) {
-    double acc = 0.0;
-    if (y < g.ny)
-      for (int bx = 0; bx < st.react_gx; bx++) acc += st.react[((size_t)y * st.react_gx + bx) * 6 + 5];
-    out[16 + y] = acc;
+    out[16 + y] = y < g.ny ? seq_sum(st.react, st.react_gx, (size_t)y * st.react_gx * 6 + 5, 6) : 0.0;
   }
   if (t < 16) out[t] = 0.0;
   auto tree = [&](double v) -> double {
@@ -297,34 +294,25 @@ __global__ __launch_bounds__(1024) void k_stats_finalize(Geo g, StatParts st, in
   const int nreact = g.ny * st.react_gx;
   const int slot_r[5] = {0, 1, 3, 6, 8};
   for (int q = 0; q < 5; q++) {
-    double a = 0.0;
-    for (int b = t; b < nreact; b += 1024) a += st.react[(size_t)b * 6 + q];
-    double r = tree(a);
+    double r = tree(seq_sum(st.react, strided_count(t, nreact, 1024), (size_t)t * 6 + q, 6 * 1024));
     if (t == 0) out[slot_r[q]] = r;
   }
   {
-    float m = 0.0f;
-    for (int b = t; b < nreact; b += 1024) m = fmax_(m, st.react_t[b]);
-    float r = treemax(m);
+    float r = treemax(seq_max(st.react_t, strided_count(t, nreact, 1024), (size_t)t, 1024));
     if (t == 0) out[4] = (double)r;
   }
   const int slot_s[3] = {2, 7, 9};
   for (int q = 0; q < 3; q++) {
-    double a = 0.0;
-    for (int b = t; b < st.n_solid; b += 1024) a += st.solid[(size_t)b * 3 + q];
-    double r = tree(a);
+    double r = tree(seq_sum(st.solid, strided_count(t, st.n_solid, 1024), (size_t)t * 3 + q, 3 * 1024));
     if (t == 0) out[slot_s[q]] = r;
   }
   {
-    float m = 0.0f;
-    for (int b = t; b < st.n_curl; b += 1024) m = fmax_(m, st.curl[b]);
-    float r = treemax(m);
+    float r = treemax(seq_max(st.curl, strided_count(t, st.n_curl, 1024), (size_t)t, 1024));
     if (t == 0) out[5] = (double)r;
   }
   for (int q = 0; q < 2; q++) {
-    double a = 0.0;
-    if (with_resid)
-      for (int b = t; b < st.n_resid; b += 1024) a += st.resid[(size_t)b * 2 + q];
+    const double a = with_resid ? seq_sum(st.resid, strided_count(t, st.n_resid, 1024), (size_t)t * 2 + q, 2 * 1024)
+                                : 0.0;
     double r = tree(a);
     if (t == 0) out[10 + q] = r;
   }

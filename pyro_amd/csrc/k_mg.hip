@@ -231,23 +231,30 @@ void launch_mg_prolong(const MgLevel &C, const MgLevel &F, hipStream_t s) {
 //   wdir = 1 it is the same small integer, exactly.
 
 __global__ __launch_bounds__(NT) void k_mg_counts(LvArgs a, FastDiv d0, const uint8_t *__restrict__ f,
-                                                   uint8_t *__restrict__ nc) {
+                                                   uint8_t *__restrict__ nc, uint8_t *__restrict__ nm) {
   uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   if (i >= a.n) return;
-  if (f[i] == 1) { nc[i] = NC_SOLID; return; }
+  if (f[i] == 1) {
+    nc[i] = NC_SOLID;
+    if (nm) nm[i] = 0;
+    return;
+  }
   uint32_t r = fdiv(i, d0);
   int x = (int)(i - r * (uint32_t)a.d0);
   uint32_t zq = fdiv(r, a.dny);
   int y = (int)(r - zq * (uint32_t)a.d1), z = (int)zq;
   const uint32_t sy = (uint32_t)a.d0, sz = (uint32_t)a.d0 * (uint32_t)a.d1;
   int fl = 0, out = 0;
-  if (x + 1 < a.d0) fl += f[i + 1] != 1; else out++;
-  if (x - 1 >= 0) fl += f[i - 1] != 1; else out++;
-  if (y + 1 < a.d1) fl += f[i + sy] != 1; else out++;
-  if (y - 1 >= 0) fl += f[i - sy] != 1;  // ground: not counted
-  if (z + 1 < a.d2) fl += f[i + sz] != 1; else out++;
-  if (z - 1 >= 0) fl += f[i - sz] != 1; else out++;
+  uint32_t m = 0;
+  if (x + 1 < a.d0) { m |= (f[i + 1] != 1) ? 1u : 0u; } else out++;
+  if (x - 1 >= 0) { m |= (f[i - 1] != 1) ? 2u : 0u; } else out++;
+  if (y + 1 < a.d1) { m |= (f[i + sy] != 1) ? 4u : 0u; } else out++;
+  if (y - 1 >= 0) m |= (f[i - sy] != 1) ? 8u : 0u;  // ground: not counted
+  if (z + 1 < a.d2) { m |= (f[i + sz] != 1) ? 16u : 0u; } else out++;
+  if (z - 1 >= 0) { m |= (f[i - sz] != 1) ? 32u : 0u; } else out++;
+  fl = __builtin_popcount(m);
   nc[i] = (fl + out) == 0 ? NC_NONE : (uint8_t)(fl | (out << 3));
+  if (nm) nm[i] = (uint8_t)m;
 }
 
 // Coarse cell is solid iff every child inside the fine box is solid (mg.wgsl:74-82).
@@ -276,7 +283,8 @@ void launch_mg_setup(const MgLevel *L, int nl, hipStream_t s) {
       k_mg_coarse_flags<<<nblocks(C.n), NT, 0, s>>>(lv_args(F), F.f, C.d[0], C.d[1], C.d[2],
                                                    make_fastdiv((uint32_t)C.d[0]), make_fastdiv((uint32_t)C.d[1]), C.f);
     }
-    k_mg_counts<<<nblocks(L[l].n), NT, 0, s>>>(lv_args(L[l]), make_fastdiv((uint32_t)L[l].d[0]), L[l].f, L[l].nc);
+    k_mg_counts<<<nblocks(L[l].n), NT, 0, s>>>(lv_args(L[l]), make_fastdiv((uint32_t)L[l].d[0]), L[l].f, L[l].nc,
+                                               L[l].nm);
   }
 }
 

@@ -28,47 +28,52 @@ Op64 op64_of(const MgLevel &L) {
   return o;
 }
 
-__device__ __forceinline__ double apply_at(const Op64 &o, const uint8_t *__restrict__ f, const uint8_t *__restrict__ nc,
+// Neighbours from the level's 6-bit mask (nm: in the level and not solid),
+// summed in the order +x, -x, +y, -y, +z, -z.  All six are loaded at once
+// (a masked one at i itself, always in range) and a masked term adds +0,
+// which leaves the sum unchanged: a running sum started at +0 is never -0.
+__device__ __forceinline__ double apply_at(const Op64 &o, const uint8_t *__restrict__ nm, const uint8_t *__restrict__ nc,
                                            const double *__restrict__ x, uint32_t i) {
   const uint8_t code = nc[i];
   if (code & 0x80) return 0.0;
-  uint32_t r = fdiv(i, o.dd0);
-  const int xx = (int)(i - r * (uint32_t)o.d0);
-  const uint32_t zq = fdiv(r, o.dd1);
-  const int yy = (int)(r - zq * (uint32_t)o.d1), zz = (int)zq;
+  const uint32_t m = nm[i];
   const uint32_t sy = (uint32_t)o.d0, sz = (uint32_t)o.d0 * (uint32_t)o.d1;
+  const double xc = x[i];
+  const double v0 = x[(m & 1u) ? i + 1 : i], v1 = x[(m & 2u) ? i - 1 : i];
+  const double v2 = x[(m & 4u) ? i + sy : i], v3 = x[(m & 8u) ? i - sy : i];
+  const double v4 = x[(m & 16u) ? i + sz : i], v5 = x[(m & 32u) ? i - sz : i];
   double sum = 0.0;
-  if (xx + 1 < o.d0 && f[i + 1] != 1) sum += x[i + 1];
-  if (xx - 1 >= 0 && f[i - 1] != 1) sum += x[i - 1];
-  if (yy + 1 < o.d1 && f[i + sy] != 1) sum += x[i + sy];
-  if (yy - 1 >= 0 && f[i - sy] != 1) sum += x[i - sy];
-  if (zz + 1 < o.d2 && f[i + sz] != 1) sum += x[i + sz];
-  if (zz - 1 >= 0 && f[i - sz] != 1) sum += x[i - sz];
+  sum += (m & 1u) ? v0 : 0.0;
+  sum += (m & 2u) ? v1 : 0.0;
+  sum += (m & 4u) ? v2 : 0.0;
+  sum += (m & 8u) ? v3 : 0.0;
+  sum += (m & 16u) ? v4 : 0.0;
+  sum += (m & 32u) ? v5 : 0.0;
   const double cnt = (double)((code & 7u) + ((code >> 3) & 7u));
-  return (sum - cnt * x[i]) * o.ih2;
+  return (sum - cnt * xc) * o.ih2;
 }
 
 // r = b - A x  (fluid cells; 0 on solids)
-__global__ __launch_bounds__(NT) void k_resid64(Op64 o, const uint8_t *__restrict__ f, const uint8_t *__restrict__ nc,
+__global__ __launch_bounds__(NT) void k_resid64(Op64 o, const uint8_t *__restrict__ nm, const uint8_t *__restrict__ nc,
                                                  const double *__restrict__ x, const double *__restrict__ b,
                                                  double *__restrict__ r) {
   uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   if (i >= o.n) return;
-  r[i] = (nc[i] & 0x80) ? 0.0 : b[i] - apply_at(o, f, nc, x, i);
+  r[i] = (nc[i] & 0x80) ? 0.0 : b[i] - apply_at(o, nm, nc, x, i);
 }
 // y = A x
-__global__ __launch_bounds__(NT) void k_apply64(Op64 o, const uint8_t *__restrict__ f, const uint8_t *__restrict__ nc,
+__global__ __launch_bounds__(NT) void k_apply64(Op64 o, const uint8_t *__restrict__ nm, const uint8_t *__restrict__ nc,
                                                  const double *__restrict__ x, double *__restrict__ y) {
   uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   if (i >= o.n) return;
-  y[i] = apply_at(o, f, nc, x, i);
+  y[i] = apply_at(o, nm, nc, x, i);
 }
 
 void launch_resid64(const MgLevel &L, const double *x, const double *b, double *r, hipStream_t s) {
-  k_resid64<<<nblocks(L.n), NT, 0, s>>>(op64_of(L), L.f, L.nc, x, b, r);
+  k_resid64<<<nblocks(L.n), NT, 0, s>>>(op64_of(L), L.nm, L.nc, x, b, r);
 }
 void launch_apply64(const MgLevel &L, const double *x, double *y, hipStream_t s) {
-  k_apply64<<<nblocks(L.n), NT, 0, s>>>(op64_of(L), L.f, L.nc, x, y);
+  k_apply64<<<nblocks(L.n), NT, 0, s>>>(op64_of(L), L.nm, L.nc, x, y);
 }
 
 // Up to 4 dot products at once: out[k] = sum_i a_k[i] * b_k[i].
@@ -183,9 +188,8 @@ __global__ __launch_bounds__(1024) void k_sum_final(const double *__restrict__ p
                                                     int slot) {
   __shared__ double sh[1024];
   for (int q = 0; q < k; q++) {
-    double a = 0.0;
-    for (int i = threadIdx.x; i < nb; i += 1024) a += part[(size_t)q * nb + i];
-    sh[threadIdx.x] = a;
+    const int t = (int)threadIdx.x;
+    sh[t] = seq_sum(part, strided_count(t, nb, 1024), (size_t)q * nb + t, 1024);
     __syncthreads();
     for (int s = 512; s > 0; s >>= 1) {
       if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
@@ -197,13 +201,13 @@ __global__ __launch_bounds__(1024) void k_sum_final(const double *__restrict__ p
 }
 
 // q = A p with partials of p.q and p.r
-__global__ __launch_bounds__(NT) void k_apply_dots(Op64 o, const uint8_t *__restrict__ f, const uint8_t *__restrict__ nc,
+__global__ __launch_bounds__(NT) void k_apply_dots(Op64 o, const uint8_t *__restrict__ nm, const uint8_t *__restrict__ nc,
                                                     const double *__restrict__ p, const double *__restrict__ r,
                                                     double *__restrict__ q, double *__restrict__ part) {
   const uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   double pq = 0.0, pr = 0.0;
   if (i < o.n) {
-    const double qi = apply_at(o, f, nc, p, i);
+    const double qi = apply_at(o, nm, nc, p, i);
     q[i] = qi;
     pq = p[i] * qi;
     pr = p[i] * r[i];
@@ -267,13 +271,13 @@ __global__ __launch_bounds__(NT) void k_xpby_dev(uint32_t n, const double *__res
 }
 
 // r = b - A x with partials of r.r
-__global__ __launch_bounds__(NT) void k_resid_rr(Op64 o, const uint8_t *__restrict__ f, const uint8_t *__restrict__ nc,
+__global__ __launch_bounds__(NT) void k_resid_rr(Op64 o, const uint8_t *__restrict__ nm, const uint8_t *__restrict__ nc,
                                                   const double *__restrict__ x, const double *__restrict__ b,
                                                   double *__restrict__ r, double *__restrict__ part) {
   const uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   double rr = 0.0;
   if (i < o.n) {
-    const double ri = (nc[i] & 0x80) ? 0.0 : b[i] - apply_at(o, f, nc, x, i);
+    const double ri = (nc[i] & 0x80) ? 0.0 : b[i] - apply_at(o, nm, nc, x, i);
     r[i] = ri;
     rr = ri * ri;
   }
@@ -286,13 +290,13 @@ uint32_t cg_part_blocks(uint32_t n) { return nblocks(n); }
 void launch_resid_rr(const MgLevel &L, const double *x, const double *b, double *r, double *part, double *dev,
                      hipStream_t s) {
   const unsigned nb = nblocks(L.n);
-  k_resid_rr<<<nb, NT, 0, s>>>(op64_of(L), L.f, L.nc, x, b, r, part);
+  k_resid_rr<<<nb, NT, 0, s>>>(op64_of(L), L.nm, L.nc, x, b, r, part);
   k_sum_final<<<1, 1024, 0, s>>>(part, (int)nb, 1, dev, 2);
 }
 void launch_apply_dots(const MgLevel &L, const double *p, const double *r, double *q, double *part, double *dev,
                        hipStream_t s) {
   const unsigned nb = nblocks(L.n);
-  k_apply_dots<<<nb, NT, 0, s>>>(op64_of(L), L.f, L.nc, p, r, q, part);
+  k_apply_dots<<<nb, NT, 0, s>>>(op64_of(L), L.nm, L.nc, p, r, q, part);
   k_sum_final<<<1, 1024, 0, s>>>(part, (int)nb, 2, dev, 0);
 }
 void launch_update_rr(uint32_t n, const double *dev, const double *p, const double *q, double *x, double *r,

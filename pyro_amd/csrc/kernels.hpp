@@ -62,6 +62,11 @@ struct MgLevel {
   float *tmp;    // second phi buffer of the ping-pong sweeps
   uint8_t *f;    // 1 = solid
   uint8_t *nc;   // neighbour codes (k_mg.hip setup pass)
+  // fluid neighbours as a 6-bit mask (+x, -x, +y, -y, +z, -z; bit set when
+  // the neighbour is inside the level and not solid), for the FP64 operator
+  // of the pressure solvers; set up with the codes where allocated (level 0
+  // of a Poisson handle), null elsewhere
+  uint8_t *nm = nullptr;
   uint32_t n;
 };
 // Coarse flags and neighbour codes of every level (once per step).

@@ -265,6 +265,7 @@ int pyro_poisson_create(uint32_t nx, uint32_t ny, uint32_t nz, float h, const ui
     if (!rc) rc |= palloc(P, &L.tmp, L.n);
     if (!rc) rc |= palloc(P, &L.f, L.n);
     if (!rc) rc |= palloc(P, &L.nc, L.n);
+    if (!rc && l == 0) rc |= palloc(P, &L.nm, L.n);  // the FP64 operator's neighbour mask
     // reference hierarchy: the first levels of the same halving, a level is
     // kept and the halving stops after 4 levels or once min(dims) < 16
     // (sim.rs:146-168).  It never outlasts `pc`, which stops only below
