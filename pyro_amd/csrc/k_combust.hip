@@ -267,22 +267,27 @@ __global__ __launch_bounds__(NT) void k_react(const Params *__restrict__ P, Geo 
       float nut = (P->cs * dx) * (P->cs * dx) * curl[i].w;
       if (P->cs > 0.0f) {
         float kd = fmin_(nut / 0.5f * dt / (dx * dx), 0.15f);
+        // every neighbour is loaded (one outside the grid at the cell itself,
+        // always in range) so the loads are in flight together; a skipped
+        // neighbour (solid, or below the ground) adds +0, which leaves these
+        // sums unchanged: a running sum started at +0 is never -0
         float lx = 0.0f, ly = 0.0f, lz = 0.0f, lw = 0.0f, lb = 0.0f;
+#pragma unroll
         for (int k = 0; k < 6; k++) {
           int ddx, ddy, ddz;
           dirOf(k, ddx, ddy, ddz);
-          int nx_ = x + ddx, ny_ = y + ddy, nz_ = z + ddz;
-          if (inb(g, nx_, ny_, nz_)) {
-            uint32_t j = sidx(g, nx_, ny_, nz_);
-            if (flags[j] != 1) {
-              float4 q = s0in[j];
-              lx += q.x - a.x; ly += q.y - a.y; lz += q.z - a.z; lw += q.w - a.w;
-              lb += s1in[j].x - b.x;
-            }
-          } else if (ny_ >= 0) {
-            lx += P->t0 - a.x; ly += 0.0f - a.y; lz += P->yo2 - a.z; lw += 0.0f - a.w;
-            lb -= b.x;
-          }
+          const int nx_ = x + ddx, ny_ = y + ddy, nz_ = z + ddz;
+          const bool in = inb(g, nx_, ny_, nz_);
+          const uint32_t j = in ? sidx(g, nx_, ny_, nz_) : i;
+          const float4 q = s0in[j];
+          const float qb = s1in[j].x;
+          const bool fluid = in && flags[j] != 1;
+          const bool amb = !in && ny_ >= 0;  // open side or top: ambient air
+          lx += fluid ? q.x - a.x : (amb ? P->t0 - a.x : 0.0f);
+          ly += fluid ? q.y - a.y : (amb ? 0.0f - a.y : 0.0f);
+          lz += fluid ? q.z - a.z : (amb ? P->yo2 - a.z : 0.0f);
+          lw += fluid ? q.w - a.w : (amb ? 0.0f - a.w : 0.0f);
+          lb += fluid ? qb - b.x : (amb ? -b.x : 0.0f);
         }
         a.x += kd * lx; a.y += kd * ly; a.z += kd * lz; a.w += kd * lw;
         b.x += kd * lb;

@@ -120,6 +120,10 @@ __global__ __launch_bounds__(NT) void k_forces(const Params *__restrict__ P, Geo
   int x, y, z;
   decode_face(g, i, x, y, z);
   const float dt = P->dt;
+  // the face's own cell (x, y, z) is the b side of all three components:
+  // its confinement force is evaluated once
+  float cbx = 0.0f, cby = 0.0f, cbz = 0.0f;
+  if (inb(g, x, y, z)) confine(P, g, s0, s1, curl, x, y, z, cbx, cby, cbz);
   for (int k = 0; k < 3; k++) {
     int hx = g.nx - 1 + (k == 0), hy = g.ny - 1 + (k == 1), hz = g.nz - 1 + (k == 2);
     if (x > hx || y > hy || z > hz) continue;
@@ -143,9 +147,7 @@ __global__ __launch_bounds__(NT) void k_forces(const Params *__restrict__ P, Geo
       fcx += cx; fcy += cy; fcz += cz; n += 1.0f;
     }
     if (ib) {
-      float cx, cy, cz;
-      confine(P, g, s0, s1, curl, x, y, z, cx, cy, cz);
-      fcx += cx; fcy += cy; fcz += cz; n += 1.0f;
+      fcx += cbx; fcy += cby; fcz += cbz; n += 1.0f;
     }
     if (n > 0.0f) f += (k == 0 ? fcx : (k == 1 ? fcy : fcz)) / n;
     float *C = k == 0 ? U : (k == 1 ? V : W);
