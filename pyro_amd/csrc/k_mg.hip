@@ -113,23 +113,36 @@ __global__ __launch_bounds__(NT) void k_mg_restrict(LvArgs F, int ntx, int nty, 
   const int tx = bid % ntx, rr = bid / ntx, ty = rr % nty, tz = rr / nty;
   const int fx0 = 2 * RS_TX * tx - 1, fy0 = 2 * RS_TY * ty - 1, fz0 = 2 * RS_TZ * tz - 1;  // footprint origin
   const int sy = F.d0, sz = F.d0 * F.d1;
-  for (int s = threadIdx.x; s < RS_FN; s += NT) {
-    const int lx = s % RS_FX, r2 = s / RS_FX, ly = r2 % RS_FY, lz = r2 / RS_FY;
+  // staging: every load is unconditional (outside the level at element 0)
+  // and selected afterwards, so all of a thread's loads are in flight at once
+#pragma unroll
+  for (int q = 0; q < (RS_FN + NT - 1) / NT; q++) {
+    const int s = threadIdx.x + q * NT;
+    const int se = s < RS_FN ? s : RS_FN - 1;
+    const int lx = se % RS_FX, r2 = se / RS_FX, ly = r2 % RS_FY, lz = r2 / RS_FY;
     const int gx = fx0 + lx, gy = fy0 + ly, gz = fz0 + lz;
-    float v = 0.0f;
-    uint8_t c = NC_NONE;
-    if (gx >= 0 && gx < F.d0 && gy >= 0 && gy < F.d1 && gz >= 0 && gz < F.d2) {
-      const int g = gx + sy * gy + sz * gz;
-      c = ncf[g];
-      if (!(c & NC_SOLID)) v = pf[g];
+    const bool in = gx >= 0 && gx < F.d0 && gy >= 0 && gy < F.d1 && gz >= 0 && gz < F.d2;
+    const int g = in ? gx + sy * gy + sz * gz : 0;
+    const uint8_t c = ncf[g];
+    const float p = pf[g];
+    if (s < RS_FN) {
+      sp[s] = (in && !(c & NC_SOLID)) ? p : 0.0f;
+      sc[s] = in ? c : NC_NONE;
     }
-    sp[s] = v;
-    sc[s] = c;
   }
   __syncthreads();
   const int lcx = threadIdx.x % RS_TX, lr = threadIdx.x / RS_TX, lcy = lr % RS_TY, lcz = lr / RS_TY;
   const int x = RS_TX * tx + lcx, y = RS_TY * ty + lcy, z = RS_TZ * tz + lcz;
   if (x >= c0 || y >= c1 || z >= c2) return;
+  // the eight children's h^2 b, loaded together (a child outside the level
+  // reads element 0 and is skipped)
+  float bv[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const int a = 2 * x + (k & 1), bb = 2 * y + ((k >> 1) & 1), c = 2 * z + ((k >> 2) & 1);
+    const bool in = a < F.d0 && bb < F.d1 && c < F.d2;
+    bv[k] = bf[in ? a + sy * bb + sz * c : 0];
+  }
   float acc = 0.0f, nfl = 0.0f;
 #pragma unroll
   for (int k = 0; k < 8; k++) {
@@ -147,7 +160,7 @@ __global__ __launch_bounds__(NT) void k_mg_restrict(LvArgs F, int ntx, int nty, 
     sum += sp[pi + RS_FX * RS_FY];
     sum += sp[pi - RS_FX * RS_FY];
     const float cnt = nc_count(code, F.wdir);
-    acc += bf[a + sy * bb + sz * c] - (sum - cnt * sp[pi]) / F.hh;
+    acc += bv[k] - (sum - cnt * sp[pi]) / F.hh;
   }
   const uint32_t i = (uint32_t)x + (uint32_t)c0 * ((uint32_t)y + (uint32_t)c1 * (uint32_t)z);
   bc[i] = nfl > 0.0f ? acc / fmax_(nfl, 1.0f) : 0.0f;
@@ -191,13 +204,26 @@ __global__ __launch_bounds__(NT) void k_mg_prolong(int f0, int f1, int f2, int n
   const float px = ((float)x + 0.5f) * 0.5f - 0.5f;
   const int i0x = (int)floorf(px);
   const float frx = px - (float)i0x;
+  // the thread's cells' flags and phi are loaded together first (a cell
+  // outside the level reads element 0 and is skipped)
+  constexpr int NQ = (PL_TY * PL_TZ) / (NT / PL_TX);
+  uint8_t fv[NQ];
+  float pv[NQ];
 #pragma unroll
-  for (int q = 0; q < (PL_TY * PL_TZ) / (NT / PL_TX); q++) {
+  for (int q = 0; q < NQ; q++) {
+    const int slot = lyz + q * (NT / PL_TX);
+    const int y = y0 + slot % PL_TY, z = z0 + slot / PL_TY;
+    const uint32_t i = (y < f1 && z < f2) ? (uint32_t)x + (uint32_t)f0 * ((uint32_t)y + (uint32_t)f1 * (uint32_t)z) : 0u;
+    fv[q] = ff[i];
+    pv[q] = pf[i];
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
     const int slot = lyz + q * (NT / PL_TX);
     const int y = y0 + slot % PL_TY, z = z0 + slot / PL_TY;
     if (y >= f1 || z >= f2) continue;
     const uint32_t i = (uint32_t)x + (uint32_t)f0 * ((uint32_t)y + (uint32_t)f1 * (uint32_t)z);
-    if (ff[i] == 1) continue;
+    if (fv[q] == 1) continue;
     const float py = ((float)y + 0.5f) * 0.5f - 0.5f, pz = ((float)z + 0.5f) * 0.5f - 0.5f;
     const int i0y = (int)floorf(py), i0z = (int)floorf(pz);
     const float fry = py - (float)i0y, frz = pz - (float)i0z;
@@ -212,7 +238,7 @@ __global__ __launch_bounds__(NT) void k_mg_prolong(int f0, int f1, int f2, int n
       acc += wgt * sp[si];
       ws += wgt;
     }
-    if (ws > 1e-4f) pf[i] += acc / ws;
+    if (ws > 1e-4f) pf[i] = pv[q] + acc / ws;
   }
 }
 void launch_mg_prolong(const MgLevel &C, const MgLevel &F, hipStream_t s) {
