@@ -182,12 +182,16 @@ __global__ __launch_bounds__(NT) void k_divergence(const Params *__restrict__ P,
                                                     const uint8_t *__restrict__ fl, float *__restrict__ b) {
   uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   if (i >= g.n) return;
-  if (fl[i] == 1) { b[i] = 0.0f; return; }
   int x, y, z;
   decode_cell(g, i, x, y, z);
   uint32_t j = vidx(g, x, y, z);
-  float d = (U[j + 1] - U[j] + V[j + g.vx] - V[j] + W[j + (uint32_t)g.vx * (uint32_t)g.vy] - W[j]) / P->dx;
-  b[i] = d - s1[i].z;
+  // the faces are loaded with the flag (not behind it): one round trip
+  const uint8_t f = fl[i];
+  const float u1 = U[j + 1], u0 = U[j], v1 = V[j + g.vx], v0 = V[j];
+  const float w1 = W[j + (uint32_t)g.vx * (uint32_t)g.vy], w0 = W[j], sz = s1[i].z;
+  if (f == 1) { b[i] = 0.0f; return; }
+  float d = (u1 - u0 + v1 - v0 + w1 - w0) / P->dx;
+  b[i] = d - sz;
 }
 
 void launch_divergence(const Fields &f, hipStream_t s) {
@@ -203,18 +207,35 @@ __global__ __launch_bounds__(NT) void k_project(const Params *__restrict__ P, Ge
   int x, y, z;
   decode_face(g, i, x, y, z);
   const float dx = P->dx;
-  bool ib = inb(g, x, y, z);
-  float pc = ib ? phi[sidx(g, x, y, z)] : 0.0f;
-  bool sc = ib && fl[sidx(g, x, y, z)] == 1;
+  // every load is issued first (a cell outside the grid reads cell 0 and is
+  // selected away), then the three components are formed
+  const bool ib = inb(g, x, y, z);
+  const uint32_t ic = ib ? sidx(g, x, y, z) : 0u;
+  const float pcl = phi[ic];
+  const uint8_t fcl = fl[ic];
+  float cv[3], pav[3];
+  uint8_t fav[3];
+  bool ia[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const int ax = x - (k == 0), ay = y - (k == 1), az = z - (k == 2);
+    ia[k] = inb(g, ax, ay, az);
+    const uint32_t j = ia[k] ? sidx(g, ax, ay, az) : 0u;
+    pav[k] = phi[j];
+    fav[k] = fl[j];
+    cv[k] = (k == 0 ? U : (k == 1 ? V : W))[i];
+  }
+  const float pc = ib ? pcl : 0.0f;
+  const bool sc = ib && fcl == 1;
+#pragma unroll
   for (int k = 0; k < 3; k++) {
     int hx = g.nx - 1 + (k == 0), hy = g.ny - 1 + (k == 1), hz = g.nz - 1 + (k == 2);
     if (x > hx || y > hy || z > hz) continue;
-    int ax = x - (k == 0), ay = y - (k == 1), az = z - (k == 2);
     float *C = k == 0 ? U : (k == 1 ? V : W);
     float u = 0.0f;
-    if (!((k == 1 && y == 0) || sc || solidAt(g, fl, ax, ay, az))) {
-      u = C[i];
-      float pa = inb(g, ax, ay, az) ? phi[sidx(g, ax, ay, az)] : 0.0f;
+    if (!((k == 1 && y == 0) || sc || (ia[k] && fav[k] == 1))) {
+      u = cv[k];
+      const float pa = ia[k] ? pav[k] : 0.0f;
       u -= (pc - pa) / dx;
     }
     C[i] = u;
