@@ -450,12 +450,33 @@ __global__ __launch_bounds__(SW_NT) void k_mg_sweep(LvArgs a, int ntx, int nty, 
       sc[i] = cv;
     }
   }
-  __syncthreads();
   // Colour of (gx, gy, gz) = (gx + gy + gz) & 1 with gx = x0 + lx etc.; x0 is
   // a multiple of 4, so the colour-c cell of an x-pair starting at an odd
   // lx is lx + ((K + ly + lz + c) & 1) with K = y0 + z0 + 1 (ly, lz counted
   // from the box the pass walks).
   const int K = y0 + z0 + 1;
+  // h^2 b of the six cells this thread updates (maps below), loaded while
+  // the tile is staged rather than inside the colour passes
+  float hb0[4], hb1[2];
+  {
+    const int px = t % 17, r0 = t / 17;
+#pragma unroll
+    for (int it = 0; it < 4; it++) {
+      const int r = r0 + 15 * it, lz = r / SW_BY, ly = r - SW_BY * lz;
+      const int lx = 3 + 2 * px + ((K + ly + lz) & 1);
+      hb0[it] = sweep_hb<VEC>(a, rbb, b, x0 + lx, y0 + 1 + ly, z0 + 1 + lz);
+    }
+  }
+  {
+    const int px = t % 16, r0 = t / 16;
+#pragma unroll
+    for (int it = 0; it < 2; it++) {
+      const int r = r0 + 16 * it, ly = r % SW_TY, lz = r / SW_TY;
+      const int lx = 4 + 2 * px + ((K + ly + lz) & 1);
+      hb1[it] = sweep_hb<VEC>(a, rbb, b, x0 + lx, y0 + 2 + ly, z0 + 2 + lz);
+    }
+  }
+  __syncthreads();
   // colour 0 on the tile + ring 1: x-pairs from lx = 3 (17 of them), rows
   // (ly, lz) of the 10 x 6 ring-1 box (LDS offset 1 in y and z); thread
   // t < 255 takes x-pair t % 17 of rows r0 + 15 it, r0 = t / 17, it = 0..3
@@ -467,8 +488,7 @@ __global__ __launch_bounds__(SW_NT) void k_mg_sweep(LvArgs a, int ntx, int nty, 
       const int r = r0 + 15 * it, lz = r / SW_BY, ly = r - SW_BY * lz;
       const int lx = 3 + 2 * px + ((K + ly + lz) & 1);
       const int pi = ((lz + 1) * SW_PY + (ly + 1)) * SW_PX + lx;
-      sweep_cell<ICNT>(sp, sc, sweep_hb<VEC>(a, rbb, b, x0 + lx, y0 + 1 + ly, z0 + 1 + lz), ct, pi, omega,
-                       a.wdir);
+      sweep_cell<ICNT>(sp, sc, hb0[it], ct, pi, omega, a.wdir);
     }
   }
   __syncthreads();
@@ -482,8 +502,7 @@ __global__ __launch_bounds__(SW_NT) void k_mg_sweep(LvArgs a, int ntx, int nty, 
       const int r = r0 + 16 * it, ly = r % SW_TY, lz = r / SW_TY;
       const int lx = 4 + 2 * px + ((K + ly + lz) & 1);
       const int pi = ((lz + 2) * SW_PY + (ly + 2)) * SW_PX + lx;
-      sweep_cell<ICNT>(sp, sc, sweep_hb<VEC>(a, rbb, b, x0 + lx, y0 + 2 + ly, z0 + 2 + lz), ct, pi, omega,
-                       a.wdir);
+      sweep_cell<ICNT>(sp, sc, hb1[it], ct, pi, omega, a.wdir);
     }
   }
   __syncthreads();
