@@ -436,18 +436,22 @@ __global__ __launch_bounds__(SW_NT) void k_mg_sweep(LvArgs a, int ntx, int nty, 
       }
     }
   } else {
-    for (int i = t; i < SW_PN; i += SW_NT) {
-      const int lx = i % SW_PX, row = i / SW_PX, ly = row % SW_PY, lz = row / SW_PY;
+    // every load unconditional (outside the level at element 0) and
+    // selected, so a thread's loads are all in flight together
+#pragma unroll
+    for (int q = 0; q < (SW_PN + SW_NT - 1) / SW_NT; q++) {
+      const int i = t + q * SW_NT;
+      const int ie = i < SW_PN ? i : SW_PN - 1;
+      const int lx = ie % SW_PX, row = ie / SW_PX, ly = row % SW_PY, lz = row / SW_PY;
       const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
-      float pv = 0.0f;
-      uint8_t cv = NC_NONE;
-      if (gx >= 0 && gx < a.d0 && gy >= 0 && gy < a.d1 && gz >= 0 && gz < a.d2) {
-        const int g = gx + sy * gy + sz * gz;
-        cv = nc[g];
-        pv = (cv & NC_SOLID) ? 0.0f : pin[g];
+      const bool in = gx >= 0 && gx < a.d0 && gy >= 0 && gy < a.d1 && gz >= 0 && gz < a.d2;
+      const int g = in ? gx + sy * gy + sz * gz : 0;
+      const uint8_t c = nc[g];
+      const float p = pin[g];
+      if (i < SW_PN) {
+        sp[i] = (in && !(c & NC_SOLID)) ? p : 0.0f;
+        sc[i] = in ? c : NC_NONE;
       }
-      sp[i] = pv;
-      sc[i] = cv;
     }
   }
   // Colour of (gx, gy, gz) = (gx + gy + gz) & 1 with gx = x0 + lx etc.; x0 is
@@ -579,7 +583,8 @@ __global__ __launch_bounds__(CO_NT) void k_mg_coarse(LvArgs a, const float *__re
   const int P0 = a.d0 + 2, P1 = a.d1 + 2, P2 = a.d2 + 2;
   const int PN = P0 * P1 * P2;
   const uint32_t sy = (uint32_t)a.d0, sz = (uint32_t)a.d0 * (uint32_t)a.d1;
-  for (int i = threadIdx.x; i < PN; i += CO_NT) {
+  const int nt = (int)blockDim.x;  // whole waves covering a colour's cells
+  for (int i = threadIdx.x; i < PN; i += nt) {
     int lx = i % P0, t = i / P0, ly = t % P1, lz = t / P1;
     int gx = lx - 1, gy = ly - 1, gz = lz - 1;
     float v = 0.0f;
@@ -589,16 +594,19 @@ __global__ __launch_bounds__(CO_NT) void k_mg_coarse(LvArgs a, const float *__re
     }
     sp[i] = v;
   }
-  // per owned cell: LDS index (bits 0-15) | fluid (16-18) | outside (19-21) | active (31)
+  // per owned cell: LDS index (bits 0-15) | fluid (16-18) | outside (19-21) | active (31);
+  // an inactive slot points at grid cell (0, 0, 0), whose six neighbours
+  // are inside the padded array, and is never stored
+  const uint32_t idle = (uint32_t)(1 + P0 + P0 * P1);
   uint32_t pk[2][CO_K];
   float hb[2][CO_K];
 #pragma unroll
   for (int c = 0; c < 2; c++) {
 #pragma unroll
     for (int j = 0; j < CO_K; j++) {
-      pk[c][j] = 0;
+      pk[c][j] = idle;
       hb[c][j] = 0.0f;
-      uint32_t t = threadIdx.x + (uint32_t)j * CO_NT;
+      uint32_t t = threadIdx.x + (uint32_t)j * (uint32_t)nt;
       if (t >= a.nhalf) continue;
       uint32_t r = fdiv(t, a.dhx);
       int gxh = (int)(t - r * a.dhx.d);
@@ -621,10 +629,11 @@ __global__ __launch_bounds__(CO_NT) void k_mg_coarse(LvArgs a, const float *__re
     for (int c = 0; c < 2; c++) {
 #pragma unroll
       for (int j0 = 0; j0 < CO_K; j0 += CO_C) {
+        if ((uint32_t)j0 * (uint32_t)nt >= a.nhalf) break;  // no thread owns a cell here (uniform)
         float nb[CO_C][7];
 #pragma unroll
         for (int q = 0; q < CO_C; q++) {
-          const int i = (int)(pk[c][j0 + q] & 0xffffu);  // inactive cells read ghost cell 0
+          const int i = (int)(pk[c][j0 + q] & 0xffffu);
           nb[q][0] = sp[i + o1]; nb[q][1] = sp[i - o1]; nb[q][2] = sp[i + o2];
           nb[q][3] = sp[i - o2]; nb[q][4] = sp[i + o3]; nb[q][5] = sp[i - o3];
           nb[q][6] = sp[i];
@@ -647,7 +656,7 @@ __global__ __launch_bounds__(CO_NT) void k_mg_coarse(LvArgs a, const float *__re
       __syncthreads();
     }
   }
-  for (int i = threadIdx.x; i < PN; i += CO_NT) {
+  for (int i = threadIdx.x; i < PN; i += nt) {
     int lx = i % P0, t = i / P0, ly = t % P1, lz = t / P1;
     int gx = lx - 1, gy = ly - 1, gz = lz - 1;
     if (gx >= 0 && gy >= 0 && gz >= 0 && gx < a.d0 && gy < a.d1 && gz < a.d2) {
@@ -871,8 +880,12 @@ void launch_mg_coarse(const MgLevel &L, int iters, float omega, hipStream_t s) {
     k_mg_coarse_z<22><<<1, colt, 0, s>>>(a, L.b, L.f, L.nc, L.phi, iters, omega);
     return;
   }
-  if (mg_coarse_fits_lds(L))
-    k_mg_coarse<<<1, CO_NT, 0, s>>>(a, L.b, L.f, L.nc, L.phi, iters, omega);
+  if (mg_coarse_fits_lds(L)) {
+    // one cell of a colour per thread where that fits, whole waves only
+    // (the level fits: nhalf <= CO_NT * CO_K)
+    const uint32_t nt = a.nhalf >= (uint32_t)CO_NT ? (uint32_t)CO_NT : ((a.nhalf + 63u) / 64u) * 64u;
+    k_mg_coarse<<<1, nt, 0, s>>>(a, L.b, L.f, L.nc, L.phi, iters, omega);
+  }
   else
     k_mg_coarse_global<<<1, 1024, 0, s>>>(a, L.b, L.f, L.phi, iters, omega);
 }
