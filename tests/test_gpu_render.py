@@ -94,6 +94,31 @@ def test_async_handoff_overlaps_steps():
         buf_b.close()
 
 
+def test_pack_on_odd_dims():
+    """Grid dims that are not multiples of 16 or 4 (partial blocks, the
+    velocity faces of the last row and plane)."""
+    import oracle
+
+    sc = mini_scene()
+    p = sc.params()
+    nx, ny, nz = 22, 38, 18
+    p["nx"], p["ny"], p["nz"] = nx, ny, nz
+    sol = np.zeros((nz, ny, nx, 4), np.float32)
+    sol[..., 3] = 293.0
+    sol[4:9, 0:3, 5:12] = (1.0, 700.0, 0.0, 360.0)
+    sol[10:14, 6:10, 14:18] = (2.0, 480.0, 20.0, 650.0)
+    o = oracle.OSim(p, sol, None, 4, 4.0)
+    g = pa.Sim(p, sol, None, 4, 4.0)
+    for sim in (o, g):
+        sim.add_velocity_noise(3.0, 5)
+    for k in range(2):
+        o.step(0.002, False)
+        g.step(0.002, False)
+    pack = g.render_pack()
+    assert pack.shape == (3, nz, ny, nx, 4)
+    _assert_pack_equal(o.render_pack(), pack)
+
+
 def test_pack_rejects_wrong_size():
     g = pa.Sim.new(mini_scene())
     n = g.dims[0] * g.dims[1] * g.dims[2]

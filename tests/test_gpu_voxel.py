@@ -117,6 +117,21 @@ def test_sim_device_voxelize_steps_identically():
     np.testing.assert_array_equal(a.read_stats().raw, b.read_stats().raw)
 
 
+def test_scene_without_objects():
+    """No shapes: SOL = (0, 0, 0, T0) everywhere on both paths, and a
+    simulation created without SOL starts the same way."""
+    sc = pa.Scene.parse("[domain]\nsize = [0.8, 0.8, 0.8]\ncell = 0.05\n[sim]\nambient_temp = 288.0\n"
+                        "[fuel]\nmaterial = \"heptane\"\n")
+    assert sc.shapes().size == 0
+    host = sc.voxelize()
+    dev = sc.voxelize_device()
+    assert np.array_equal(host.view(np.uint32), dev.view(np.uint32))
+    assert (host[..., 3] == 288.0).all() and not host[..., :3].any()
+    sim = pa.Sim.new(sc, device_voxelize=True)
+    assert np.array_equal(sim.read_field("sol").view(np.uint32), host.view(np.uint32))
+    assert not sim.read_field("flags").any()
+
+
 def test_sim_voxelize_rejects_other_grid():
     sc = mini_scene()
     sim = pa.Sim.new(sc)
