@@ -35,6 +35,15 @@ __device__ __forceinline__ float4 ld4(const RB &b, uint32_t idx) {
 struct VelR {
   RB u, v, w;
 };
+
+// Sample coordinates clamped to [0, hi] (tmpl_comp.wgsl:5, tmpl_scalar.wgsl:5
+// clamp them with compare-select).  A coordinate here is a non-negative grid
+// position minus a trace displacement and staggering offsets, never -0, so
+// fmaxf / fminf (one instruction each) give the same value as the
+// compare-select pair: the only inputs on which they differ are +-0 ties
+// with the 0 bound, and for a NaN coordinate both return the bound.  After
+// the clamp the coordinate is >= 0, so truncation to int is its floor.
+__device__ __forceinline__ float pclamp(float x, float hi) { return __builtin_fminf(__builtin_fmaxf(x, 0.0f), hi); }
 __device__ __forceinline__ VelR velr_of(const Geo &g, const VelPtrs &V) {
   const uint32_t bytes = g.nv * 4u;
   return VelR{rb_of(V.u, bytes), rb_of(V.v, bytes), rb_of(V.w, bytes)};
@@ -44,8 +53,8 @@ __device__ __forceinline__ VelR velr_of(const Geo &g, const VelPtrs &V) {
 // of g (tmpl_comp.wgsl:2-15); hi = largest valid node index.
 __device__ __forceinline__ float triC(const Geo &g, const RB &X, float gx0, float gy0, float gz0,
                                       int hx, int hy, int hz) {
-  float gx = clampf_(gx0, 0.0f, (float)hx), gy = clampf_(gy0, 0.0f, (float)hy), gz = clampf_(gz0, 0.0f, (float)hz);
-  int bx = (int)floorf(gx), by = (int)floorf(gy), bz = (int)floorf(gz);
+  float gx = pclamp(gx0, (float)hx), gy = pclamp(gy0, (float)hy), gz = pclamp(gz0, (float)hz);
+  int bx = (int)gx, by = (int)gy, bz = (int)gz;
   bx = bx > hx - 1 ? hx - 1 : bx;
   by = by > hy - 1 ? hy - 1 : by;
   bz = bz > hz - 1 ? hz - 1 : bz;
@@ -139,12 +148,11 @@ struct CBase {
   float fx, fy, fz;
 };
 __device__ __forceinline__ CBase cbase(float gx0, float gy0, float gz0, int hx, int hy, int hz) {
-  const float gx = clampf_(gx0, 0.0f, (float)hx), gy = clampf_(gy0, 0.0f, (float)hy),
-              gz = clampf_(gz0, 0.0f, (float)hz);
+  const float gx = pclamp(gx0, (float)hx), gy = pclamp(gy0, (float)hy), gz = pclamp(gz0, (float)hz);
   CBase b;
-  b.bx = (int)floorf(gx);
-  b.by = (int)floorf(gy);
-  b.bz = (int)floorf(gz);
+  b.bx = (int)gx;
+  b.by = (int)gy;
+  b.bz = (int)gz;
   b.bx = b.bx > hx - 1 ? hx - 1 : b.bx;
   b.by = b.by > hy - 1 ? hy - 1 : b.by;
   b.bz = b.bz > hz - 1 ? hz - 1 : b.bz;
@@ -319,9 +327,8 @@ struct SSample {
   float fx, fy, fz;
 };
 __device__ __forceinline__ SSample sbase(const Geo &g, float gx0, float gy0, float gz0) {
-  float gx = clampf_(gx0, 0.0f, (float)(g.nx - 1)), gy = clampf_(gy0, 0.0f, (float)(g.ny - 1)),
-        gz = clampf_(gz0, 0.0f, (float)(g.nz - 1));
-  int bx = (int)floorf(gx), by = (int)floorf(gy), bz = (int)floorf(gz);
+  float gx = pclamp(gx0, (float)(g.nx - 1)), gy = pclamp(gy0, (float)(g.ny - 1)), gz = pclamp(gz0, (float)(g.nz - 1));
+  int bx = (int)gx, by = (int)gy, bz = (int)gz;
   bx = bx > g.nx - 2 ? g.nx - 2 : bx;
   by = by > g.ny - 2 ? g.ny - 2 : by;
   bz = bz > g.nz - 2 ? g.nz - 2 : bz;
