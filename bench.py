@@ -512,7 +512,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
-              "the multi-GPU slab decomposition is design-only (DESIGN.md).", file=sys.stderr)
+              "the multi-GPU z-slab decomposition is a design, not built.", file=sys.stderr)
         sys.exit(2)
     if int(os.environ.get("RANK", "0")) != 0:
         sys.exit(0)

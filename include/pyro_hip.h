@@ -3,8 +3,8 @@
  *
  * Drop-in boundary for the wgpu dispatch of the reference's Sim
  * (src/sim.rs).  A Rust FFI crate wraps these calls as Sim::new / step /
- * read_stats / read_scalars with the reference's signatures (see
- * INTEGRATION.md); C++ and Python hosts call them directly.
+ * read_stats / read_scalars with the reference's signatures; C++ and Python
+ * hosts call them directly.
  *
  * Conventions
  *  - Return 0 on success, a negative PYRO_E_* code on failure; the message is

@@ -6,7 +6,8 @@ FP32 arithmetic in source order (``-ffp-contract=off``), OpenMP for speed.
 hosts the tests run on.
 
 The reference itself (Rust + wgpu + WGSL) cannot be built in this
-environment, so there is no ``oracle/_ref`` build; see DESIGN.md.
+environment (no Rust toolchain, its crates are not vendored, no Vulkan
+driver), so there is no ``oracle/_ref`` build.
 """
 from __future__ import annotations
 

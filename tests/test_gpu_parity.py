@@ -1,6 +1,6 @@
 """GPU (HIP, gfx950) vs CPU oracle parity.
 
-Tolerances (north_star): every field within 1e-5 relative L2 after one step
+Tolerances: every field within 1e-5 relative L2 after one step
 and 1e-3 over a 10-step window.  The kernels follow the oracle's operation
 order with identical builtins (pexp, IEEE sqrt/div, -ffp-contract=off), so
 the per-stage checks below also demand bit-identical output.
