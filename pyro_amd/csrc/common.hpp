@@ -27,7 +27,8 @@ __device__ __forceinline__ RB rb_of(const void *p, uint32_t bytes) {
   b.r = __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, (int)bytes, 0x00020000);
   return b;
 }
-constexpr uint32_t RB_OOB = 0x80000000u;  // past every array: reads 0
+constexpr uint32_t RB_OOB = 0x80000000u;  // reads 0 from every array below 2 GiB, which are the
+                                           // only ones addressed with it (phi, b, codes, velocity)
 __device__ __forceinline__ float4 rb_ld4(const RB &b, uint32_t byte_off) {
   return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(b.r, byte_off, 0, 0));
 }

@@ -32,6 +32,39 @@ def test_reference_vcycles_bitwise_equal_oracle():
     assert np.array_equal(got.astype(np.float32), ref)
 
 
+@pytest.mark.parametrize("dims,levels", [((160, 160, 14), 1), ((320, 320, 32), 3)])
+def test_coarsest_level_beyond_lds_bitwise(dims, levels):
+    """A coarsest level too large for LDS (here a single 160x160x14 level, and
+    80x80x8 under two finer ones) is solved from global memory by one
+    workgroup; its 80 SOR iterations and the V-cycles around them stay
+    bit-identical to the oracle."""
+    nx, ny, nz = dims
+    rng = np.random.default_rng(4)
+    flags = np.zeros((nz, ny, nx), np.uint32)
+    for _ in range(12):
+        x0, y0, z0 = rng.integers(0, nx - 8), rng.integers(0, ny - 8), rng.integers(0, nz - 4)
+        flags[z0:z0 + 3, y0:y0 + 6, x0:x0 + 7] = 1
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    b = (np.sin(0.11 * x) * np.cos(0.07 * y) + 0.3 * np.sin(0.5 * z)).astype(np.float64)
+    b[flags == 1] = 0.0
+    h = 0.05
+    P = Poisson(flags, h)
+    assert P.levels[0] == levels
+    phi0 = (rng.normal(size=flags.shape) * 0.01).astype(np.float32)
+    P.set(b=b.astype(np.float32).astype(np.float64), phi=phi0.astype(np.float64))
+    P.set_sealed_projection("never")
+    info = P.solve("vcycle", tol=0.0, max_iters=2)
+    assert info.vcycles == 2
+    m = oracle.OMg((nx, ny, nz), h, 4, flags)
+    assert m.levels == levels
+    m.set(0, phi0, b.astype(np.float32))
+    m.vcycles(2)
+    ref, _, _ = m.get(0)
+    got = P.get().astype(np.float32)
+    bad = got.view(np.uint32) != ref.view(np.uint32)
+    assert not bad.any(), f"{int(bad.sum())} values differ"
+
+
 @pytest.mark.parametrize("method", ["mgpcg", "refine"])
 def test_solvers_reach_1e_minus_8_and_match_fp64_oracle(method):
     n, h = 64, 1.0 / 64
