@@ -48,7 +48,11 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-SWEEP_BYTES_PER_CELL = 16.0  # read phi 4 + b 4 + flags 4, write phi 4 (reference layout)
+# one red-black colour pass of mg.wgsl moves 12 B per cell in the reference
+# layout (phi 4, flags 4, half of b 2, half of phi written 2); one launch of
+# the fused sweep runs both passes.  The HBM bytes it really moves (both
+# passes fused, u8 codes) are the rocprof `traffic`, reported beside it.
+SWEEP_BYTES_PER_CELL = 24.0
 # renderer pack: read s0, s1, sol 48 + flags 4 + three velocity faces 12, write three half4 24
 RENDER_PACK_BYTES_PER_CELL = 88.0
 TRAFFIC_FILE = ROOT / "profiles" / "mg_sweep_l0_pool_2cm_traffic.json"
@@ -558,6 +562,7 @@ def main():
             "unit": "GB/s",
             "frac": r["achieved"] / HBM_PEAK_GBS,
             "traffic": traffic,
+            "traffic_GBs": (traffic / (r["smooth_ms"] * 1e-3) / 1e9) if traffic else None,
             "algo_bytes_per_launch": r["algo_bytes"],
             "avg_launch_us": r["smooth_ms"] * 1e3,
         },
