@@ -100,3 +100,38 @@ def load(path: Path = WEIGHTS) -> Net:
     net.set_params(z["params"])
     net.output_scale = float(z["output_scale"])
     return net
+
+
+def trained(path: Path = WEIGHTS, frames: int = 2000, steps: int = 3000, log=None) -> Net:
+    """The network saved at `path`; when the file is absent it is trained
+    first (pool at 4 cm, `frames` pairs, `steps` Adam steps: about 20 s on the
+    GPU with the defaults) and saved there."""
+    path = Path(path)
+    if path.exists():
+        return load(path)
+    net, rep = train_pool_net(frames=frames, steps=steps, lr=1e-3, log=log)
+    save(net, path)
+    if log:
+        log(f"U-Net trained: held-out loss {rep['heldout_loss']:.4f}, saved to {path}")
+    return net
+
+
+def main(argv=None) -> None:
+    """``python -m pyro_amd.learned [--out PATH] [--frames N] [--steps N]``:
+    train the preconditioner on the pool at 4 cm and save its weights."""
+    import argparse
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=str(WEIGHTS))
+    ap.add_argument("--frames", type=int, default=2000)
+    ap.add_argument("--steps", type=int, default=3000)
+    args = ap.parse_args(argv)
+    t0 = time.perf_counter()
+    net, rep = train_pool_net(frames=args.frames, steps=args.steps, lr=1e-3, log=print)
+    save(net, Path(args.out))
+    print(f"held-out loss {rep['heldout_loss']:.4f} (train {rep['train_loss']:.4f}), "
+          f"{time.perf_counter() - t0:.1f} s; saved to {args.out}", flush=True)
+
+
+if __name__ == "__main__":
+    main()

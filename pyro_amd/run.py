@@ -148,7 +148,9 @@ def run_scene(argv=None) -> dict:
         if args.solver == "learned":
             from pyro_amd import learned
 
-            P.attach_net(learned.load(args.weights or learned.WEIGHTS), "first")
+            # the given weights, or the default file (trained and saved on first use)
+            net = learned.load(args.weights) if args.weights else learned.trained(log=print)
+            P.attach_net(net, "first")
 
     fps = f32(scene.get("sim.fps"))
     frame_dt = f32(f32(1.0) / fps)

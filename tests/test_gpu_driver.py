@@ -20,3 +20,23 @@ def test_driver_runs_and_resumes(tmp_path, capsys):
                      "--solver", "mgpcg", "--tol", "1e-5", "--quiet"]) == 0
     out = capsys.readouterr().out
     assert "mean_solver_iterations" in out and "'frames': 3" in out
+
+
+def test_learned_weights_made_on_the_command_line_drive_the_loop(tmp_path, capsys):
+    """`python -m pyro_amd.learned --out W` trains the pool network and saves
+    it; learned.trained(W) then loads that file (no second training), and the
+    driver runs with --solver learned --weights W."""
+    import numpy as np
+
+    from pyro_amd import learned
+
+    w = tmp_path / "unet.npz"
+    learned.main(["--out", str(w), "--frames", "24", "--steps", "40"])
+    assert w.exists() and "saved to" in capsys.readouterr().out
+    a, b = learned.load(w), learned.trained(w)
+    assert np.array_equal(a.params(), b.params()) and a.output_scale == b.output_scale
+    scene = tmp_path / "mini.toml"
+    scene.write_text(MINI_SCENE)
+    assert run.main([str(scene), "--duration", "0.1", "--preroll", "0", "--solver", "learned",
+                     "--weights", str(w), "--quiet"]) == 0
+    assert "mean_solver_iterations" in capsys.readouterr().out
