@@ -668,29 +668,44 @@ __global__ __launch_bounds__(CO_NT) void k_mg_coarse(LvArgs a, const float *__re
 
 // The same solve with one thread per (x, y) column and the column's phi in
 // registers, for an outside weight of 1 (integer counts 1..6, div_count).
-// LDS holds phi z-fastest (rows of PZ = d2 + 2 rounded up to odd, so the 64
-// lanes of a wave, at consecutive x, fall in distinct banks); a cell reads
-// its four x/y neighbours there and its two z neighbours from registers.
-// Register slot j of a column holds z = j - ((x + y) & 1), so the cells of
-// colour c are the slots j = c, c + 2, ... on every lane; slots outside the
-// column hold phi = 0 and count 0 (never updated), which gives the Dirichlet
-// phi = 0 above the top and below the bottom as the ghost ring does.
+// LDS holds phi plane-major: plane z + 1 (z = -1 .. ZJ - 1) of CZ_PLANE
+// floats, the cell of column c = x + d0 * y at offset c.  CZ_PLANE is a
+// multiple of 64, so a read or store of one slot by the 64 lanes of a wave
+// (consecutive columns) touches 64 distinct banks.  Columns c >= d0 * d1 are
+// zero; a lane on a side of the domain reads its missing x or y neighbour
+// from the zero block at CZ_ZERO + (the neighbour's column mod 64), the bank
+// the neighbour would have used, which gives the Dirichlet phi = 0 beyond the
+// sides.  A cell reads its four x/y neighbours in LDS and its two z
+// neighbours from registers.  Register slot j of a column holds
+// z = j - ((x + y) & 1), so the cells of colour c are the slots j = c, c + 2,
+// ... on every lane; slots outside the column hold phi = 0 and count 0 (never
+// updated), which gives the Dirichlet phi = 0 above the top and below the
+// bottom.
+constexpr int CZ_ZERO = CO_NT, CZ_PLANE = CO_NT + 64;
+
 template <int ZJ>
 __global__ __launch_bounds__(CO_NT) void k_mg_coarse_z(LvArgs a, const float *__restrict__ b,
                                                         const uint8_t *__restrict__ f,
                                                         const uint8_t *__restrict__ nc, float *__restrict__ p,
                                                         int iters, float omega) {
-  __shared__ float sp[CO_PAD_MAX];
-  const int PZ = (a.d2 + 2) | 1, PX = a.d0 + 2, PY = a.d1 + 2;
-  const int SY = PX * PZ, PN = PY * SY;
+  constexpr int PN = (ZJ + 1) * CZ_PLANE;  // planes z + 1 = j + 1 - par for slots j < ZJ
+  static_assert(PN <= CO_PAD_MAX, "planes exceed the LDS budget");
+  __shared__ float sp[PN];
   const uint32_t sy = (uint32_t)a.d0, sz = (uint32_t)a.d0 * (uint32_t)a.d1;
   for (int i = threadIdx.x; i < PN; i += (int)blockDim.x) sp[i] = 0.0f;
+  const int ncol = a.d0 * a.d1;  // <= CO_NT (host)
   const int col = threadIdx.x;
-  const bool has = col < a.d0 * a.d1;
+  const bool has = col < ncol;
   const int x = has ? col % a.d0 : 0, y = has ? col / a.d0 : 0;
   const int par = (x + y) & 1;
-  // LDS index of slot j is B + j
-  const int B = ((y + 1) * PX + (x + 1)) * PZ + 1 - par;
+  // LDS index of slot j is B + j * CZ_PLANE.  A lane without a column
+  // (ncol <= col < CO_NT) reads columns col +- 1 and the zero block, all
+  // inside a plane, and never stores.
+  const int B = (1 - par) * CZ_PLANE + col;
+  const int cxp = !has || x < a.d0 - 1 ? col + 1 : CZ_ZERO + ((col + 1) & 63);
+  const int cxm = !has || x > 0 ? col - 1 : CZ_ZERO + ((col - 1) & 63);
+  const int cyp = has && y < a.d1 - 1 ? col + a.d0 : CZ_ZERO + ((col + a.d0) & 63);
+  const int cym = has && y > 0 ? col - a.d0 : CZ_ZERO + ((col - a.d0) & 63);
   float ph[ZJ + 2], hb[ZJ], rc[ZJ];
   uint32_t cp[(ZJ + 3) / 4];
   ph[0] = 0.0f;
@@ -722,19 +737,20 @@ __global__ __launch_bounds__(CO_NT) void k_mg_coarse_z(LvArgs a, const float *__
 #pragma unroll
     for (int j = 0; j < ZJ; j++) {
       const int z = j - par;
-      if (z >= 0 && z < a.d2) sp[B + j] = ph[j + 1];
+      if (z >= 0 && z < a.d2) sp[B + j * CZ_PLANE] = ph[j + 1];
     }
   }
   __syncthreads();
-  const float *sxp = sp + B + PZ, *sxm = sp + B - PZ, *syp = sp + B + SY, *sym = sp + B - SY;
+  const int pb = (1 - par) * CZ_PLANE;
+  const float *sxp = sp + pb + cxp, *sxm = sp + pb + cxm, *syp = sp + pb + cyp, *sym = sp + pb + cym;
   for (int it = 0; it < iters; it++) {
 #pragma unroll
     for (int c = 0; c < 2; c++) {
 #pragma unroll
       for (int k = 0; k < (ZJ + 3) / 4; k++) asm volatile("" : "+v"(cp[k]));  // conversions stay in the loop
-      // every slot of the colour is computed (reads stay inside sp: the
-      // host keeps PN + ZJ within it) and kept only where the count is not
-      // 0; the stores follow all reads of the pass, so no read waits on them
+      // every slot of the colour is computed (reads stay inside sp: planes
+      // j + 1 - par <= ZJ) and kept only where the count is not 0; the
+      // stores follow all reads of the pass, so no read waits on them
       if constexpr (ZJ <= 12) {
         // Short columns (one wave per SIMD on small levels): the slots of a
         // colour are independent (red-black), so all of the pass's LDS reads
@@ -748,10 +764,10 @@ __global__ __launch_bounds__(CO_NT) void k_mg_coarse_z(LvArgs a, const float *__
 #pragma unroll
         for (int t = 0; t < NS; t++) {
           const int j = c + 2 * t;
-          nb[0][t] = sxp[j];
-          nb[1][t] = sxm[j];
-          nb[2][t] = syp[j];
-          nb[3][t] = sym[j];
+          nb[0][t] = sxp[j * CZ_PLANE];
+          nb[1][t] = sxm[j * CZ_PLANE];
+          nb[2][t] = syp[j * CZ_PLANE];
+          nb[3][t] = sym[j * CZ_PLANE];
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -793,12 +809,15 @@ __global__ __launch_bounds__(CO_NT) void k_mg_coarse_z(LvArgs a, const float *__
         // Long columns: many waves share each SIMD and hide the latency;
         // registers are the limit (no per-pass arrays).  The next slot's
         // four reads are issued before this slot's arithmetic.
-        float nb[4] = {sxp[c], sxm[c], syp[c], sym[c]};
+        float nb[4] = {sxp[c * CZ_PLANE], sxm[c * CZ_PLANE], syp[c * CZ_PLANE], sym[c * CZ_PLANE]};
 #pragma unroll
         for (int j = c; j < ZJ; j += 2) {
           float nx[4] = {0.0f, 0.0f, 0.0f, 0.0f};
           if (j + 2 < ZJ) {
-            nx[0] = sxp[j + 2]; nx[1] = sxm[j + 2]; nx[2] = syp[j + 2]; nx[3] = sym[j + 2];
+            nx[0] = sxp[(j + 2) * CZ_PLANE];
+            nx[1] = sxm[(j + 2) * CZ_PLANE];
+            nx[2] = syp[(j + 2) * CZ_PLANE];
+            nx[3] = sym[(j + 2) * CZ_PLANE];
           }
           const float cn = (float)((cp[j >> 2] >> (8 * (j & 3))) & 0xffu);
           float sum = 0.0f;
@@ -816,7 +835,7 @@ __global__ __launch_bounds__(CO_NT) void k_mg_coarse_z(LvArgs a, const float *__
       }
 #pragma unroll
       for (int j = c; j < ZJ; j += 2)
-        if ((cp[j >> 2] >> (8 * (j & 3))) & 0xffu) sp[B + j] = ph[j + 1];
+        if ((cp[j >> 2] >> (8 * (j & 3))) & 0xffu) sp[B + j * CZ_PLANE] = ph[j + 1];
       __syncthreads();
     }
   }
@@ -866,9 +885,7 @@ bool mg_coarse_fits_lds(const MgLevel &L) {
 
 void launch_mg_coarse(const MgLevel &L, int iters, float omega, hipStream_t s) {
   LvArgs a = lv_args(L);
-  const uint32_t pz = (uint32_t)((L.d[2] + 2) | 1);
-  const bool colz = L.wdir == 1.0f && L.d[0] * L.d[1] <= CO_NT &&
-                    pz * (uint32_t)(L.d[0] + 2) * (uint32_t)(L.d[1] + 2) + 24u <= (uint32_t)CO_PAD_MAX;
+  const bool colz = L.wdir == 1.0f && L.d[0] * L.d[1] <= CO_NT;
   // one thread per column, whole waves only: a wave without columns would
   // still run every colour pass's arithmetic on the CU
   const int colt = ((L.d[0] * L.d[1] + 63) / 64) * 64;
