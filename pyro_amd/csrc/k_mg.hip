@@ -78,9 +78,11 @@ __device__ __forceinline__ float nc_count(uint32_t code, float wdir) {
 // (Markstein's correction).  Checked exhaustively over every finite float
 // x != -0 for each c: q equals the IEEE quotient whenever q is not subnormal
 // (the only mismatches are c = 6 with 0 < |x| < 2^-125), and a subnormal q
-// takes the IEEE division.  The callers' x = (neighbour sum from +0) - h^2 b
+// takes the IEEE division.  In the sweep, x = (neighbour sum from +0) - h^2 b
 // is never -0: a sum that starts at +0 never becomes -0, and s - t is -0
-// only for s = -0.
+// only for s = -0.  For x = -0 the result is +0 (IEEE: -0), and with c = r =
+// 0 it is +-0; the coarse solve uses q only in phi + (q - phi) omega, which
+// is the same for q = +0 and q = -0 (+0 for phi = +-0, else q - phi = -phi).
 __device__ __forceinline__ float div_count(float x, float c, float r) {
   const float q0 = x * r;
   const float e = __builtin_fmaf(-q0, c, x);
@@ -715,13 +717,18 @@ __global__ __launch_bounds__(CO_NT) void k_mg_coarse_z(LvArgs a, const float *__
 #pragma unroll
   for (int j = 0; j < ZJ; j++) {
     const int z = j - par;
-    float v = 0.0f, h = 0.0f, r = 1.0f;
+    // a slot that is never updated (outside the column, solid, or a cell
+    // with no fluid neighbour and no open face) holds phi = 0, count 0 and
+    // reciprocal 0: its quotient is then +-0 and the update keeps +0, so the
+    // passes need no per-slot select.  Such a cell's phi is read only by
+    // solid neighbours, which are not updated, and is not written back.
+    float v = 0.0f, h = 0.0f, r = 0.0f;
     uint32_t cnt = 0u;
     if (has && z >= 0 && z < a.d2) {
       const uint32_t g = (uint32_t)x + sy * (uint32_t)y + sz * (uint32_t)z;
       const uint8_t code = nc[g];
-      if (!(code & NC_SOLID)) v = p[g];
       if (!(code & (NC_SOLID | NC_NONE))) {
+        v = p[g];
         cnt = (code & 7u) + ((code >> 3) & 7u);
         h = a.hh * b[g];
         r = 1.0f / (float)cnt;
@@ -748,9 +755,12 @@ __global__ __launch_bounds__(CO_NT) void k_mg_coarse_z(LvArgs a, const float *__
     for (int c = 0; c < 2; c++) {
 #pragma unroll
       for (int k = 0; k < (ZJ + 3) / 4; k++) asm volatile("" : "+v"(cp[k]));  // conversions stay in the loop
-      // every slot of the colour is computed (reads stay inside sp: planes
-      // j + 1 - par <= ZJ) and kept only where the count is not 0; the
-      // stores follow all reads of the pass, so no read waits on them
+      // every slot of the colour is computed and stored (reads and stores
+      // stay inside sp: planes j + 1 - par <= ZJ, columns < CZ_PLANE; a slot
+      // that is never updated stores its +0); the stores follow all reads of
+      // the pass, so no read waits on them.  Each sum starts with its first
+      // term rather than +0 + term: the two differ only in the sign of a
+      // zero sum, and the update maps either to the same value.
       if constexpr (ZJ <= 12) {
         // Short columns (one wave per SIMD on small levels): the slots of a
         // colour are independent (red-black), so all of the pass's LDS reads
@@ -774,8 +784,7 @@ __global__ __launch_bounds__(CO_NT) void k_mg_coarse_z(LvArgs a, const float *__
         for (int t = 0; t < NS; t++) {
           const int j = c + 2 * t;
           const float cn = (float)((cp[j >> 2] >> (8 * (j & 3))) & 0xffu);
-          float sum = 0.0f;
-          sum += nb[0][t];
+          float sum = nb[0][t];
           sum += nb[1][t];
           sum += nb[2][t];
           sum += nb[3][t];
@@ -802,8 +811,7 @@ __global__ __launch_bounds__(CO_NT) void k_mg_coarse_z(LvArgs a, const float *__
 #pragma unroll
         for (int t = 0; t < NS; t++) {
           const int j = c + 2 * t;
-          const float nv = mixf_(ph[j + 1], qs[t], omega);
-          ph[j + 1] = ((cp[j >> 2] >> (8 * (j & 3))) & 0xffu) ? nv : ph[j + 1];
+          ph[j + 1] = mixf_(ph[j + 1], qs[t], omega);
         }
       } else {
         // Long columns: many waves share each SIMD and hide the latency;
@@ -820,22 +828,19 @@ __global__ __launch_bounds__(CO_NT) void k_mg_coarse_z(LvArgs a, const float *__
             nx[3] = sym[(j + 2) * CZ_PLANE];
           }
           const float cn = (float)((cp[j >> 2] >> (8 * (j & 3))) & 0xffu);
-          float sum = 0.0f;
-          sum += nb[0];
+          float sum = nb[0];
           sum += nb[1];
           sum += nb[2];
           sum += nb[3];
           sum += ph[j + 2];
           sum += ph[j];
-          const float nv = mixf_(ph[j + 1], div_count(sum - hb[j], cn, rc[j]), omega);
-          ph[j + 1] = cn != 0.0f ? nv : ph[j + 1];
+          ph[j + 1] = mixf_(ph[j + 1], div_count(sum - hb[j], cn, rc[j]), omega);
 #pragma unroll
           for (int k = 0; k < 4; k++) nb[k] = nx[k];
         }
       }
 #pragma unroll
-      for (int j = c; j < ZJ; j += 2)
-        if ((cp[j >> 2] >> (8 * (j & 3))) & 0xffu) sp[B + j * CZ_PLANE] = ph[j + 1];
+      for (int j = c; j < ZJ; j += 2) sp[B + j * CZ_PLANE] = ph[j + 1];
       __syncthreads();
     }
   }
@@ -845,7 +850,7 @@ __global__ __launch_bounds__(CO_NT) void k_mg_coarse_z(LvArgs a, const float *__
       const int z = j - par;
       if (z >= 0 && z < a.d2) {
         const uint32_t g = (uint32_t)x + sy * (uint32_t)y + sz * (uint32_t)z;
-        if (f[g] != 1) p[g] = ph[j + 1];
+        if (!(nc[g] & (NC_SOLID | NC_NONE))) p[g] = ph[j + 1];
       }
     }
   }
