@@ -683,15 +683,27 @@ __global__ __launch_bounds__(CO_NT) void k_mg_coarse(LvArgs a, const float *__re
 // ... on every lane; slots outside the column hold phi = 0 and count 0 (never
 // updated), which gives the Dirichlet phi = 0 above the top and below the
 // bottom.
-constexpr int CZ_ZERO = CO_NT, CZ_PLANE = CO_NT + 64;
+//
+// EX (whole waves): the block has blockDim.x = 64 k < d0 d1 threads, one per
+// column c < blockDim.x; the cells of the remaining columns (each SIMD runs
+// its waves' VALU work in turn, so a 13th wave of columns would add a
+// quarter to one SIMD's pass) are taken one per thread and colour: extra
+// index e = thread of colour c is leftover column blockDim.x + e % NL, cell
+// z = ((c + parity) & 1) + 2 (e / NL), NL = d0 d1 - blockDim.x leftover
+// columns (consecutive lanes, consecutive columns: distinct banks).  An extra cell
+// reads all six neighbours in LDS (z +- 1: planes, the planes z = -1 and
+// z = d2 are zero) and keeps its phi in a register.
+constexpr int CZ_ZERO = CO_NT, CZ_PLANE = CO_NT + 64, CZ_EX_NT = 768;
 
-template <int ZJ>
-__global__ __launch_bounds__(CO_NT) void k_mg_coarse_z(LvArgs a, const float *__restrict__ b,
-                                                        const uint8_t *__restrict__ f,
-                                                        const uint8_t *__restrict__ nc, float *__restrict__ p,
-                                                        int iters, float omega) {
+template <int ZJ, bool EX = false>
+__global__ __launch_bounds__(EX ? CZ_EX_NT : CO_NT) void k_mg_coarse_z(LvArgs a, const float *__restrict__ b,
+                                                                        const uint8_t *__restrict__ f,
+                                                                        const uint8_t *__restrict__ nc,
+                                                                        float *__restrict__ p, int iters,
+                                                                        float omega) {
   constexpr int PN = (ZJ + 1) * CZ_PLANE;  // planes z + 1 = j + 1 - par for slots j < ZJ
   static_assert(PN <= CO_PAD_MAX, "planes exceed the LDS budget");
+  static_assert(!EX || ZJ > 12, "extra cells are taken by the long-column passes");
   __shared__ float sp[PN];
   const uint32_t sy = (uint32_t)a.d0, sz = (uint32_t)a.d0 * (uint32_t)a.d1;
   for (int i = threadIdx.x; i < PN; i += (int)blockDim.x) sp[i] = 0.0f;
@@ -739,6 +751,38 @@ __global__ __launch_bounds__(CO_NT) void k_mg_coarse_z(LvArgs a, const float *__
     rc[j] = r;
     cp[j >> 2] |= cnt << (8 * (j & 3));
   }
+  // extra cell of each colour (EX): LDS index, neighbour indices, count,
+  // reciprocal, h^2 b, phi; a cell that is never updated holds 0 as above
+  bool ev[2] = {false, false};
+  int ei[2] = {0, 0}, en[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+  float ec[2] = {0.0f, 0.0f}, er[2] = {0.0f, 0.0f}, eh[2] = {0.0f, 0.0f}, ep[2] = {0.0f, 0.0f};
+  if constexpr (EX) {
+    const int nl = ncol - (int)blockDim.x;  // 1 .. 63 leftover columns
+    const int k = col / nl;
+    const int cl = (int)blockDim.x + (col - k * nl);  // leftover column
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      const int xl = cl % a.d0, yl = cl / a.d0;
+      const int z = ((c + xl + yl) & 1) + 2 * k;
+      if (z >= a.d2) continue;
+      const int pl = (z + 1) * CZ_PLANE;
+      ev[c] = true;
+      ei[c] = pl + cl;
+      en[c][0] = pl + (xl < a.d0 - 1 ? cl + 1 : CZ_ZERO + ((cl + 1) & 63));
+      en[c][1] = pl + (xl > 0 ? cl - 1 : CZ_ZERO + ((cl - 1) & 63));
+      en[c][2] = pl + (yl < a.d1 - 1 ? cl + a.d0 : CZ_ZERO + ((cl + a.d0) & 63));
+      en[c][3] = pl + (yl > 0 ? cl - a.d0 : CZ_ZERO + ((cl - a.d0) & 63));
+      const uint32_t g = (uint32_t)xl + sy * (uint32_t)yl + sz * (uint32_t)z;
+      const uint8_t code = nc[g];
+      if (!(code & (NC_SOLID | NC_NONE))) {
+        const uint32_t cnt = (code & 7u) + ((code >> 3) & 7u);
+        ep[c] = p[g];
+        ec[c] = (float)cnt;
+        er[c] = 1.0f / (float)cnt;
+        eh[c] = a.hh * b[g];
+      }
+    }
+  }
   __syncthreads();  // the zero fill is complete
   if (has) {
 #pragma unroll
@@ -746,6 +790,11 @@ __global__ __launch_bounds__(CO_NT) void k_mg_coarse_z(LvArgs a, const float *__
       const int z = j - par;
       if (z >= 0 && z < a.d2) sp[B + j * CZ_PLANE] = ph[j + 1];
     }
+  }
+  if constexpr (EX) {
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+      if (ev[c]) sp[ei[c]] = ep[c];
   }
   __syncthreads();
   const int pb = (1 - par) * CZ_PLANE;
@@ -841,6 +890,19 @@ __global__ __launch_bounds__(CO_NT) void k_mg_coarse_z(LvArgs a, const float *__
       }
 #pragma unroll
       for (int j = c; j < ZJ; j += 2) sp[B + j * CZ_PLANE] = ph[j + 1];
+      if constexpr (EX) {
+        if (ev[c]) {
+          const int i = ei[c];
+          float sum = sp[en[c][0]];
+          sum += sp[en[c][1]];
+          sum += sp[en[c][2]];
+          sum += sp[en[c][3]];
+          sum += sp[i + CZ_PLANE];
+          sum += sp[i - CZ_PLANE];
+          ep[c] = mixf_(ep[c], div_count(sum - eh[c], ec[c], er[c]), omega);
+          sp[i] = ep[c];
+        }
+      }
       __syncthreads();
     }
   }
@@ -852,6 +914,15 @@ __global__ __launch_bounds__(CO_NT) void k_mg_coarse_z(LvArgs a, const float *__
         const uint32_t g = (uint32_t)x + sy * (uint32_t)y + sz * (uint32_t)z;
         if (!(nc[g] & (NC_SOLID | NC_NONE))) p[g] = ph[j + 1];
       }
+    }
+  }
+  if constexpr (EX) {
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      if (!ev[c]) continue;
+      const int z = ei[c] / CZ_PLANE - 1, cl = ei[c] - (z + 1) * CZ_PLANE;
+      const uint32_t g = (uint32_t)cl + sz * (uint32_t)z;  // column index cl = x + d0 y
+      if (!(nc[g] & (NC_SOLID | NC_NONE))) p[g] = ep[c];
     }
   }
 }
@@ -890,16 +961,34 @@ bool mg_coarse_fits_lds(const MgLevel &L) {
 
 void launch_mg_coarse(const MgLevel &L, int iters, float omega, hipStream_t s) {
   LvArgs a = lv_args(L);
-  const bool colz = L.wdir == 1.0f && L.d[0] * L.d[1] <= CO_NT;
+  const int ncol = L.d[0] * L.d[1];
+  const bool colz = L.wdir == 1.0f && ncol <= CO_NT;
   // one thread per column, whole waves only: a wave without columns would
   // still run every colour pass's arithmetic on the CU
-  const int colt = ((L.d[0] * L.d[1] + 63) / 64) * 64;
+  const int colt = ((ncol + 63) / 64) * 64;
   if (colz && L.d[2] + 1 <= 12) {
     k_mg_coarse_z<12><<<1, colt, 0, s>>>(a, L.b, L.f, L.nc, L.phi, iters, omega);
     return;
   }
   if (colz && L.d[2] + 1 <= 22) {
-    k_mg_coarse_z<22><<<1, colt, 0, s>>>(a, L.b, L.f, L.nc, L.phi, iters, omega);
+    // waves run on SIMD w % 4.  Per colour pass a SIMD issues ZJ / 2 slots
+    // for each of its column waves plus one for each of its waves holding
+    // extra cells: take the extra-cell layout when its busiest SIMD issues
+    // fewer slots.
+    constexpr int ns = 22 / 2;  // slots per colour pass
+    const int full = (ncol / 64) * 64, zh = (L.d[2] + 1) / 2;
+    const int wc = colt / 64, wf = full / 64, we = ((ncol - full) * zh + 63) / 64;
+    const int load_col = ns * ((wc + 3) / 4);
+    int load_ex = 0;
+    for (int sm = 0; sm < 4; sm++) {
+      int l = 0;
+      for (int w = sm; w < wf; w += 4) l += ns + (w < we ? 1 : 0);
+      load_ex = l > load_ex ? l : load_ex;
+    }
+    if (full >= 64 && full <= CZ_EX_NT && (ncol - full) * zh <= full && load_ex < load_col)
+      k_mg_coarse_z<22, true><<<1, full, 0, s>>>(a, L.b, L.f, L.nc, L.phi, iters, omega);
+    else
+      k_mg_coarse_z<22><<<1, colt, 0, s>>>(a, L.b, L.f, L.nc, L.phi, iters, omega);
     return;
   }
   if (mg_coarse_fits_lds(L)) {
