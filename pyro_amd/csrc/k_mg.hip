@@ -971,24 +971,32 @@ void launch_mg_coarse(const MgLevel &L, int iters, float omega, hipStream_t s) {
     return;
   }
   if (colz && L.d[2] + 1 <= 22) {
-    // waves run on SIMD w % 4.  Per colour pass a SIMD issues ZJ / 2 slots
-    // for each of its column waves plus one for each of its waves holding
-    // extra cells: take the extra-cell layout when its busiest SIMD issues
-    // fewer slots.
-    constexpr int ns = 22 / 2;  // slots per colour pass
+    // waves run on SIMD w % 4.  Per iteration (both colours) a SIMD issues
+    // ZJ slots for each of its column waves plus two for each of its waves
+    // holding extra cells: take the extra-cell layout when its busiest SIMD
+    // issues fewer slots.
+    const int zj = L.d[2] + 1 <= 21 ? 21 : 22;
     const int full = (ncol / 64) * 64, zh = (L.d[2] + 1) / 2;
     const int wc = colt / 64, wf = full / 64, we = ((ncol - full) * zh + 63) / 64;
-    const int load_col = ns * ((wc + 3) / 4);
+    const int load_col = zj * ((wc + 3) / 4);
     int load_ex = 0;
     for (int sm = 0; sm < 4; sm++) {
       int l = 0;
-      for (int w = sm; w < wf; w += 4) l += ns + (w < we ? 1 : 0);
+      for (int w = sm; w < wf; w += 4) l += zj + (w < we ? 2 : 0);
       load_ex = l > load_ex ? l : load_ex;
     }
-    if (full >= 64 && full <= CZ_EX_NT && (ncol - full) * zh <= full && load_ex < load_col)
-      k_mg_coarse_z<22, true><<<1, full, 0, s>>>(a, L.b, L.f, L.nc, L.phi, iters, omega);
-    else
-      k_mg_coarse_z<22><<<1, colt, 0, s>>>(a, L.b, L.f, L.nc, L.phi, iters, omega);
+    const bool ex = full >= 64 && full <= CZ_EX_NT && (ncol - full) * zh <= full && load_ex < load_col;
+    if (zj == 21) {
+      if (ex)
+        k_mg_coarse_z<21, true><<<1, full, 0, s>>>(a, L.b, L.f, L.nc, L.phi, iters, omega);
+      else
+        k_mg_coarse_z<21><<<1, colt, 0, s>>>(a, L.b, L.f, L.nc, L.phi, iters, omega);
+    } else {
+      if (ex)
+        k_mg_coarse_z<22, true><<<1, full, 0, s>>>(a, L.b, L.f, L.nc, L.phi, iters, omega);
+      else
+        k_mg_coarse_z<22><<<1, colt, 0, s>>>(a, L.b, L.f, L.nc, L.phi, iters, omega);
+    }
     return;
   }
   if (mg_coarse_fits_lds(L)) {

@@ -70,14 +70,15 @@ def test_coarsest_level_beyond_lds_bitwise(dims, levels):
 
 
 @pytest.mark.parametrize("dims,levels", [((15, 52, 20), 1), ((30, 104, 40), 2), ((160, 320, 160), 4),
-                                         ((14, 43, 20), 1)])
+                                         ((14, 43, 20), 1), ((15, 52, 21), 1), ((12, 60, 21), 1)])
 def test_coarse_column_layouts_bitwise(dims, levels):
-    """The coarsest-level column solve in both of its layouts.  780 columns
+    """The coarsest-level column solve in both of its layouts, with 21 and 22
+    register slots per column (levels 20 and 21 cells deep).  780 columns
     (15x52) and 800 (20x40, the pool at 2 cm) run as 12 waves of whole
     columns with the leftover columns' cells taken one per thread and colour;
-    602 columns (14x43) run as 10 waves of columns, which issue fewer slots on
-    the busiest SIMD.  Random initial phi and solid blocks; bit-identical to
-    the oracle's V-cycles."""
+    602 columns (14x43) and 720 (12x60) run as waves of columns, which issue
+    fewer slots on the busiest SIMD.  Random initial phi and solid blocks;
+    bit-identical to the oracle's V-cycles."""
     _vcycles_match_oracle(dims, levels, blocks=6)
 
 
