@@ -115,6 +115,18 @@ __global__ __launch_bounds__(NT) void k_mg_restrict(LvArgs F, int ntx, int nty, 
   const int tx = bid % ntx, rr = bid / ntx, ty = rr % nty, tz = rr / nty;
   const int fx0 = 2 * RS_TX * tx - 1, fy0 = 2 * RS_TY * ty - 1, fz0 = 2 * RS_TZ * tz - 1;  // footprint origin
   const int sy = F.d0, sz = F.d0 * F.d1;
+  const int lcx = threadIdx.x % RS_TX, lr = threadIdx.x / RS_TX, lcy = lr % RS_TY, lcz = lr / RS_TY;
+  const int x = RS_TX * tx + lcx, y = RS_TY * ty + lcy, z = RS_TZ * tz + lcz;
+  // the eight children's h^2 b, issued with the staging loads (a child
+  // outside the level, also of a coarse cell outside it, reads element 0
+  // and is skipped)
+  float bv[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const int a = 2 * x + (k & 1), bb = 2 * y + ((k >> 1) & 1), c = 2 * z + ((k >> 2) & 1);
+    const bool in = a < F.d0 && bb < F.d1 && c < F.d2;
+    bv[k] = bf[in ? a + sy * bb + sz * c : 0];
+  }
   // staging: every load is unconditional (outside the level at element 0)
   // and selected afterwards, so all of a thread's loads are in flight at once
 #pragma unroll
@@ -133,18 +145,7 @@ __global__ __launch_bounds__(NT) void k_mg_restrict(LvArgs F, int ntx, int nty, 
     }
   }
   __syncthreads();
-  const int lcx = threadIdx.x % RS_TX, lr = threadIdx.x / RS_TX, lcy = lr % RS_TY, lcz = lr / RS_TY;
-  const int x = RS_TX * tx + lcx, y = RS_TY * ty + lcy, z = RS_TZ * tz + lcz;
   if (x >= c0 || y >= c1 || z >= c2) return;
-  // the eight children's h^2 b, loaded together (a child outside the level
-  // reads element 0 and is skipped)
-  float bv[8];
-#pragma unroll
-  for (int k = 0; k < 8; k++) {
-    const int a = 2 * x + (k & 1), bb = 2 * y + ((k >> 1) & 1), c = 2 * z + ((k >> 2) & 1);
-    const bool in = a < F.d0 && bb < F.d1 && c < F.d2;
-    bv[k] = bf[in ? a + sy * bb + sz * c : 0];
-  }
   float acc = 0.0f, nfl = 0.0f;
 #pragma unroll
   for (int k = 0; k < 8; k++) {
@@ -192,6 +193,23 @@ __global__ __launch_bounds__(NT) void k_mg_prolong(int f0, int f1, int f2, int n
   const int tx = bid % ntx, rr = bid / ntx, ty = rr % nty, tz = rr / nty;
   const int x0 = PL_TX * tx, y0 = PL_TY * ty, z0 = PL_TZ * tz;
   const int cx0 = x0 / 2 - 1, cy0 = y0 / 2 - 1, cz0 = z0 / 2 - 1;  // coarse footprint origin (unclamped)
+  // a thread's cells: x = x0 + lx, y = y0 + ly, z = z0 + q for q < PL_TZ
+  static_assert(NT / PL_TX == PL_TY && PL_TZ % 2 == 0, "one (x, y) column of PL_TZ cells per thread");
+  constexpr int NQ = PL_TZ;
+  const int lx = threadIdx.x % PL_TX, ly = threadIdx.x / PL_TX;
+  const int x = x0 + lx, y = y0 + ly;
+  // the thread's cells' flags and phi, issued with the staging loads (a
+  // cell outside the level reads element 0 and is skipped)
+  uint8_t fv[NQ];
+  float pv[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    const int z = z0 + q;
+    const uint32_t i =
+        (x < f0 && y < f1 && z < f2) ? (uint32_t)x + (uint32_t)f0 * ((uint32_t)y + (uint32_t)f1 * (uint32_t)z) : 0u;
+    fv[q] = ff[i];
+    pv[q] = pf[i];
+  }
   for (int s = threadIdx.x; s < PL_CN; s += NT) {
     const int u = s % PL_CX, r2 = s / PL_CX, v = r2 % PL_CY, w = r2 / PL_CY;
     const int cx = clampi(cx0 + u, 0, c0 - 1), cy = clampi(cy0 + v, 0, c1 - 1), cz = clampi(cz0 + w, 0, c2 - 1);
@@ -200,47 +218,49 @@ __global__ __launch_bounds__(NT) void k_mg_prolong(int f0, int f1, int f2, int n
     sf[s] = fc[j];
   }
   __syncthreads();
-  const int lx = threadIdx.x % PL_TX, lyz = threadIdx.x / PL_TX;  // 8 (y, z) slots of 4 cells
-  const int x = x0 + lx;
-  if (x >= f0) return;
-  const float px = ((float)x + 0.5f) * 0.5f - 0.5f;
-  const int i0x = (int)floorf(px);
-  const float frx = px - (float)i0x;
-  // the thread's cells' flags and phi are loaded together first (a cell
-  // outside the level reads element 0 and is skipped)
-  constexpr int NQ = (PL_TY * PL_TZ) / (NT / PL_TX);
-  uint8_t fv[NQ];
-  float pv[NQ];
+  if (x >= f0 || y >= f1) return;
+  // A fine index's fraction is 0.75 (even) or 0.25 (odd), exactly, so the
+  // weights are exact: wx wy is formed once per thread, and z = z0 + q (z0 a
+  // multiple of 4) has the parity of q, so frz is a constant; the product
+  // keeps the per-cell formula's order, (wx wy) wz.
+  const float px = ((float)x + 0.5f) * 0.5f - 0.5f, py = ((float)y + 0.5f) * 0.5f - 0.5f;
+  const int i0x = (int)floorf(px), i0y = (int)floorf(py);
+  const float frx = px - (float)i0x, fry = py - (float)i0y;
+  float wxy[4];
 #pragma unroll
-  for (int q = 0; q < NQ; q++) {
-    const int slot = lyz + q * (NT / PL_TX);
-    const int y = y0 + slot % PL_TY, z = z0 + slot / PL_TY;
-    const uint32_t i = (y < f1 && z < f2) ? (uint32_t)x + (uint32_t)f0 * ((uint32_t)y + (uint32_t)f1 * (uint32_t)z) : 0u;
-    fv[q] = ff[i];
-    pv[q] = pf[i];
+  for (int k = 0; k < 4; k++) {
+    const float wx = (k & 1) ? frx : 1.0f - frx, wy = (k >> 1) ? fry : 1.0f - fry;
+    wxy[k] = wx * wy;
   }
+  const int sxy = (i0y - cy0) * PL_CX + (i0x - cx0);
 #pragma unroll
   for (int q = 0; q < NQ; q++) {
-    const int slot = lyz + q * (NT / PL_TX);
-    const int y = y0 + slot % PL_TY, z = z0 + slot / PL_TY;
-    if (y >= f1 || z >= f2) continue;
+    const int z = z0 + q;
+    if (z >= f2) continue;
     const uint32_t i = (uint32_t)x + (uint32_t)f0 * ((uint32_t)y + (uint32_t)f1 * (uint32_t)z);
     if (fv[q] == 1) continue;
-    const float py = ((float)y + 0.5f) * 0.5f - 0.5f, pz = ((float)z + 0.5f) * 0.5f - 0.5f;
-    const int i0y = (int)floorf(py), i0z = (int)floorf(pz);
-    const float fry = py - (float)i0y, frz = pz - (float)i0z;
+    const float pz = ((float)z + 0.5f) * 0.5f - 0.5f;
+    const int i0z = (int)floorf(pz);
+    const float frz = (q & 1) ? 0.25f : 0.75f;  // = pz - i0z
     float acc = 0.0f, ws = 0.0f;
 #pragma unroll
     for (int k = 0; k < 8; k++) {
       const int ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
-      const int si = ((i0z + oz - cz0) * PL_CY + (i0y + oy - cy0)) * PL_CX + (i0x + ox - cx0);
+      const int si = (i0z + oz - cz0) * (PL_CY * PL_CX) + sxy + oy * PL_CX + ox;
       if (sf[si] == 1) continue;
-      const float wx = ox == 1 ? frx : 1.0f - frx, wy = oy == 1 ? fry : 1.0f - fry, wz = oz == 1 ? frz : 1.0f - frz;
-      const float wgt = wx * wy * wz;
+      const float wgt = wxy[k & 3] * (oz == 1 ? frz : 1.0f - frz);
       acc += wgt * sp[si];
       ws += wgt;
     }
-    if (ws > 1e-4f) pf[i] = pv[q] + acc / ws;
+    // with no solid corner the weights sum to exactly 1 and acc / 1 = acc:
+    // the division runs (a branch, not a select) only for the other cells
+    float corr = acc;
+    if (ws != 1.0f) {
+      float a = acc;
+      asm volatile("" : "+v"(a));
+      corr = a / ws;
+    }
+    if (ws > 1e-4f) pf[i] = pv[q] + corr;
   }
 }
 void launch_mg_prolong(const MgLevel &C, const MgLevel &F, hipStream_t s) {
