@@ -281,17 +281,25 @@ void launch_velocity_noise(const Fields &f, float amp, uint32_t seed, hipStream_
 
 // ---------------------------------------------------------------- stats
 // Deterministic combination of the per-block partials into the 1040 slots of
-// stats.wgsl:2-3 (FP64; maxima are exact in any order).
+// stats.wgsl:2-3 (FP64; maxima are exact in any order).  The reductions are
+// independent, so each runs in a block of its own, concurrently: blocks 0-4
+// the react sums, 5 the react maximum, 6-8 the solid sums, 9 the curl
+// maximum, 10-11 the residual sums, 12 the per-layer sums (and the unused
+// slots 12-15).  Each keeps its one-block order: every thread sums its
+// strided partials in order, then a fixed tree combines the threads.
+constexpr int STATS_JOBS = 13;
 __global__ __launch_bounds__(1024) void k_stats_finalize(Geo g, StatParts st, int with_resid) {
   __shared__ double sh[1024];
   __shared__ float shf[1024];
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, job = blockIdx.x;
   double *out = st.out;
-  // layer sums 16+y: one thread per layer, blocks in order
-  for (int y = t; y < 1024; y += 1024) {
-    out[16 + y] = y < g.ny ? seq_sum(st.react, st.react_gx, (size_t)y * st.react_gx * 6 + 5, 6) : 0.0;
+  if (job == 12) {
+    // layer sums 16+y: one thread per layer, blocks in order
+    for (int y = t; y < 1024; y += 1024)
+      out[16 + y] = y < g.ny ? seq_sum(st.react, st.react_gx, (size_t)y * st.react_gx * 6 + 5, 6) : 0.0;
+    if (t >= 12 && t < 16) out[t] = 0.0;
+    return;
   }
-  if (t < 16) out[t] = 0.0;
   auto tree = [&](double v) -> double {
     sh[t] = v;
     __syncthreads();
@@ -299,9 +307,7 @@ __global__ __launch_bounds__(1024) void k_stats_finalize(Geo g, StatParts st, in
       if (t < s) sh[t] += sh[t + s];
       __syncthreads();
     }
-    double r = sh[0];
-    __syncthreads();
-    return r;
+    return sh[0];
   };
   auto treemax = [&](float v) -> float {
     shf[t] = v;
@@ -310,38 +316,33 @@ __global__ __launch_bounds__(1024) void k_stats_finalize(Geo g, StatParts st, in
       if (t < s) shf[t] = fmax_(shf[t], shf[t + s]);
       __syncthreads();
     }
-    float r = shf[0];
-    __syncthreads();
-    return r;
+    return shf[0];
   };
   const int nreact = g.ny * st.react_gx;
-  const int slot_r[5] = {0, 1, 3, 6, 8};
-  for (int q = 0; q < 5; q++) {
-    double r = tree(seq_sum(st.react, strided_count(t, nreact, 1024), (size_t)t * 6 + q, 6 * 1024));
-    if (t == 0) out[slot_r[q]] = r;
-  }
-  {
-    float r = treemax(seq_max(st.react_t, strided_count(t, nreact, 1024), (size_t)t, 1024));
+  if (job < 5) {
+    const int slot_r[5] = {0, 1, 3, 6, 8};
+    const double r = tree(seq_sum(st.react, strided_count(t, nreact, 1024), (size_t)t * 6 + job, 6 * 1024));
+    if (t == 0) out[slot_r[job]] = r;
+  } else if (job == 5) {
+    const float r = treemax(seq_max(st.react_t, strided_count(t, nreact, 1024), (size_t)t, 1024));
     if (t == 0) out[4] = (double)r;
-  }
-  const int slot_s[3] = {2, 7, 9};
-  for (int q = 0; q < 3; q++) {
-    double r = tree(seq_sum(st.solid, strided_count(t, st.n_solid, 1024), (size_t)t * 3 + q, 3 * 1024));
+  } else if (job < 9) {
+    const int q = job - 6, slot_s[3] = {2, 7, 9};
+    const double r = tree(seq_sum(st.solid, strided_count(t, st.n_solid, 1024), (size_t)t * 3 + q, 3 * 1024));
     if (t == 0) out[slot_s[q]] = r;
-  }
-  {
-    float r = treemax(seq_max(st.curl, strided_count(t, st.n_curl, 1024), (size_t)t, 1024));
+  } else if (job == 9) {
+    const float r = treemax(seq_max(st.curl, strided_count(t, st.n_curl, 1024), (size_t)t, 1024));
     if (t == 0) out[5] = (double)r;
-  }
-  for (int q = 0; q < 2; q++) {
-    const double a = with_resid ? seq_sum(st.resid, strided_count(t, st.n_resid, 1024), (size_t)t * 2 + q, 2 * 1024)
-                                : 0.0;
-    double r = tree(a);
+  } else {
+    const int q = job - 10;
+    const double a =
+        with_resid ? seq_sum(st.resid, strided_count(t, st.n_resid, 1024), (size_t)t * 2 + q, 2 * 1024) : 0.0;
+    const double r = tree(a);
     if (t == 0) out[10 + q] = r;
   }
 }
 void launch_stats_finalize(const Geo &g, const StatParts &st, int with_resid, hipStream_t s) {
-  k_stats_finalize<<<1, 1024, 0, s>>>(g, st, with_resid);
+  k_stats_finalize<<<STATS_JOBS, 1024, 0, s>>>(g, st, with_resid);
 }
 
 // ---------------------------------------------------------------- copy
