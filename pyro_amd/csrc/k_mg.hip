@@ -282,25 +282,28 @@ __global__ __launch_bounds__(NT) void k_mg_counts(LvArgs a, FastDiv d0, const ui
                                                    uint8_t *__restrict__ nc, uint8_t *__restrict__ nm) {
   uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   if (i >= a.n) return;
-  if (f[i] == 1) {
-    nc[i] = NC_SOLID;
-    if (nm) nm[i] = 0;
-    return;
-  }
   uint32_t r = fdiv(i, d0);
   int x = (int)(i - r * (uint32_t)a.d0);
   uint32_t zq = fdiv(r, a.dny);
   int y = (int)(r - zq * (uint32_t)a.d1), z = (int)zq;
   const uint32_t sy = (uint32_t)a.d0, sz = (uint32_t)a.d0 * (uint32_t)a.d1;
-  int fl = 0, out = 0;
-  uint32_t m = 0;
-  if (x + 1 < a.d0) { m |= (f[i + 1] != 1) ? 1u : 0u; } else out++;
-  if (x - 1 >= 0) { m |= (f[i - 1] != 1) ? 2u : 0u; } else out++;
-  if (y + 1 < a.d1) { m |= (f[i + sy] != 1) ? 4u : 0u; } else out++;
-  if (y - 1 >= 0) m |= (f[i - sy] != 1) ? 8u : 0u;  // ground: not counted
-  if (z + 1 < a.d2) { m |= (f[i + sz] != 1) ? 16u : 0u; } else out++;
-  if (z - 1 >= 0) { m |= (f[i - sz] != 1) ? 32u : 0u; } else out++;
-  fl = __builtin_popcount(m);
+  const bool hxp = x + 1 < a.d0, hxm = x > 0, hyp = y + 1 < a.d1, hym = y > 0, hzp = z + 1 < a.d2, hzm = z > 0;
+  // the cell's and its six neighbours' flags are loaded together (a
+  // neighbour outside the level reads the cell itself and is not used)
+  const uint8_t fc = f[i];
+  const uint8_t fxp = f[hxp ? i + 1 : i], fxm = f[hxm ? i - 1 : i];
+  const uint8_t fyp = f[hyp ? i + sy : i], fym = f[hym ? i - sy : i];
+  const uint8_t fzp = f[hzp ? i + sz : i], fzm = f[hzm ? i - sz : i];
+  if (fc == 1) {
+    nc[i] = NC_SOLID;
+    if (nm) nm[i] = 0;
+    return;
+  }
+  const uint32_t m = (hxp && fxp != 1 ? 1u : 0u) | (hxm && fxm != 1 ? 2u : 0u) | (hyp && fyp != 1 ? 4u : 0u) |
+                     (hym && fym != 1 ? 8u : 0u) | (hzp && fzp != 1 ? 16u : 0u) | (hzm && fzm != 1 ? 32u : 0u);
+  // neighbours outside the box on a side or the top; the ground (y - 1) is not counted
+  const int out = (hxp ? 0 : 1) + (hxm ? 0 : 1) + (hyp ? 0 : 1) + (hzp ? 0 : 1) + (hzm ? 0 : 1);
+  const int fl = __builtin_popcount(m);
   nc[i] = (fl + out) == 0 ? NC_NONE : (uint8_t)(fl | (out << 3));
   if (nm) nm[i] = (uint8_t)m;
 }
