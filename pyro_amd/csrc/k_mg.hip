@@ -780,7 +780,7 @@ __global__ __launch_bounds__(EX ? CZ_EX_NT : CO_NT) void k_mg_coarse_z(LvArgs a,
   int ei[2] = {0, 0}, en[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
   float ec[2] = {0.0f, 0.0f}, er[2] = {0.0f, 0.0f}, eh[2] = {0.0f, 0.0f}, ep[2] = {0.0f, 0.0f};
   if constexpr (EX) {
-    const int nl = ncol - (int)blockDim.x;  // 1 .. 63 leftover columns
+    const int nl = ncol - (int)blockDim.x;  // 1 .. 63 leftover columns (the host launches EX only then)
     const int k = col / nl;
     const int cl = (int)blockDim.x + (col - k * nl);  // leftover column
 #pragma unroll
@@ -1008,7 +1008,7 @@ void launch_mg_coarse(const MgLevel &L, int iters, float omega, hipStream_t s) {
       for (int w = sm; w < wf; w += 4) l += zj + (w < we ? 2 : 0);
       load_ex = l > load_ex ? l : load_ex;
     }
-    const bool ex = full >= 64 && full <= CZ_EX_NT && (ncol - full) * zh <= full && load_ex < load_col;
+    const bool ex = full >= 64 && full <= CZ_EX_NT && ncol > full && (ncol - full) * zh <= full && load_ex < load_col;
     if (zj == 21) {
       if (ex)
         k_mg_coarse_z<21, true><<<1, full, 0, s>>>(a, L.b, L.f, L.nc, L.phi, iters, omega);
