@@ -703,19 +703,19 @@ __global__ __launch_bounds__(CO_NT) void k_mg_coarse(LvArgs a, const float *__re
 // sides.  A cell reads its four x/y neighbours in LDS and its two z
 // neighbours from registers.  Register slot j of a column holds
 // z = j - ((x + y) & 1), so the cells of colour c are the slots j = c, c + 2,
-// ... on every lane; slots outside the column hold phi = 0 and count 0 (never
-// updated), which gives the Dirichlet phi = 0 above the top and below the
-// bottom.
+// ... on every lane; slots outside the column hold phi = 0 and count 0 (each
+// pass computes them like the others and they keep +0), which gives the
+// Dirichlet phi = 0 above the top and below the bottom.
 //
 // EX (whole waves): the block has blockDim.x = 64 k < d0 d1 threads, one per
-// column c < blockDim.x; the cells of the remaining columns (each SIMD runs
-// its waves' VALU work in turn, so a 13th wave of columns would add a
-// quarter to one SIMD's pass) are taken one per thread and colour: extra
+// column c < blockDim.x; the cells of the remaining columns (each SIMD issues
+// its waves' VALU work in turn, so a wave of columns beyond a multiple of
+// four adds a whole wave's slots to one SIMD) are taken one per thread and colour: extra
 // index e = thread of colour c is leftover column blockDim.x + e % NL, cell
 // z = ((c + parity) & 1) + 2 (e / NL), NL = d0 d1 - blockDim.x leftover
 // columns (consecutive lanes, consecutive columns: distinct banks).  An extra cell
-// reads all six neighbours in LDS (z +- 1: planes, the planes z = -1 and
-// z = d2 are zero) and keeps its phi in a register.
+// reads all six neighbours in LDS (z +- 1: planes; the planes z = -1 and
+// z = d2 hold +0) and keeps its phi in a register.
 constexpr int CZ_ZERO = CO_NT, CZ_PLANE = CO_NT + 64, CZ_EX_NT = 768;
 
 template <int ZJ, bool EX = false>
