@@ -36,6 +36,7 @@ extern "C" {
 #define PYRO_E_NONFINITE (-4) /* a finiteness check found NaN/Inf */
 #define PYRO_E_STATE (-5)     /* call not valid in the current state */
 #define PYRO_E_NODEVICE (-6)  /* no usable GPU */
+#define PYRO_E_DIVERGED (-7)  /* an iterative solve's residual grew past 1e3 x its best */
 
 /* Field ids for read/write_field (superset of sim.rs:494 read_scalars). */
 enum PyroField {
@@ -149,6 +150,13 @@ int pyro_sim_load(PyroSim *sim, const char *path);
 int pyro_sim_begin_step(PyroSim *sim, float dt, int want_stats);
 int pyro_sim_run_stage(PyroSim *sim, int stage);
 int pyro_sim_end_step(PyroSim *sim);
+/* Close an open step without ending it: the ping-pong roles, time and frame
+ * stay those before begin_step (vel[cur] and sol[sol_cur] are unchanged), but
+ * the stages already run have written their outputs -- after react, s0/s1
+ * hold the step's scalars -- so a retry of the step needs the state restored
+ * (pyro_sim_load).  Stats of the aborted step are not finalised.  Returns 0
+ * when no step is open. */
+int pyro_sim_abort_step(PyroSim *sim);
 /* Time `reps` launches of one stage with HIP events on the sim stream
  * (average ms per launch; state is advanced reps times). */
 int pyro_sim_time_stage(PyroSim *sim, int stage, int reps, float *avg_ms);
@@ -189,6 +197,7 @@ typedef struct {
   double rel_resid; /* sqrt(sum r^2 / sum b^2) over fluid cells, FP64 */
   float ms;       /* device time of the solve (HIP events) */
   int fallbacks;  /* LEARNED: 1 if the network's direction was rejected and plain V-cycles took over */
+  int diverged;   /* 1 if the iteration stopped because the residual grew past 1e3 x its best */
 } PyroSolveInfo;
 const char *pyro_poisson_last_error(void);
 /* flags: N x u32, 1 = solid (Neumann). */
@@ -200,7 +209,10 @@ int pyro_poisson_levels(const PyroPoisson *p, int *ref_levels, int *pc_levels);
 int pyro_poisson_set(PyroPoisson *p, const double *b, const double *phi);
 int pyro_poisson_get(PyroPoisson *p, double *phi);
 int pyro_poisson_residual(PyroPoisson *p, double *rel);
-/* Iterate until the relative residual is <= tol or max_iters outer iterations.
+/* Iterate until the relative residual is <= tol (tol >= 0) or max_iters outer
+ * iterations, or until the residual grows past 1e3 times the best one reached
+ * (returns 0 with info->diverged = 1; the iterate is the last one).  A NaN or
+ * Inf residual returns PYRO_E_NONFINITE.
  * Sealed fluid components (pure-Neumann pockets: face-connected fluid cells
  * none of which lies on a side or the top) make the system singular and, with
  * a right-hand side that does not sum to zero over them, inconsistent: no
@@ -225,7 +237,11 @@ int pyro_poisson_set_flags_from_sim(PyroPoisson *p, PyroSim *sim);
 int pyro_poisson_set_from_sim(PyroPoisson *p, PyroSim *sim, int warm);
 /* One step (as pyro_sim_step) whose pressure solve runs `method` on p to a
  * relative residual of tol (at most max_iters outer iterations), warm-started
- * from the previous phi, instead of the fixed V-cycle count. */
+ * from the previous phi, instead of the fixed V-cycle count.  The arguments
+ * (method, tol, max_iters, an attached network for LEARNED, the grid) are
+ * checked before the step begins.  A failure after that -- a HIP error, a
+ * non-finite residual (PYRO_E_NONFINITE), a diverged solve (PYRO_E_DIVERGED,
+ * info filled) -- aborts the step as pyro_sim_abort_step does. */
 int pyro_sim_step_solve(PyroSim *sim, PyroPoisson *p, int method, double tol, int max_iters, float dt,
                         int want_stats, PyroSolveInfo *info);
 

@@ -68,6 +68,7 @@ def hip() -> C.CDLL:
         lib.pyro_sim_begin_step.argtypes = [P, f32, i32]
         lib.pyro_sim_run_stage.argtypes = [P, i32]
         lib.pyro_sim_end_step.argtypes = [P]
+        lib.pyro_sim_abort_step.argtypes = [P]
         lib.pyro_sim_time_stage.argtypes = [P, i32, i32, C.POINTER(f32)]
         lib.pyro_sim_time_kernel.argtypes = [P, i32, i32, i32, C.POINTER(f32)]
         lib.pyro_sim_add_velocity_noise.argtypes = [P, f32, u32]

@@ -287,16 +287,21 @@ void launch_velocity_noise(const Fields &f, float amp, uint32_t seed, hipStream_
 // maximum, 10-11 the residual sums, 12 the per-layer sums (and the unused
 // slots 12-15).  Each keeps its one-block order: every thread sums its
 // strided partials in order, then a fixed tree combines the threads.
+// `ran` has a bit per stats-producing stage that ran in this step
+// (STATS_RAN_*); the slots of a stage that did not run are 0, as they are in a
+// step whose accumulators start at 0.
 constexpr int STATS_JOBS = 13;
-__global__ __launch_bounds__(1024) void k_stats_finalize(Geo g, StatParts st, int with_resid) {
+__global__ __launch_bounds__(1024) void k_stats_finalize(Geo g, StatParts st, uint32_t ran) {
   __shared__ double sh[1024];
   __shared__ float shf[1024];
   const int t = threadIdx.x, job = blockIdx.x;
   double *out = st.out;
+  const bool react = ran & STATS_RAN_REACT, solid = ran & STATS_RAN_SOLID;
+  const bool curl = ran & STATS_RAN_CURL, resid = ran & STATS_RAN_RESID;
   if (job == 12) {
     // layer sums 16+y: one thread per layer, blocks in order
     for (int y = t; y < 1024; y += 1024)
-      out[16 + y] = y < g.ny ? seq_sum(st.react, st.react_gx, (size_t)y * st.react_gx * 6 + 5, 6) : 0.0;
+      out[16 + y] = y < g.ny && react ? seq_sum(st.react, st.react_gx, (size_t)y * st.react_gx * 6 + 5, 6) : 0.0;
     if (t >= 12 && t < 16) out[t] = 0.0;
     return;
   }
@@ -319,30 +324,38 @@ __global__ __launch_bounds__(1024) void k_stats_finalize(Geo g, StatParts st, in
     return shf[0];
   };
   const int nreact = g.ny * st.react_gx;
+  // the block-uniform `ran` tests select what a block sums; every thread still
+  // reaches each barrier of the tree
   if (job < 5) {
     const int slot_r[5] = {0, 1, 3, 6, 8};
-    const double r = tree(seq_sum(st.react, strided_count(t, nreact, 1024), (size_t)t * 6 + job, 6 * 1024));
+    const double a =
+        react ? seq_sum(st.react, strided_count(t, nreact, 1024), (size_t)t * 6 + job, 6 * 1024) : 0.0;
+    const double r = tree(a);
     if (t == 0) out[slot_r[job]] = r;
   } else if (job == 5) {
-    const float r = treemax(seq_max(st.react_t, strided_count(t, nreact, 1024), (size_t)t, 1024));
+    const float a = react ? seq_max(st.react_t, strided_count(t, nreact, 1024), (size_t)t, 1024) : 0.0f;
+    const float r = treemax(a);
     if (t == 0) out[4] = (double)r;
   } else if (job < 9) {
     const int q = job - 6, slot_s[3] = {2, 7, 9};
-    const double r = tree(seq_sum(st.solid, strided_count(t, st.n_solid, 1024), (size_t)t * 3 + q, 3 * 1024));
+    const double a =
+        solid ? seq_sum(st.solid, strided_count(t, st.n_solid, 1024), (size_t)t * 3 + q, 3 * 1024) : 0.0;
+    const double r = tree(a);
     if (t == 0) out[slot_s[q]] = r;
   } else if (job == 9) {
-    const float r = treemax(seq_max(st.curl, strided_count(t, st.n_curl, 1024), (size_t)t, 1024));
+    const float a = curl ? seq_max(st.curl, strided_count(t, st.n_curl, 1024), (size_t)t, 1024) : 0.0f;
+    const float r = treemax(a);
     if (t == 0) out[5] = (double)r;
   } else {
     const int q = job - 10;
     const double a =
-        with_resid ? seq_sum(st.resid, strided_count(t, st.n_resid, 1024), (size_t)t * 2 + q, 2 * 1024) : 0.0;
+        resid ? seq_sum(st.resid, strided_count(t, st.n_resid, 1024), (size_t)t * 2 + q, 2 * 1024) : 0.0;
     const double r = tree(a);
     if (t == 0) out[10 + q] = r;
   }
 }
-void launch_stats_finalize(const Geo &g, const StatParts &st, int with_resid, hipStream_t s) {
-  k_stats_finalize<<<STATS_JOBS, 1024, 0, s>>>(g, st, with_resid);
+void launch_stats_finalize(const Geo &g, const StatParts &st, uint32_t ran, hipStream_t s) {
+  k_stats_finalize<<<STATS_JOBS, 1024, 0, s>>>(g, st, ran);
 }
 
 // ---------------------------------------------------------------- copy

@@ -49,7 +49,10 @@ void launch_react(const Fields &f, StatParts *st, hipStream_t s);
 void launch_forces(const Fields &f, hipStream_t s);
 void launch_divergence(const Fields &f, hipStream_t s);
 void launch_project(const Fields &f, hipStream_t s);
-void launch_stats_finalize(const Geo &g, const StatParts &st, int with_resid, hipStream_t s);
+// bits of the stats-producing stages that ran in a step (k_stats_finalize)
+constexpr uint32_t STATS_RAN_SOLID = 1u, STATS_RAN_CURL = 2u, STATS_RAN_REACT = 4u, STATS_RAN_RESID = 8u;
+constexpr uint32_t STATS_RAN_ALL = 15u;
+void launch_stats_finalize(const Geo &g, const StatParts &st, uint32_t ran, hipStream_t s);
 void launch_velocity_noise(const Fields &f, float amp, uint32_t seed, hipStream_t s);
 void launch_set_step(Params *P, float dt, float time, float ign_on, uint32_t frame, hipStream_t s);
 

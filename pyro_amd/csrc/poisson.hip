@@ -23,6 +23,7 @@
 #include "kernels.hpp"
 #include "net.hpp"
 #include "pyro_hip.h"
+#include "solve_guard.hpp"
 
 using namespace pyro;
 
@@ -354,7 +355,7 @@ int pyro_poisson_residual(PyroPoisson *P, double *rel) {
 }
 
 int pyro_poisson_solve(PyroPoisson *P, int method, double tol, int max_iters, PyroSolveInfo *info) {
-  if (!P || max_iters < 1) return pfail(PYRO_E_INVALID, "bad argument");
+  if (!P || max_iters < 1 || !(tol >= 0.0)) return pfail(PYRO_E_INVALID, "bad argument");
   if (method < PYRO_SOLVER_VCYCLE || method > PYRO_SOLVER_LEARNED) return pfail(PYRO_E_INVALID, "unknown method");
   if (method == PYRO_SOLVER_LEARNED && !P->net) return pfail(PYRO_E_STATE, "no network attached");
   hipEvent_t e0, e1;
@@ -365,9 +366,22 @@ int pyro_poisson_solve(PyroPoisson *P, int method, double tol, int max_iters, Py
     launch_seal_project(P->seal_comp, P->seal_off, P->seal_idx, P->b, P->stream);
     P->b_fresh = false;
   }
-  int it = 0, cycles = 0, rc = 0, fallbacks = 0;
+  int it = 0, cycles = 0, rc = 0, fallbacks = 0, diverged = 0;
   double rel = 0.0;
   const uint32_t n = P->n;
+  // Every residual read is checked (solve_guard.hpp): NaN/Inf fails the
+  // solve, growth past kDivergeFactor times the best residual stops it with
+  // info->diverged set.  Returns true when the loop must stop.
+  ResidWatch rw;
+  auto watch = [&](int k) -> bool {
+    switch (rw.step(rel, k)) {
+      case Watch::NonFinite:
+        rc = pfail(PYRO_E_NONFINITE, "non-finite residual after " + std::to_string(k) + " iterations");
+        return true;
+      case Watch::Diverged: diverged = 1; return true;
+      default: return false;
+    }
+  };
   if (method == PYRO_SOLVER_VCYCLE) {
     // the reference's V-cycles on the FP32 level-0 state, FP64 residual after each
     MgLevel &L0 = P->ref[0];
@@ -376,7 +390,8 @@ int pyro_poisson_solve(PyroPoisson *P, int method, double tol, int max_iters, Py
     double bb = 0.0;
     rc = rel_resid(P, &rel, &bb);
     if (bb <= 0.0) bb = 1.0;
-    while (!rc && rel > tol && it < max_iters) {
+    if (!rc) watch(0);
+    while (!rc && !diverged && rel > tol && it < max_iters) {
       enqueue_vcycle(P->ref, P->nl_ref, 80, P->stream);
       launch_to_f64(n, L0.phi, P->x, P->stream);
       cycles++;
@@ -385,14 +400,17 @@ int pyro_poisson_solve(PyroPoisson *P, int method, double tol, int max_iters, Py
       double rr = 0.0;
       PCHK(hipMemcpyAsync(&rr, P->dev + 2, sizeof(double), hipMemcpyDeviceToHost, P->stream));
       rc = sync(P);
+      if (rc) break;
       rel = std::sqrt(rr / bb);
+      if (watch(it)) break;
     }
   } else if (method == PYRO_SOLVER_REFINE) {
     // x += V(b - A x) with the residual in FP64
     double bb = 0.0;
     rc = rel_resid(P, &rel, &bb);
     if (bb <= 0.0) bb = 1.0;
-    while (!rc && rel > tol && it < max_iters) {
+    if (!rc) watch(0);
+    while (!rc && !diverged && rel > tol && it < max_iters) {
       MgLevel &L0 = P->pc[0];
       launch_to_f32_zero(n, P->r, L0.b, L0.phi, P->stream);
       enqueue_vcycle(P->pc, P->nl_pc, P->coarse_iters_pc, P->stream);
@@ -403,7 +421,9 @@ int pyro_poisson_solve(PyroPoisson *P, int method, double tol, int max_iters, Py
       double rr = 0.0;
       PCHK(hipMemcpyAsync(&rr, P->dev + 2, sizeof(double), hipMemcpyDeviceToHost, P->stream));
       rc = sync(P);
+      if (rc) break;
       rel = std::sqrt(rr / bb);
+      if (watch(it)) break;
     }
   } else {
     // flexible CG (Notay's FCG(1)): p_{k+1} = z_{k+1} - (z_{k+1}.q_k / p_k.q_k) p_k.
@@ -441,7 +461,8 @@ int pyro_poisson_solve(PyroPoisson *P, int method, double tol, int max_iters, Py
     double bb = 0.0;
     rc = rel_resid(P, &rel, &bb);  // r = b - A x
     if (bb <= 0.0) bb = 1.0;
-    if (!rc && rel > tol) {
+    if (!rc) watch(0);
+    if (!rc && !diverged && rel > tol) {
       rc = apply_m(rel * rel * bb, false);
       cycles++;
       if (!rc) PCHK(hipMemcpyAsync(P->p, P->z, n * sizeof(double), hipMemcpyDeviceToDevice, P->stream));
@@ -454,13 +475,14 @@ int pyro_poisson_solve(PyroPoisson *P, int method, double tol, int max_iters, Py
         rc = sync(P);
         if (rc) break;
         rel = std::sqrt(rr / bb);
-        if (rel <= tol) break;
+        if (watch(it) || rel <= tol) break;
         rc = apply_m(rr, true);  // z, dev[3] z.q
         if (rc) break;
         cycles++;
         launch_xpby_dev(n, P->z, P->dev, P->p, P->stream);
       }
       if (!rc) rc = rel_resid(P, &rel, nullptr);  // true residual of the final x
+      if (!rc && !std::isfinite(rel)) rc = pfail(PYRO_E_NONFINITE, "non-finite residual of the final iterate");
     }
   }
   PCHK(hipEventRecord(e1, P->stream));
@@ -476,6 +498,7 @@ int pyro_poisson_solve(PyroPoisson *P, int method, double tol, int max_iters, Py
     info->rel_resid = rel;
     info->ms = ms;
     info->fallbacks = fallbacks;
+    info->diverged = diverged;
   }
   return 0;
 }
@@ -544,23 +567,48 @@ int pyro_poisson_set_from_sim(PyroPoisson *P, PyroSim *sim, int warm) {
 int pyro_sim_step_solve(PyroSim *sim, PyroPoisson *P, int method, double tol, int max_iters, float dt,
                         int want_stats, PyroSolveInfo *info) {
   if (!sim || !P) return pfail(PYRO_E_INVALID, "null argument");
-  auto simfail = [&]() { return pfail(PYRO_E_STATE, std::string("simulation: ") + pyro_last_error()); };
-  if (pyro_sim_begin_step(sim, dt, want_stats)) return simfail();
+  // every argument is checked before the step begins
+  if (max_iters < 1 || !(tol >= 0.0) || !std::isfinite(tol)) return pfail(PYRO_E_INVALID, "bad argument");
+  if (method < PYRO_SOLVER_VCYCLE || method > PYRO_SOLVER_LEARNED) return pfail(PYRO_E_INVALID, "unknown method");
+  if (method == PYRO_SOLVER_LEARNED && !P->net) return pfail(PYRO_E_STATE, "no network attached");
+  uint32_t sd[3] = {0, 0, 0};
+  if (pyro_sim_info(sim, sd, nullptr, nullptr, nullptr, nullptr))
+    return pfail(PYRO_E_INVALID, std::string("simulation: ") + pyro_last_error());
+  if ((int)sd[0] != P->d[0] || (int)sd[1] != P->d[1] || (int)sd[2] != P->d[2])
+    return pfail(PYRO_E_INVALID, "grid mismatch between the simulation and the pressure problem");
+  if (pyro_sim_begin_step(sim, dt, want_stats))
+    return pfail(PYRO_E_STATE, std::string("simulation: ") + pyro_last_error());
+  // From here a failure aborts the step: vel[cur], sol[sol_cur], time and
+  // frame stay those before it; s0/s1 and phi hold the partial step.
+  auto abort_with = [&](int rc) {
+    (void)pyro_sim_abort_step(sim);
+    return rc;
+  };
+  auto simfail = [&]() {
+    return abort_with(pfail(PYRO_E_STATE, std::string("simulation: ") + pyro_last_error()));
+  };
   for (int st = PYRO_STAGE_SOLID; st <= PYRO_STAGE_DIVERGENCE; st++)
     if (pyro_sim_run_stage(sim, st)) return simfail();
   int rc = pyro_poisson_set_flags_from_sim(P, sim);
   if (!rc) rc = pyro_poisson_set_from_sim(P, sim, 1);
   PyroSolveInfo si{};
   if (!rc) rc = pyro_poisson_solve(P, method, tol, max_iters, &si);
-  if (rc) return rc;
+  if (rc) return abort_with(rc);
+  if (si.diverged) {
+    if (info) *info = si;
+    return abort_with(pfail(PYRO_E_DIVERGED, "pressure solve diverged: relative residual " +
+                                                 std::to_string(si.rel_resid) + " after " +
+                                                 std::to_string(si.iterations) + " iterations"));
+  }
   void *pp = nullptr;
   size_t nb = 0;
   if (pyro_sim_device_ptr(sim, PYRO_FIELD_LEVEL_BASE + 0, &pp, &nb)) return simfail();
   launch_to_f32(P->n, P->x, static_cast<float *>(pp), P->stream);
   rc = sync(P);
-  if (rc) return rc;
+  if (rc) return abort_with(rc);
   if (want_stats && pyro_sim_run_stage(sim, PYRO_STAGE_RESID)) return simfail();
-  if (pyro_sim_run_stage(sim, PYRO_STAGE_PROJECT) || pyro_sim_end_step(sim)) return simfail();
+  if (pyro_sim_run_stage(sim, PYRO_STAGE_PROJECT)) return simfail();
+  if (pyro_sim_end_step(sim)) return pfail(PYRO_E_STATE, std::string("simulation: ") + pyro_last_error());
   if (info) *info = si;
   return 0;
 }

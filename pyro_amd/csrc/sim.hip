@@ -56,6 +56,7 @@ struct PyroSim {
   bool use_graph = true;
   hipGraphExec_t gexec[2][2][2] = {};
   bool in_step = false, step_stats = false;
+  uint32_t stats_ran = 0;  // STATS_RAN_* bits of the stages run in the open step
   std::vector<void *> allocs;
   int *d_flag = nullptr;
   // renderer hand-off: packed volumes, the stream that copies them out and
@@ -129,7 +130,27 @@ void enqueue_step_body(PyroSim *s, bool stats, hipStream_t st) {
   for (int k = PYRO_STAGE_SOLID; k <= PYRO_STAGE_VCYCLES; k++) enqueue_stage(s, k, stats, st);
   if (stats) enqueue_stage(s, PYRO_STAGE_RESID, true, st);
   enqueue_stage(s, PYRO_STAGE_PROJECT, stats, st);
-  if (stats) launch_stats_finalize(s->g, s->st, 1, st);
+  if (stats) launch_stats_finalize(s->g, s->st, STATS_RAN_ALL, st);
+}
+
+void drop_graphs(PyroSim *s) {
+  for (auto &a : s->gexec)
+    for (auto &b : a)
+      for (auto &c : b)
+        if (c) {
+          (void)hipGraphExecDestroy(c);
+          c = nullptr;
+        }
+}
+
+uint32_t stats_bit(int stage) {
+  switch (stage) {
+    case PYRO_STAGE_SOLID: return STATS_RAN_SOLID;
+    case PYRO_STAGE_CURL: return STATS_RAN_CURL;
+    case PYRO_STAGE_REACT: return STATS_RAN_REACT;
+    case PYRO_STAGE_RESID: return STATS_RAN_RESID;
+    default: return 0u;
+  }
 }
 
 int get_graph(PyroSim *s, bool stats, hipGraphExec_t *out) {
@@ -433,10 +454,7 @@ int pyro_sim_destroy(PyroSim *s) {
   if (s->ev_packed) (void)hipEventDestroy(s->ev_packed);
   if (s->ev_copied) (void)hipEventDestroy(s->ev_copied);
   if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
-  for (auto &a : s->gexec)
-    for (auto &b : a)
-      for (auto &c : b)
-        if (c) (void)hipGraphExecDestroy(c);
+  drop_graphs(s);
   for (void *p : s->allocs) (void)hipFree(p);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
@@ -456,6 +474,7 @@ int pyro_sim_begin_step(PyroSim *s, float dt, int want_stats) {
   s->dt = dt;
   s->in_step = true;
   s->step_stats = want_stats != 0;
+  s->stats_ran = 0;
   return 0;
 }
 
@@ -464,6 +483,7 @@ int pyro_sim_run_stage(PyroSim *s, int stage) {
   if (!s->in_step) return fail(PYRO_E_STATE, "run_stage outside begin_step/end_step");
   if (stage < 0 || stage >= PYRO_STAGE_COUNT) return fail(PYRO_E_INVALID, "bad stage id");
   enqueue_stage(s, stage, s->step_stats, s->stream);
+  s->stats_ran |= stats_bit(stage);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(PYRO_E_HIP, std::string("launch: ") + hipGetErrorString(e));
   return 0;
@@ -472,7 +492,7 @@ int pyro_sim_run_stage(PyroSim *s, int stage) {
 int pyro_sim_end_step(PyroSim *s) {
   if (!s) return fail(PYRO_E_INVALID, "null sim");
   if (!s->in_step) return fail(PYRO_E_STATE, "end_step without begin_step");
-  if (s->step_stats) launch_stats_finalize(s->g, s->st, 1, s->stream);
+  if (s->step_stats) launch_stats_finalize(s->g, s->st, s->stats_ran, s->stream);
   // sim.rs:419-422
   s->cur = 1 - s->cur;
   s->sol_cur = 1 - s->sol_cur;
@@ -482,6 +502,16 @@ int pyro_sim_end_step(PyroSim *s) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(PYRO_E_HIP, std::string("launch: ") + hipGetErrorString(e));
   return 0;
+}
+
+int pyro_sim_abort_step(PyroSim *s) {
+  if (!s) return fail(PYRO_E_INVALID, "null sim");
+  if (!s->in_step) return 0;
+  // roles, time and frame stay those before begin_step
+  s->in_step = false;
+  s->step_stats = false;
+  s->stats_ran = 0;
+  return sync_check(s);
 }
 
 int pyro_sim_step(PyroSim *s, float dt, int want_stats) {
@@ -634,6 +664,12 @@ int pyro_sim_load(PyroSim *s, const char *path) {
           " does not match the simulation's";
   else if (std::fread(&hp, sizeof(Params), 1, f) != 1)
     err = "truncated checkpoint";
+  else if (hp.nx != h.nx || hp.ny != h.ny || hp.nz != h.nz)
+    err = "checkpoint Params grid does not match its header";
+  else if (std::memcmp(&hp.dx, &s->hp.dx, sizeof(float)) != 0)
+    // the multigrid levels' h and h^2 are derived from dx at creation
+    err = "checkpoint cell size dx = " + std::to_string(hp.dx) + " differs from the simulation's " +
+          std::to_string(s->hp.dx);
   std::vector<std::vector<char>> data(fl.size());
   for (size_t k = 0; k < fl.size() && err.empty(); k++) {
     data[k].resize(fl[k].bytes);
@@ -641,7 +677,10 @@ int pyro_sim_load(PyroSim *s, const char *path) {
   }
   std::fclose(f);
   if (!err.empty()) return fail(PYRO_E_INVALID, err + " (" + path + ")");
-  // nothing on the device changes before the whole file has been read
+  // nothing on the device changes before the whole file has been read.
+  // Captured graphs hold the V-cycle count and the ping-pong roles of the
+  // running simulation: they are captured again at the next step.
+  drop_graphs(s);
   s->hp = hp;
   HIPCHK(hipMemcpy(s->dP, &s->hp, sizeof(Params), hipMemcpyHostToDevice));
   for (size_t k = 0; k < fl.size(); k++) HIPCHK(hipMemcpy(fl[k].ptr, data[k].data(), fl[k].bytes, hipMemcpyHostToDevice));

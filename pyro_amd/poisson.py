@@ -27,7 +27,7 @@ METHODS = {"vcycle": 0, "refine": 1, "mgpcg": 2, "learned": 3}
 
 class _Info(C.Structure):
     _fields_ = [("iterations", C.c_int), ("vcycles", C.c_int), ("rel_resid", C.c_double), ("ms", C.c_float),
-                ("fallbacks", C.c_int)]
+                ("fallbacks", C.c_int), ("diverged", C.c_int)]
 
 
 @dataclass
@@ -37,6 +37,11 @@ class SolveInfo:
     rel_resid: float
     ms: float
     fallbacks: int = 0
+    diverged: bool = False
+
+    @classmethod
+    def _of(cls, info: "_Info") -> "SolveInfo":
+        return cls(info.iterations, info.vcycles, info.rel_resid, info.ms, info.fallbacks, bool(info.diverged))
 
 
 def _lib():
@@ -138,9 +143,11 @@ class Poisson:
         return r.value
 
     def solve(self, method: str = "mgpcg", tol: float = 1e-6, max_iters: int = 200) -> SolveInfo:
+        if method not in METHODS:
+            raise ValueError(f"unknown method {method!r}; choices: {sorted(METHODS)}")
         info = _Info()
         _check(_lib().pyro_poisson_solve(self._h, METHODS[method], float(tol), int(max_iters), C.byref(info)))
-        return SolveInfo(info.iterations, info.vcycles, info.rel_resid, info.ms, info.fallbacks)
+        return SolveInfo._of(info)
 
     def attach_net(self, net, mode: str = "every") -> None:
         """Use `net` (a pyro_amd.net.Net) in the "learned" method: in every
@@ -209,8 +216,14 @@ def make_c3_problem(n: int = 256, seed: int = 0, h: float = 1.0 / 256, solid_fra
 
 def step_solve(sim, P: Poisson, dt: float, method: str = "mgpcg", tol: float = 1e-4, max_iters: int = 100,
                want_stats: bool = False) -> SolveInfo:
-    """One simulation step whose pressure solve runs `method` on P to `tol`."""
+    """One simulation step whose pressure solve runs `method` on P to `tol`.
+
+    Raises PyroError on a bad argument before the step begins; on a failure
+    inside the step (HIP error, non-finite or diverged solve) the step is
+    aborted (pyro_sim_abort_step) and PyroError is raised."""
+    if method not in METHODS:
+        raise ValueError(f"unknown method {method!r}; choices: {sorted(METHODS)}")
     info = _Info()
     _check(_lib().pyro_sim_step_solve(sim._h, P._h, METHODS[method], float(tol), int(max_iters), float(dt),
                                       int(bool(want_stats)), C.byref(info)))
-    return SolveInfo(info.iterations, info.vcycles, info.rel_resid, info.ms, info.fallbacks)
+    return SolveInfo._of(info)

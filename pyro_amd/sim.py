@@ -221,6 +221,10 @@ class Sim:
     def end_step(self) -> None:
         _native.check_hip(_native.hip().pyro_sim_end_step(self._h))
 
+    def abort_step(self) -> None:
+        """Close an open step without ending it (roles, time and frame unchanged)."""
+        _native.check_hip(_native.hip().pyro_sim_abort_step(self._h))
+
     def time_stage(self, stage, reps: int) -> float:
         sid = STAGES.index(stage) if isinstance(stage, str) else int(stage)
         ms = C.c_float()
