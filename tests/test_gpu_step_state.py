@@ -34,9 +34,24 @@ def _sim(scene, seed=None, cycles=4):
     return sim
 
 
+# entries of the padded (nz+1, ny+1, nx+1) velocity arrays that are a
+# component of the state (compValid, common.wgsl:66-72); a step neither reads
+# nor writes the others, so a simulation keeps whatever its buffers held there
+_VALID = {"u": (0, 0, 1), "v": (0, 1, 0), "w": (1, 0, 0)}
+
+
+def _state(sim, n):
+    a = sim.read_field(n)
+    if n in _VALID:
+        ez, ey, ex = _VALID[n]
+        nz, ny, nx = (d - 1 for d in a.shape)
+        a = a[:nz + ez, :ny + ey, :nx + ex]
+    return a
+
+
 def _same(a, b, names):
     for n in names:
-        assert np.array_equal(a.read_field(n), b.read_field(n)), n
+        assert np.array_equal(_state(a, n), _state(b, n)), n
 
 
 def test_rejected_step_solve_leaves_the_sim_usable():
