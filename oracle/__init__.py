@@ -52,6 +52,8 @@ def lib() -> C.CDLL:
         L.osim_time.argtypes = [V]
         L.osim_time.restype = C.c_double
         L.osim_add_velocity_noise.argtypes = [V, C.c_float, C.c_uint32]
+        L.osim_set_clock.argtypes = [V, C.c_double, C.c_uint32, C.c_float]
+        L.osim_get_params.argtypes = [V, V]
         L.osim_set_threads.argtypes = [C.c_int]
         L.osim_render_pack.argtypes = [V, V]
         L.oracle_expf.argtypes = [C.c_float]
@@ -90,8 +92,82 @@ def noise(comp: int, idx: int, seed: int) -> float:
     return lib().oracle_noise(comp, idx, seed)
 
 
+# Checkpoint files of pyro_sim_save (pyro_amd/csrc/sim.hip): this fixed
+# header, the 576-byte Params, then seven raw little-endian FP32 fields in the
+# device layout -- u, v, w of the current velocity (padded (nz+1, ny+1, nx+1)
+# each), s0, s1, the current SOL ((nz, ny, nx, 4) each) and level-0 phi
+# ((nz, ny, nx)).  Read and written here independently of the product code.
+CKPT_MAGIC = b"PYROCKPT"
+CKPT_HEADER = np.dtype([("magic", "S8"), ("version", "<u4"), ("nx", "<u4"), ("ny", "<u4"), ("nz", "<u4"),
+                        ("params_bytes", "<u4"), ("time", "<f4"), ("ign_until", "<f4"), ("frame", "<u4"),
+                        ("cycles", "<u4"), ("nfields", "<u4"), ("pad", "<u4")])
+CKPT_FIELDS = ("u", "v", "w", "s0", "s1", "sol", "phi")
+
+
+def _ckpt_shapes(nx, ny, nz):
+    fv, c4 = (nz + 1, ny + 1, nx + 1), (nz, ny, nx, 4)
+    return (fv, fv, fv, c4, c4, c4, (nz, ny, nx))
+
+
+def read_checkpoint(path) -> dict:
+    """Header values, the Params blob and the seven fields of a checkpoint."""
+    raw = Path(path).read_bytes()
+    h = np.frombuffer(raw, CKPT_HEADER, count=1)[0]
+    if h["magic"] != CKPT_MAGIC or int(h["version"]) != 1 or int(h["params_bytes"]) != 576:
+        raise ValueError(f"{path}: not a version-1 checkpoint")
+    nx, ny, nz = int(h["nx"]), int(h["ny"]), int(h["nz"])
+    out = {k: h[k].item() for k in ("time", "ign_until", "frame", "cycles")}
+    off = CKPT_HEADER.itemsize
+    out["params"] = raw[off:off + 576]
+    off += 576
+    if int(h["nfields"]) != len(CKPT_FIELDS):
+        raise ValueError(f"{path}: {int(h['nfields'])} fields")
+    for name, shape in zip(CKPT_FIELDS, _ckpt_shapes(nx, ny, nz)):
+        cnt = int(np.prod(shape))
+        if off + 4 * cnt > len(raw):
+            raise ValueError(f"{path}: truncated")
+        out[name] = np.frombuffer(raw, "<f4", count=cnt, offset=off).reshape(shape).copy()
+        off += 4 * cnt
+    if off != len(raw):
+        raise ValueError(f"{path}: {len(raw) - off} trailing bytes")
+    out["dims"] = (nx, ny, nz)
+    return out
+
+
 class OSim:
     """CPU restatement of one Sim (same methods as pyro_amd.Sim)."""
+
+    @classmethod
+    def from_checkpoint(cls, path) -> "OSim":
+        """Continue a saved run: Params, clock, V-cycle count, velocity, s0,
+        s1, SOL and the level-0 phi warm start from the file."""
+        ck = read_checkpoint(path)
+        o = cls(ck["params"], ck["sol"], ck["s0"], ck["cycles"], ck["ign_until"])
+        for name in ("u", "v", "w", "s1"):
+            o.write_field(name, ck[name])
+        o.write_field(LEVEL_BASE, ck["phi"])
+        lib().osim_set_clock(o._h, float(ck["time"]), int(ck["frame"]), float(ck["ign_until"]))
+        return o
+
+    def params_blob(self) -> bytes:
+        buf = (C.c_uint8 * 576)()
+        lib().osim_get_params(self._h, buf)
+        return bytes(buf)
+
+    def save_checkpoint(self, path) -> None:
+        """Write the current state in the checkpoint format (between steps)."""
+        nx, ny, nz = self.dims
+        blob = self.params_blob()
+        frame = int(np.frombuffer(blob, np.uint32, count=1, offset=12)[0])
+        h = np.zeros(1, CKPT_HEADER)
+        h["magic"], h["version"], h["nx"], h["ny"], h["nz"] = CKPT_MAGIC, 1, nx, ny, nz
+        h["params_bytes"], h["time"], h["ign_until"] = 576, self.time, self.ign_until
+        h["frame"], h["cycles"], h["nfields"] = frame, self.cycles, len(CKPT_FIELDS)
+        parts = [h.tobytes(), blob]
+        for name in CKPT_FIELDS:
+            a = self.read_field(LEVEL_BASE if name == "phi" else name)
+            parts.append(np.ascontiguousarray(a, "<f4").tobytes())
+        Path(path).write_bytes(b"".join(parts))
 
     def __init__(self, params, sol_init, s0_init=None, pressure_cycles=4, ign_until=0.0):
         blob = params.bytes() if hasattr(params, "bytes") else bytes(params)
@@ -107,6 +183,7 @@ class OSim:
         if not self._h:
             raise RuntimeError("osim_create failed")
         self.cycles = pressure_cycles
+        self.ign_until = float(np.float32(ign_until))
 
     def close(self):
         if getattr(self, "_h", None):

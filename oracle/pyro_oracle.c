@@ -1152,6 +1152,17 @@ void osim_step(OSim *s, float dt, int want_stats) {
 
 double osim_time(const OSim *s) { return s->time; }
 
+/* The simulation's Params blob (576 bytes) as the next step starts from it. */
+void osim_get_params(const OSim *s, void *out) { memcpy(out, &s->P, sizeof(OParams)); }
+
+/* Clock of a simulation continued from a checkpoint: time, frame counter
+   (Params.frame) and the ignition end time. */
+void osim_set_clock(OSim *s, double time, uint32_t frame, float ign_until) {
+  s->time = time;
+  s->P.frame = frame;
+  s->ign_until = ign_until;
+}
+
 void osim_read_stats(const OSim *s, double *out) { memcpy(out, s->stats, sizeof(s->stats)); }
 
 static void *field_ptr(const OSim *s, int field, size_t *bytes) {

@@ -53,6 +53,8 @@ int osim_read_field(const OSim *s, int field, void *dst, size_t bytes);
 int osim_write_field(OSim *s, int field, const void *src, size_t bytes);
 void osim_read_stats(const OSim *s, double *out /* 1040 */);
 double osim_time(const OSim *s);
+void osim_get_params(const OSim *s, void *out);
+void osim_set_clock(OSim *s, double time, uint32_t frame, float ign_until);
 /* C1 harness noise: u,v,w += amp*U[-1,1] on every valid face of vel[cur]. */
 void osim_add_velocity_noise(OSim *s, float amp, uint32_t seed);
 /* render.wgsl:16-37: vol | sol | vel (3 x N x vec4 f32, before the f16 store) */

@@ -589,3 +589,39 @@ def test_render_pack_closed_forms():
     np.testing.assert_array_equal(vel[..., 1], (xc - 0.5) + 5 * yc - 4 * (zc - 0.5))
     np.testing.assert_array_equal(vel[..., 2], -(xc - 0.5) + (yc - 0.5) + 7 * zc)
     assert not vel[..., 3].any()
+
+
+# ------------------------------------------------------------------ checkpoints
+def test_checkpoint_round_trip_in_the_oracle(tmp_path):
+    """The oracle writes and reads the checkpoint format of pyro_sim_save:
+    a run continued from its own file is bit-identical to the uninterrupted
+    run; damaged files are refused."""
+    from harness import mini_scene, perturbed_state
+
+    sc = mini_scene()
+    a = oracle.OSim(sc.params(), sc.voxelize(), None, 3, sc.get("ignition.until"))
+    st = perturbed_state(sc, 9)
+    for name in ("s0", "s1", "u", "v", "w", "sol"):
+        a.write_field(name, st[name])
+    a.write_field(pa.level_field(0, "phi"), st["phi"])
+    for k in range(2):
+        a.step(0.004, False)
+    ck = tmp_path / "o.ckpt"
+    a.save_checkpoint(ck)
+    head = oracle.read_checkpoint(ck)
+    assert head["dims"] == sc.grid().dims and head["cycles"] == 3 and head["frame"] == 2
+    assert head["time"] == a.time and head["ign_until"] == F32(sc.get("ignition.until"))
+    b = oracle.OSim.from_checkpoint(ck)
+    assert b.time == a.time and b.params_blob() == a.params_blob()
+    for k in range(2):
+        a.step(0.004, k == 1)
+        b.step(0.004, k == 1)
+    for n in ("s0", "s1", "sol", "pyr", "flags", "u", "v", "w", pa.level_field(0, "phi")):
+        assert np.array_equal(a.read_field(n), b.read_field(n)), n
+    assert np.array_equal(a.read_stats_raw(), b.read_stats_raw())
+    raw = ck.read_bytes()
+    for bad in (raw[:-4], b"PYROCKPX" + raw[8:], raw + b"\0"):
+        p = tmp_path / "bad.ckpt"
+        p.write_bytes(bad)
+        with pytest.raises(ValueError):
+            oracle.read_checkpoint(p)
