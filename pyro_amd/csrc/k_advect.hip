@@ -91,21 +91,24 @@ __device__ __forceinline__ void trace(const Geo &g, const VelR &V, float k, floa
 }
 // ---------------------------------------------------------------- velocity tile
 // Each block stages the three padded velocity arrays over its brick plus a
-// halo of VT_H nodes in LDS.  A sample whose eight corners all lie in the
-// tile reads them there; any other (a trace longer than the halo) gathers
-// from global memory as before.  The arithmetic is the same either way, so
-// the results do not depend on where the corners came from.  A cell brick
-// and a face brick index the same padded grid (face (x, y, z) is the low
-// face of cell (x, y, z)), so one tile shape serves every kernel.  It
-// pays where a thread samples the velocity often (adv_vel1: 21 samples,
-// adv_vel2: 33, adv_s2: 9 plus the limiter's gathers); adv_s1 (6 samples)
-// is faster gathering directly, and so were all of them with a 3-node halo
-// (29.6 KB of LDS per 16x2x8 block: fewer blocks in flight, longer
-// staging).  Two nodes (15 KB per 16x4x4 block) cover traces up to about
-// one cell per step, the fires' range; a longer trace reads global memory.
+// halo of VT_H nodes in LDS.  A cell brick and a face brick index the same
+// padded grid (face (x, y, z) is the low face of cell (x, y, z)), so one
+// tile shape serves every kernel.  It pays where a thread samples the
+// velocity often (adv_vel1, adv_vel2, adv_s2); adv_s1 gathers directly.  Two
+// nodes (15 KB per 16x4x4 block) measured best against three (29.6 KB per
+// 16x2x8 block: fewer blocks in flight, longer staging).
+//
+// Staging also finds the largest |velocity| M of the tile.  Every velocity
+// sample is a convex combination of tile nodes, so no trace point moves more
+// than k M cells (k = dt/dx) from its start, and with k M < 1.4 every corner
+// of every sample lies in the tile (the halo needs k M <= 1.5).  Such a block
+// ("fast") reads all corners from LDS with no per-sample test; any other
+// block tests each sample and gathers the corners outside the tile from
+// global memory.  The arithmetic is the same on both paths.
 constexpr int VT_H = 2;
 constexpr int VT_X = BR_X + 2 * VT_H, VT_Y = BR_Y + 2 * VT_H, VT_Z = BR_Z + 2 * VT_H;
 constexpr int VT_N = VT_X * VT_Y * VT_Z;
+constexpr float FAST_REACH = 1.4f;
 static_assert(VT_X * VT_Y + VT_X + 1 < 256, "corner offsets fit a ds_read2 offset");
 
 struct VTile {
@@ -123,9 +126,15 @@ __device__ __forceinline__ void brick_origin(uint32_t b, const FastDiv &dbx, con
 }
 
 // Fill the tile (entries outside the padded arrays read 0 through the
-// buffer resource's range check; no sample reads them) and wait for it.
-__device__ __forceinline__ VTile stage_vtile(const Geo &g, const VelR &V, float *sm, int bx0, int by0, int bz0) {
-  const VTile T{sm, bx0 - VT_H, by0 - VT_H, bz0 - VT_H};
+// buffer resource's range check; no sample reads them), wait for it, and
+// return whether the block is fast.  M is reduced on the bit patterns of
+// |v| (unsigned order = float order for non-negative floats, and a NaN or an
+// infinity exceeds every finite value, so such a tile is never fast).
+__device__ __forceinline__ bool stage_vtile(const Geo &g, const VelR &V, float *sm, uint32_t *smax, float k,
+                                            int bx0, int by0, int bz0, VTile &T) {
+  T = VTile{sm, bx0 - VT_H, by0 - VT_H, bz0 - VT_H};
+  if (threadIdx.x == 0) *smax = 0u;
+  uint32_t mb = 0u;
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     const RB &X = c == 0 ? V.u : (c == 1 ? V.v : V.w);
@@ -135,45 +144,55 @@ __device__ __forceinline__ VTile stage_vtile(const Geo &g, const VelR &V, float 
       const int gx = T.x0 + lx, gy = T.y0 + ly, gz = T.z0 + lz;
       const bool in = (unsigned)gx < (unsigned)g.vx && (unsigned)gy < (unsigned)g.vy && (unsigned)gz <= (unsigned)g.nz;
       const uint32_t off = in ? vidx(g, gx, gy, gz) * 4u : RB_OOB;
-      sm[c * VT_N + e] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(X.r, off, 0, 0));
+      const uint32_t bits = __builtin_amdgcn_raw_buffer_load_b32(X.r, off, 0, 0);
+      sm[c * VT_N + e] = __builtin_bit_cast(float, bits);
+      const uint32_t ab = bits & 0x7fffffffu;
+      mb = ab > mb ? ab : mb;
     }
   }
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint32_t t = (uint32_t)__shfl_xor((int)mb, o, 64);
+    mb = t > mb ? t : mb;
+  }
+  __syncthreads();  // the reset of *smax is visible
+  if ((threadIdx.x & 63) == 0) atomicMax(smax, mb);
   __syncthreads();
-  return T;
+  const uint32_t m = (uint32_t)__builtin_amdgcn_readfirstlane((int)*smax);
+  return m < 0x7f800000u && k * __builtin_bit_cast(float, m) < FAST_REACH;
 }
 
-// Base node and weights of a staggered sample, as in triC / mmC.
-struct CBase {
-  int bx, by, bz;
-  float fx, fy, fz;
+// One axis of a staggered trilinear sample (tmpl_comp.wgsl:2-15): the
+// coordinate clamped to [0, h], its base node min(floor, h - 1) and weight.
+// A coordinate is a non-negative grid position minus a trace displacement
+// and staggering offsets, never -0, so the clamp's fmaxf/fminf pair equals
+// the reference's compare-select clamp (for a NaN coordinate both give the
+// bound); after it the truncation to int is the floor.  A fast block's
+// coordinates are finite, and there one v_med3 does the clamp.
+struct Ax {
+  int b;
+  float f;
 };
-__device__ __forceinline__ CBase cbase(float gx0, float gy0, float gz0, int hx, int hy, int hz) {
-  const float gx = pclamp(gx0, (float)hx), gy = pclamp(gy0, (float)hy), gz = pclamp(gz0, (float)hz);
-  CBase b;
-  b.bx = (int)gx;
-  b.by = (int)gy;
-  b.bz = (int)gz;
-  b.bx = b.bx > hx - 1 ? hx - 1 : b.bx;
-  b.by = b.by > hy - 1 ? hy - 1 : b.by;
-  b.bz = b.bz > hz - 1 ? hz - 1 : b.bz;
-  b.fx = gx - (float)b.bx;
-  b.fy = gy - (float)b.by;
-  b.fz = gz - (float)b.bz;
-  return b;
+template <bool FAST>
+__device__ __forceinline__ Ax axis(float g0, int h) {
+  const float g = FAST ? __builtin_amdgcn_fmed3f(g0, 0.0f, (float)h) : pclamp(g0, (float)h);
+  int b = (int)g;
+  b = b > h - 1 ? h - 1 : b;
+  return Ax{b, g - (float)b};
 }
 // The four x-pairs of a sample's corners: from the tile, or gathered.
-__device__ __forceinline__ void corners(const Geo &g, const VTile &T, int c, const RB &X, const CBase &b,
+template <bool FAST>
+__device__ __forceinline__ void corners(const Geo &g, const VTile &T, int c, const RB &X, int bx, int by, int bz,
                                         float2 &p00, float2 &p10, float2 &p01, float2 &p11) {
-  const int lx = b.bx - T.x0, ly = b.by - T.y0, lz = b.bz - T.z0;
-  if ((unsigned)lx < (unsigned)(VT_X - 1) && (unsigned)ly < (unsigned)(VT_Y - 1) &&
-      (unsigned)lz < (unsigned)(VT_Z - 1)) {
+  const int lx = bx - T.x0, ly = by - T.y0, lz = bz - T.z0;
+  if (FAST || ((unsigned)lx < (unsigned)(VT_X - 1) && (unsigned)ly < (unsigned)(VT_Y - 1) &&
+               (unsigned)lz < (unsigned)(VT_Z - 1))) {
     const float *q = T.s + c * VT_N + (lz * VT_Y + ly) * VT_X + lx;
     p00 = make_float2(q[0], q[1]);
     p10 = make_float2(q[VT_X], q[VT_X + 1]);
     p01 = make_float2(q[VT_X * VT_Y], q[VT_X * VT_Y + 1]);
     p11 = make_float2(q[VT_X * VT_Y + VT_X], q[VT_X * VT_Y + VT_X + 1]);
   } else {
-    const uint32_t sy = (uint32_t)g.vx, sz = (uint32_t)g.vx * (uint32_t)g.vy, i = vidx(g, b.bx, b.by, b.bz);
+    const uint32_t sy = (uint32_t)g.vx, sz = (uint32_t)g.vx * (uint32_t)g.vy, i = vidx(g, bx, by, bz);
     p00 = ld2(X, i);
     p10 = ld2(X, i + sy);
     p01 = ld2(X, i + sz);
@@ -181,23 +200,28 @@ __device__ __forceinline__ void corners(const Geo &g, const VTile &T, int c, con
   }
 }
 // triC (tmpl_comp.wgsl:2-15) of velocity component c through the tile.
-__device__ __forceinline__ float triCT(const Geo &g, const VTile &T, int c, const RB &X, float gx0, float gy0,
-                                       float gz0, int hx, int hy, int hz) {
-  const CBase b = cbase(gx0, gy0, gz0, hx, hy, hz);
+template <bool FAST>
+__device__ __forceinline__ float triT(const Geo &g, const VTile &T, int c, const RB &X, Ax ax, Ax ay, Ax az) {
   float2 p00, p10, p01, p11;
-  corners(g, T, c, X, b, p00, p10, p01, p11);
-  const float c00 = mixf_(p00.x, p00.y, b.fx);
-  const float c10 = mixf_(p10.x, p10.y, b.fx);
-  const float c01 = mixf_(p01.x, p01.y, b.fx);
-  const float c11 = mixf_(p11.x, p11.y, b.fx);
-  return mixf_(mixf_(c00, c10, b.fy), mixf_(c01, c11, b.fy), b.fz);
+  corners<FAST>(g, T, c, X, ax.b, ay.b, az.b, p00, p10, p01, p11);
+  const float c00 = mixf_(p00.x, p00.y, ax.f);
+  const float c10 = mixf_(p10.x, p10.y, ax.f);
+  const float c01 = mixf_(p01.x, p01.y, ax.f);
+  const float c11 = mixf_(p11.x, p11.y, ax.f);
+  return mixf_(mixf_(c00, c10, ay.f), mixf_(c01, c11, ay.f), az.f);
+}
+template <bool FAST>
+__device__ __forceinline__ float triCT(const Geo &g, const VTile &T, int c, const RB &X, float gx, float gy,
+                                       float gz, int hx, int hy, int hz) {
+  return triT<FAST>(g, T, c, X, axis<FAST>(gx, hx), axis<FAST>(gy, hy), axis<FAST>(gz, hz));
 }
 // mmC (tmpl_comp.wgsl:16-25) through the tile.
-__device__ __forceinline__ void mmCT(const Geo &g, const VTile &T, int c, const RB &X, float gx0, float gy0,
-                                     float gz0, int hx, int hy, int hz, float &lo, float &hi) {
-  const CBase b = cbase(gx0, gy0, gz0, hx, hy, hz);
+template <bool FAST>
+__device__ __forceinline__ void mmCT(const Geo &g, const VTile &T, int c, const RB &X, float gx, float gy, float gz,
+                                     int hx, int hy, int hz, float &lo, float &hi) {
+  const Ax ax = axis<FAST>(gx, hx), ay = axis<FAST>(gy, hy), az = axis<FAST>(gz, hz);
   float2 p[4];
-  corners(g, T, c, X, b, p[0], p[1], p[2], p[3]);
+  corners<FAST>(g, T, c, X, ax.b, ay.b, az.b, p[0], p[1], p[2], p[3]);
   float l = 1e30f, h = -1e30f;
 #pragma unroll
   for (int j = 0; j < 4; j++) {
@@ -209,60 +233,135 @@ __device__ __forceinline__ void mmCT(const Geo &g, const VTile &T, int c, const 
   lo = l;
   hi = h;
 }
+// velAt (vel.wgsl:2-13): the three components' samples share their axes
+// pairwise -- v and w the x axis (px - 1/2, bound nx - 1), u and w the y axis
+// (py - 1/2, ny - 1), u and v the z axis (pz - 1/2, nz - 1).
+template <bool FAST>
 __device__ __forceinline__ void velAtT(const Geo &g, const VTile &T, const VelR &V, float px, float py, float pz,
                                        float &ux, float &uy, float &uz) {
-  ux = triCT(g, T, 0, V.u, px, py - 0.5f, pz - 0.5f, g.nx, g.ny - 1, g.nz - 1);
-  uy = triCT(g, T, 1, V.v, px - 0.5f, py, pz - 0.5f, g.nx - 1, g.ny, g.nz - 1);
-  uz = triCT(g, T, 2, V.w, px - 0.5f, py - 0.5f, pz, g.nx - 1, g.ny - 1, g.nz);
+  const Ax x0 = axis<FAST>(px, g.nx), x1 = axis<FAST>(px - 0.5f, g.nx - 1);
+  const Ax y0 = axis<FAST>(py, g.ny), y1 = axis<FAST>(py - 0.5f, g.ny - 1);
+  const Ax z0 = axis<FAST>(pz, g.nz), z1 = axis<FAST>(pz - 0.5f, g.nz - 1);
+  ux = triT<FAST>(g, T, 0, V.u, x0, y1, z1);
+  uy = triT<FAST>(g, T, 1, V.v, x1, y0, z1);
+  uz = triT<FAST>(g, T, 2, V.w, x1, y1, z0);
 }
-__device__ __forceinline__ void traceBackT(const Geo &g, const VTile &T, const VelR &V, float k, float x, float y,
-                                           float z, float &ox, float &oy, float &oz) {
-  float u1x, u1y, u1z, u2x, u2y, u2z;
-  velAtT(g, T, V, x, y, z, u1x, u1y, u1z);
-  const float hk = 0.5f * k;
-  velAtT(g, T, V, x - hk * u1x, y - hk * u1y, z - hk * u1z, u2x, u2y, u2z);
-  ox = x - k * u2x;
-  oy = y - k * u2y;
-  oz = z - k * u2z;
+
+// Tile node (x, y, z) of component c.
+__device__ __forceinline__ float tn(const VTile &T, int c, int x, int y, int z) {
+  return T.s[c * VT_N + ((z - T.z0) * VT_Y + (y - T.y0)) * VT_X + (x - T.x0)];
 }
-__device__ __forceinline__ void trace2T(const Geo &g, const VTile &T, const VelR &V, float k, float x, float y,
-                                        float z, float &bx, float &by, float &bz, float &fx, float &fy, float &fz) {
+// The velocity at a trace's start point, where every sample weight is 0 or
+// 1/2.  Valid on a fast block (finite velocities) for the interior points
+// named below, where no clamp acts and every base node is the natural one.
+// A weight-0 lerp a + (b - a) * 0 is a up to the sign of a zero result, and
+// + - * keep "equal up to the sign of zero" for finite operands, so these
+// sums differ from the general sample at most in the sign of a zero.  The
+// start velocity only moves the start point p (p - hk u, p + hk u), and
+// every start coordinate is >= 1/2, so the trace's points are bit-identical.
+//   u face (x, y, z) at (x, y+1/2, z+1/2): 1 <= x <= nx-1, y <= ny-2, z <= nz-2
+//   v face at (x+1/2, y, z+1/2):           x <= nx-2, 1 <= y <= ny-1, z <= nz-2
+//   w face at (x+1/2, y+1/2, z):           x <= nx-2, y <= ny-2, 1 <= z <= nz-1
+//   cell centre (x+1/2, y+1/2, z+1/2):     x <= nx-2, y <= ny-2, z <= nz-2
+__device__ __forceinline__ float half_(float a, float b) { return mixf_(a, b, 0.5f); }
+__device__ __forceinline__ void start_u(const VTile &T, int x, int y, int z, float &ux, float &uy, float &uz) {
+  ux = tn(T, 0, x, y, z);
+  uy = half_(half_(tn(T, 1, x - 1, y, z), tn(T, 1, x, y, z)), half_(tn(T, 1, x - 1, y + 1, z), tn(T, 1, x, y + 1, z)));
+  uz = half_(half_(tn(T, 2, x - 1, y, z), tn(T, 2, x, y, z)), half_(tn(T, 2, x - 1, y, z + 1), tn(T, 2, x, y, z + 1)));
+}
+__device__ __forceinline__ void start_v(const VTile &T, int x, int y, int z, float &ux, float &uy, float &uz) {
+  ux = half_(half_(tn(T, 0, x, y - 1, z), tn(T, 0, x + 1, y - 1, z)), half_(tn(T, 0, x, y, z), tn(T, 0, x + 1, y, z)));
+  uy = tn(T, 1, x, y, z);
+  uz = half_(half_(tn(T, 2, x, y - 1, z), tn(T, 2, x, y, z)), half_(tn(T, 2, x, y - 1, z + 1), tn(T, 2, x, y, z + 1)));
+}
+__device__ __forceinline__ void start_w(const VTile &T, int x, int y, int z, float &ux, float &uy, float &uz) {
+  ux = half_(half_(tn(T, 0, x, y, z - 1), tn(T, 0, x + 1, y, z - 1)), half_(tn(T, 0, x, y, z), tn(T, 0, x + 1, y, z)));
+  uy = half_(half_(tn(T, 1, x, y, z - 1), tn(T, 1, x, y + 1, z - 1)), half_(tn(T, 1, x, y, z), tn(T, 1, x, y + 1, z)));
+  uz = tn(T, 2, x, y, z);
+}
+__device__ __forceinline__ void start_c(const VTile &T, int x, int y, int z, float &ux, float &uy, float &uz) {
+  ux = half_(tn(T, 0, x, y, z), tn(T, 0, x + 1, y, z));
+  uy = half_(tn(T, 1, x, y, z), tn(T, 1, x, y + 1, z));
+  uz = half_(tn(T, 2, x, y, z), tn(T, 2, x, y, z + 1));
+}
+// Start type of a trace: face component 0..2, or 3 for a cell centre.
+__device__ __forceinline__ bool start_interior(const Geo &g, int c, int x, int y, int z) {
+  switch (c) {
+    case 0: return x >= 1 && x <= g.nx - 1 && y <= g.ny - 2 && z <= g.nz - 2;
+    case 1: return x <= g.nx - 2 && y >= 1 && y <= g.ny - 1 && z <= g.nz - 2;
+    case 2: return x <= g.nx - 2 && y <= g.ny - 2 && z >= 1 && z <= g.nz - 1;
+    default: return x <= g.nx - 2 && y <= g.ny - 2 && z <= g.nz - 2;
+  }
+}
+template <bool FAST>
+__device__ __forceinline__ void start_vel(const Geo &g, const VTile &T, const VelR &V, int c, int x, int y, int z,
+                                          float px, float py, float pz, float &ux, float &uy, float &uz) {
+  if (FAST && start_interior(g, c, x, y, z)) {
+    if (c == 0) start_u(T, x, y, z, ux, uy, uz);
+    else if (c == 1) start_v(T, x, y, z, ux, uy, uz);
+    else if (c == 2) start_w(T, x, y, z, ux, uy, uz);
+    else start_c(T, x, y, z, ux, uy, uz);
+  } else {
+    velAtT<FAST>(g, T, V, px, py, pz, ux, uy, uz);
+  }
+}
+// traceBack (vel.wgsl:2-13) from the start point of type c; with the
+// forward trace (vel.wgsl:14-19) too when FWD.
+template <bool FAST, bool FWD>
+__device__ __forceinline__ void traceT(const Geo &g, const VTile &T, const VelR &V, float k, int c, int x, int y,
+                                       int z, float px, float py, float pz, float &bx, float &by, float &bz,
+                                       float &fx, float &fy, float &fz) {
   float u1x, u1y, u1z, u2x, u2y, u2z;
-  velAtT(g, T, V, x, y, z, u1x, u1y, u1z);
+  start_vel<FAST>(g, T, V, c, x, y, z, px, py, pz, u1x, u1y, u1z);
   const float hk = 0.5f * k;
-  velAtT(g, T, V, x - hk * u1x, y - hk * u1y, z - hk * u1z, u2x, u2y, u2z);
-  bx = x - k * u2x; by = y - k * u2y; bz = z - k * u2z;
-  velAtT(g, T, V, x + hk * u1x, y + hk * u1y, z + hk * u1z, u2x, u2y, u2z);
-  fx = x + k * u2x; fy = y + k * u2y; fz = z + k * u2z;
+  velAtT<FAST>(g, T, V, px - hk * u1x, py - hk * u1y, pz - hk * u1z, u2x, u2y, u2z);
+  bx = px - k * u2x;
+  by = py - k * u2y;
+  bz = pz - k * u2z;
+  if (FWD) {
+    velAtT<FAST>(g, T, V, px + hk * u1x, py + hk * u1y, pz + hk * u1z, u2x, u2y, u2z);
+    fx = px + k * u2x;
+    fy = py + k * u2y;
+    fz = pz + k * u2z;
+  }
 }
 
 // ---------------------------------------------------------------- adv_vel1
+template <bool FAST>
+__device__ __forceinline__ void adv_vel1_face(const Params *__restrict__ P, const Geo &g, const VTile &T,
+                                              const VelR &V, float k, int x, int y, int z, float *__restrict__ hu,
+                                              float *__restrict__ hv, float *__restrict__ hw) {
+  const uint32_t i = vidx(g, x, y, z);
+  float bx, by, bz, fx, fy, fz;
+  if (x <= g.nx && y <= g.ny - 1 && z <= g.nz - 1) {
+    traceT<FAST, false>(g, T, V, k, 0, x, y, z, (float)x, (float)y + 0.5f, (float)z + 0.5f, bx, by, bz, fx, fy, fz);
+    hu[i] = triCT<FAST>(g, T, 0, V.u, bx, by - 0.5f, bz - 0.5f, g.nx, g.ny - 1, g.nz - 1);
+  }
+  if (x <= g.nx - 1 && y <= g.ny && z <= g.nz - 1) {
+    traceT<FAST, false>(g, T, V, k, 1, x, y, z, (float)x + 0.5f, (float)y, (float)z + 0.5f, bx, by, bz, fx, fy, fz);
+    hv[i] = triCT<FAST>(g, T, 1, V.v, bx - 0.5f, by, bz - 0.5f, g.nx - 1, g.ny, g.nz - 1);
+  }
+  if (x <= g.nx - 1 && y <= g.ny - 1 && z <= g.nz) {
+    traceT<FAST, false>(g, T, V, k, 2, x, y, z, (float)x + 0.5f, (float)y + 0.5f, (float)z, bx, by, bz, fx, fy, fz);
+    hw[i] = triCT<FAST>(g, T, 2, V.w, bx - 0.5f, by - 0.5f, bz, g.nx - 1, g.ny - 1, g.nz);
+  }
+}
 __global__ __launch_bounds__(NT) void k_adv_vel1(const Params *__restrict__ P, Geo g, VelPtrs VP,
                                                   float *__restrict__ hu, float *__restrict__ hv,
                                                   float *__restrict__ hw) {
   __shared__ float sm[3 * VT_N];
+  __shared__ uint32_t smax;
   const uint32_t blk = (uint32_t)xcd_tile(blockIdx.x, gridDim.x);
   int x, y, z, x0, y0, z0;
   brick_cell(blk, threadIdx.x, g.dfbx, g.dfby, x, y, z);
   brick_origin(blk, g.dfbx, g.dfby, x0, y0, z0);
   const VelR V = velr_of(g, VP);
-  const VTile T = stage_vtile(g, V, sm, x0, y0, z0);
-  if (x >= g.vx || y >= g.vy || z > g.nz) return;
-  const uint32_t i = vidx(g, x, y, z);
   const float k = P->dt / P->dx;
-  float bx, by, bz;
-  if (x <= g.nx && y <= g.ny - 1 && z <= g.nz - 1) {
-    traceBackT(g, T, V, k, (float)x, (float)y + 0.5f, (float)z + 0.5f, bx, by, bz);
-    hu[i] = triCT(g, T, 0, V.u, bx, by - 0.5f, bz - 0.5f, g.nx, g.ny - 1, g.nz - 1);
-  }
-  if (x <= g.nx - 1 && y <= g.ny && z <= g.nz - 1) {
-    traceBackT(g, T, V, k, (float)x + 0.5f, (float)y, (float)z + 0.5f, bx, by, bz);
-    hv[i] = triCT(g, T, 1, V.v, bx - 0.5f, by, bz - 0.5f, g.nx - 1, g.ny, g.nz - 1);
-  }
-  if (x <= g.nx - 1 && y <= g.ny - 1 && z <= g.nz) {
-    traceBackT(g, T, V, k, (float)x + 0.5f, (float)y + 0.5f, (float)z, bx, by, bz);
-    hw[i] = triCT(g, T, 2, V.w, bx - 0.5f, by - 0.5f, bz, g.nx - 1, g.ny - 1, g.nz);
-  }
+  VTile T;
+  const bool fast = stage_vtile(g, V, sm, &smax, k, x0, y0, z0, T);
+  if (x >= g.vx || y >= g.vy || z > g.nz) return;
+  if (fast) adv_vel1_face<true>(P, g, T, V, k, x, y, z, hu, hv, hw);
+  else adv_vel1_face<false>(P, g, T, V, k, x, y, z, hu, hv, hw);
 }
 void launch_adv_vel1(const Fields &f, hipStream_t s) {
   VelPtrs V{f.va[0], f.va[1], f.va[2]};
@@ -274,22 +373,12 @@ __device__ __forceinline__ bool solidC(const Geo &g, const uint8_t *__restrict__
   return inb(g, x, y, z) && fl[sidx(g, x, y, z)] == 1;
 }
 
-__global__ __launch_bounds__(NT) void k_adv_vel2(const Params *__restrict__ P, Geo g, VelPtrs VP,
-                                                  const float *__restrict__ hu, const float *__restrict__ hv,
-                                                  const float *__restrict__ hw, float *__restrict__ ou,
-                                                  float *__restrict__ ov, float *__restrict__ ow,
-                                                  const uint8_t *__restrict__ fl) {
-  __shared__ float sm[3 * VT_N];
-  const uint32_t blk = (uint32_t)xcd_tile(blockIdx.x, gridDim.x);
-  int x, y, z, x0, y0, z0;
-  brick_cell(blk, threadIdx.x, g.dfbx, g.dfby, x, y, z);
-  brick_origin(blk, g.dfbx, g.dfby, x0, y0, z0);
-  const VelR V = velr_of(g, VP);
-  const VTile T = stage_vtile(g, V, sm, x0, y0, z0);
-  if (x >= g.vx || y >= g.vy || z > g.nz) return;
+template <bool FAST>
+__device__ __forceinline__ void adv_vel2_face(const Geo &g, const VTile &T, const VelR &V, float k, int x, int y,
+                                              int z, const RB &Hu, const RB &Hv, const RB &Hw,
+                                              float *__restrict__ ou, float *__restrict__ ov,
+                                              float *__restrict__ ow, const uint8_t *__restrict__ fl) {
   const uint32_t i = vidx(g, x, y, z);
-  const RB Hu = rb_of(hu, g.nv * 4u), Hv = rb_of(hv, g.nv * 4u), Hw = rb_of(hw, g.nv * 4u);
-  const float k = P->dt / P->dx;
   const bool sc = solidC(g, fl, x, y, z);
 #pragma unroll 1
   for (int c = 0; c < 3; c++) {
@@ -303,16 +392,36 @@ __global__ __launch_bounds__(NT) void k_adv_vel2(const Params *__restrict__ P, G
     const float offx = c == 0 ? 0.0f : 0.5f, offy = c == 1 ? 0.0f : 0.5f, offz = c == 2 ? 0.0f : 0.5f;
     const RB &Vc = c == 0 ? V.u : (c == 1 ? V.v : V.w);
     const RB &Hc = c == 0 ? Hu : (c == 1 ? Hv : Hw);
-    float px = (float)x + offx, py = (float)y + offy, pz = (float)z + offz;
+    const float px = (float)x + offx, py = (float)y + offy, pz = (float)z + offz;
     float bx, by, bz, fx, fy, fz;
-    trace2T(g, T, V, k, px, py, pz, bx, by, bz, fx, fy, fz);
+    traceT<FAST, true>(g, T, V, k, c, x, y, z, px, py, pz, bx, by, bz, fx, fy, fz);
     bx = bx - offx; by = by - offy; bz = bz - offz;
     fx = fx - offx; fy = fy - offy; fz = fz - offz;
     float lo, hi;
-    mmCT(g, T, c, Vc, bx, by, bz, hx, hy, hz, lo, hi);
-    float t = triC(g, Hc, fx, fy, fz, hx, hy, hz);
+    mmCT<FAST>(g, T, c, Vc, bx, by, bz, hx, hy, hz, lo, hi);
+    const float t = triC(g, Hc, fx, fy, fz, hx, hy, hz);
     O[i] = clampf_(ldf(Hc, i) + 0.5f * (ldf(Vc, i) - t), lo, hi);
   }
+}
+__global__ __launch_bounds__(NT) void k_adv_vel2(const Params *__restrict__ P, Geo g, VelPtrs VP,
+                                                  const float *__restrict__ hu, const float *__restrict__ hv,
+                                                  const float *__restrict__ hw, float *__restrict__ ou,
+                                                  float *__restrict__ ov, float *__restrict__ ow,
+                                                  const uint8_t *__restrict__ fl) {
+  __shared__ float sm[3 * VT_N];
+  __shared__ uint32_t smax;
+  const uint32_t blk = (uint32_t)xcd_tile(blockIdx.x, gridDim.x);
+  int x, y, z, x0, y0, z0;
+  brick_cell(blk, threadIdx.x, g.dfbx, g.dfby, x, y, z);
+  brick_origin(blk, g.dfbx, g.dfby, x0, y0, z0);
+  const VelR V = velr_of(g, VP);
+  const float k = P->dt / P->dx;
+  VTile T;
+  const bool fast = stage_vtile(g, V, sm, &smax, k, x0, y0, z0, T);
+  if (x >= g.vx || y >= g.vy || z > g.nz) return;
+  const RB Hu = rb_of(hu, g.nv * 4u), Hv = rb_of(hv, g.nv * 4u), Hw = rb_of(hw, g.nv * 4u);
+  if (fast) adv_vel2_face<true>(g, T, V, k, x, y, z, Hu, Hv, Hw, ou, ov, ow, fl);
+  else adv_vel2_face<false>(g, T, V, k, x, y, z, Hu, Hv, Hw, ou, ov, ow, fl);
 }
 void launch_adv_vel2(const Fields &f, hipStream_t s) {
   VelPtrs V{f.va[0], f.va[1], f.va[2]};
@@ -369,18 +478,14 @@ __device__ __forceinline__ float outside(const Geo &g, float x, float y, float z
                fmax_(fmax_(-0.5f - z, z - ((float)g.nz - 0.5f)), y - ((float)g.ny - 0.5f)));
 }
 
-__global__ __launch_bounds__(NT) void k_adv_s1(const Params *__restrict__ P, Geo g, VelPtrs VP,
-                                                const float4 *__restrict__ s0p, const float4 *__restrict__ s1p,
-                                                float4 *__restrict__ h0, float4 *__restrict__ h1) {
-  int x, y, z;
-  brick_cell((uint32_t)xcd_tile(blockIdx.x, gridDim.x), threadIdx.x, g.dcbx, g.dcby, x, y, z);
-  if (x >= g.nx || y >= g.ny || z >= g.nz) return;
-  const uint32_t i = sidx(g, x, y, z);
-  const VelR V = velr_of(g, VP);
-  const RB s0 = rb_of(s0p, g.n * 16u), s1 = rb_of(s1p, g.n * 16u);
-  const float k = P->dt / P->dx;
-  float bx, by, bz;
-  trace<-1>(g, V, k, (float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f, bx, by, bz);
+// adv_s1 (advect.wgsl:72-91): the back trace from the cell centre and the
+// sample of both fields at its foot, ambient beyond the open boundaries.
+// With S1_TILE the velocity comes through the LDS tile like the other three
+// kernels; without it every velocity sample is gathered from global memory.
+constexpr bool S1_TILE = true;
+__device__ __forceinline__ void adv_s1_out(const Params *__restrict__ P, const Geo &g, uint32_t i, float bx,
+                                           float by, float bz, const RB &s0, const RB &s1, float4 *__restrict__ h0,
+                                           float4 *__restrict__ h1) {
   bx = bx - 0.5f; by = by - 0.5f; bz = bz - 0.5f;
   SSample s = sbase(g, bx, by, bz);
   float w = clampf_(outside(g, bx, by, bz), 0.0f, 1.0f);
@@ -388,6 +493,33 @@ __global__ __launch_bounds__(NT) void k_adv_s1(const Params *__restrict__ P, Geo
   float4 c = triS(g, s1, s);
   h0[i] = make_float4(mixf_(a.x, P->t0, w), mixf_(a.y, 0.0f, w), mixf_(a.z, P->yo2, w), mixf_(a.w, 0.0f, w));
   h1[i] = make_float4(mixf_(c.x, 0.0f, w), mixf_(c.y, 0.0f, w), mixf_(c.z, 0.0f, w), mixf_(c.w, 0.0f, w));
+}
+__global__ __launch_bounds__(NT) void k_adv_s1(const Params *__restrict__ P, Geo g, VelPtrs VP,
+                                                const float4 *__restrict__ s0p, const float4 *__restrict__ s1p,
+                                                float4 *__restrict__ h0, float4 *__restrict__ h1) {
+  const uint32_t blk = (uint32_t)xcd_tile(blockIdx.x, gridDim.x);
+  int x, y, z;
+  brick_cell(blk, threadIdx.x, g.dcbx, g.dcby, x, y, z);
+  const VelR V = velr_of(g, VP);
+  const RB s0 = rb_of(s0p, g.n * 16u), s1 = rb_of(s1p, g.n * 16u);
+  const float k = P->dt / P->dx;
+  const float px = (float)x + 0.5f, py = (float)y + 0.5f, pz = (float)z + 0.5f;
+  float bx, by, bz, fx, fy, fz;
+  if (S1_TILE) {
+    __shared__ float sm[3 * VT_N];
+    __shared__ uint32_t smax;
+    int x0, y0, z0;
+    brick_origin(blk, g.dcbx, g.dcby, x0, y0, z0);
+    VTile T;
+    const bool fast = stage_vtile(g, V, sm, &smax, k, x0, y0, z0, T);
+    if (x >= g.nx || y >= g.ny || z >= g.nz) return;
+    if (fast) traceT<true, false>(g, T, V, k, 3, x, y, z, px, py, pz, bx, by, bz, fx, fy, fz);
+    else traceT<false, false>(g, T, V, k, 3, x, y, z, px, py, pz, bx, by, bz, fx, fy, fz);
+  } else {
+    if (x >= g.nx || y >= g.ny || z >= g.nz) return;
+    trace<-1>(g, V, k, px, py, pz, bx, by, bz);
+  }
+  adv_s1_out(P, g, sidx(g, x, y, z), bx, by, bz, s0, s1, h0, h1);
 }
 void launch_adv_s1(const Fields &f, hipStream_t s) {
   VelPtrs V{f.va[0], f.va[1], f.va[2]};
@@ -398,25 +530,15 @@ __device__ __forceinline__ float mcl(float r, float s, float t, float lo, float 
   return clampf_(r + 0.5f * (s - t), lo, hi);
 }
 
-__global__ __launch_bounds__(NT) void k_adv_s2(const Params *__restrict__ P, Geo g, VelPtrs VP,
-                                                const float4 *__restrict__ s0p, const float4 *__restrict__ s1p,
-                                                const float4 *__restrict__ h0p, const float4 *__restrict__ h1p,
-                                                float4 *__restrict__ o0, float4 *__restrict__ o1) {
-  __shared__ float sm[3 * VT_N];
-  const uint32_t blk = (uint32_t)xcd_tile(blockIdx.x, gridDim.x);
-  int x, y, z, x0, y0, z0;
-  brick_cell(blk, threadIdx.x, g.dcbx, g.dcby, x, y, z);
-  brick_origin(blk, g.dcbx, g.dcby, x0, y0, z0);
-  const VelR V = velr_of(g, VP);
-  const VTile T = stage_vtile(g, V, sm, x0, y0, z0);
-  if (x >= g.nx || y >= g.ny || z >= g.nz) return;
+// adv_s2 (advect.wgsl:93-125).
+template <bool FAST>
+__device__ __forceinline__ void adv_s2_cell(const Geo &g, const VTile &T, const VelR &V, float k, int x, int y,
+                                            int z, const RB &s0, const RB &s1, const RB &h0, const RB &h1,
+                                            float4 *__restrict__ o0, float4 *__restrict__ o1) {
   const uint32_t i = sidx(g, x, y, z);
-  const RB s0 = rb_of(s0p, g.n * 16u), s1 = rb_of(s1p, g.n * 16u), h0 = rb_of(h0p, g.n * 16u),
-           h1 = rb_of(h1p, g.n * 16u);
-  const float k = P->dt / P->dx;
-  float px = (float)x + 0.5f, py = (float)y + 0.5f, pz = (float)z + 0.5f;
+  const float px = (float)x + 0.5f, py = (float)y + 0.5f, pz = (float)z + 0.5f;
   float bx, by, bz, fx, fy, fz;
-  trace2T(g, T, V, k, px, py, pz, bx, by, bz, fx, fy, fz);
+  traceT<FAST, true>(g, T, V, k, 3, x, y, z, px, py, pz, bx, by, bz, fx, fy, fz);
   bx = bx - 0.5f; by = by - 0.5f; bz = bz - 0.5f;
   fx = fx - 0.5f; fy = fy - 0.5f; fz = fz - 0.5f;
   float4 r0 = ld4(h0, i), r1 = ld4(h1, i);
@@ -434,6 +556,26 @@ __global__ __launch_bounds__(NT) void k_adv_s2(const Params *__restrict__ P, Geo
   }
   o0[i] = r0;
   o1[i] = r1;
+}
+__global__ __launch_bounds__(NT) void k_adv_s2(const Params *__restrict__ P, Geo g, VelPtrs VP,
+                                                const float4 *__restrict__ s0p, const float4 *__restrict__ s1p,
+                                                const float4 *__restrict__ h0p, const float4 *__restrict__ h1p,
+                                                float4 *__restrict__ o0, float4 *__restrict__ o1) {
+  __shared__ float sm[3 * VT_N];
+  __shared__ uint32_t smax;
+  const uint32_t blk = (uint32_t)xcd_tile(blockIdx.x, gridDim.x);
+  int x, y, z, x0, y0, z0;
+  brick_cell(blk, threadIdx.x, g.dcbx, g.dcby, x, y, z);
+  brick_origin(blk, g.dcbx, g.dcby, x0, y0, z0);
+  const VelR V = velr_of(g, VP);
+  const float k = P->dt / P->dx;
+  VTile T;
+  const bool fast = stage_vtile(g, V, sm, &smax, k, x0, y0, z0, T);
+  if (x >= g.nx || y >= g.ny || z >= g.nz) return;
+  const RB s0 = rb_of(s0p, g.n * 16u), s1 = rb_of(s1p, g.n * 16u), h0 = rb_of(h0p, g.n * 16u),
+           h1 = rb_of(h1p, g.n * 16u);
+  if (fast) adv_s2_cell<true>(g, T, V, k, x, y, z, s0, s1, h0, h1, o0, o1);
+  else adv_s2_cell<false>(g, T, V, k, x, y, z, s0, s1, h0, h1, o0, o1);
 }
 void launch_adv_s2(const Fields &f, hipStream_t s) {
   VelPtrs V{f.va[0], f.va[1], f.va[2]};
