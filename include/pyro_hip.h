@@ -89,6 +89,18 @@ int pyro_sim_create(const void *params, size_t params_bytes, const float *sol_in
 struct PyroShape;
 int pyro_voxelize(const struct PyroShape *shapes, size_t count, const uint32_t dims[3], float dx, float t0,
                   uint32_t seed, float *dst, int dst_on_device);
+/* pyro_voxelize into device memory without allocating: no counterpart in the
+ * reference.  For callers that cannot call hipMalloc (code near graph
+ * capture, serving loops, harnesses that forbid allocation).  The shape list
+ * is staged in a fixed array of PYRO_VOXELIZE_NOALLOC_MAX_SHAPES entries that
+ * each GPU holds until the process ends (320 KiB); more shapes are rejected
+ * (PYRO_E_INVALID), as is a dst_device that is not device memory.  Same
+ * kernel and bits as pyro_voxelize; runs on the null stream and returns
+ * after the kernel.  Every call on one GPU shares that GPU's array: calls
+ * that overlap on one GPU (other host threads) race. */
+#define PYRO_VOXELIZE_NOALLOC_MAX_SHAPES 4096
+int pyro_voxelize_noalloc(const struct PyroShape *shapes, size_t count, const uint32_t dims[3], float dx,
+                          float t0, uint32_t seed, float *dst_device);
 /* A simulation's SOL from primitives, on the device: both SOL buffers and the
  * solid flags (as sol_init does in pyro_sim_create, which also accepts
  * sol_init = NULL: no solids). */

@@ -60,6 +60,7 @@ def hip() -> C.CDLL:
         lib.pyro_sim_sync.argtypes = [P]
         lib.pyro_voxelize.argtypes = [P, sz, P, f32, f32, u32, P, i32]
         lib.pyro_sim_voxelize.argtypes = [P, P, sz, f32, u32]
+        lib.pyro_voxelize_noalloc.argtypes = [P, sz, P, f32, f32, u32, P]
         lib.pyro_sim_render_pack.argtypes = [P, P, sz]
         lib.pyro_sim_render_wait.argtypes = [P]
         lib.pyro_host_alloc.argtypes = [sz, C.POINTER(P)]

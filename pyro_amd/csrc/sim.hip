@@ -11,6 +11,7 @@
 // (vel[cur], sol[sol_cur]) and by want_stats, so at most 8 exist.
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -241,6 +242,37 @@ __global__ void k_nonfinite(const float *p, size_t n, int *flag) {
 
 }  // namespace
 
+// Shape staging of pyro_voxelize_noalloc: one array per GPU, held until the
+// process ends, shared by every call on that GPU.  Its host-side name is not
+// a device address: host code takes the address from hipGetSymbolAddress,
+// once per device with that device current (scratch_ptr), and passes that
+// pointer to copies and kernels.
+constexpr size_t VOX_NOALLOC_MAX_SHAPES = PYRO_VOXELIZE_NOALLOC_MAX_SHAPES;  // x 80 B = 320 KiB
+static __device__ __align__(256) PyroShape scratch[VOX_NOALLOC_MAX_SHAPES];
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[MAX_DEVICES] = {};
+
+// The current device's copy of `scratch`.
+int scratch_ptr(void **out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(PYRO_E_NODEVICE, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= MAX_DEVICES) return fail(PYRO_E_INVALID, "device index " + std::to_string(dev) + " too large");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(PYRO_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[dev] = p;
+  }
+  *out = g_scratch[dev];
+  return 0;
+}
+}  // namespace
+
 extern "C" {
 
 const char *pyro_last_error(void) { return g_err.c_str(); }
@@ -381,6 +413,34 @@ int pyro_voxelize(const struct PyroShape *shapes, size_t count, const uint32_t d
   if (!dst_on_device && hipMemcpy(dst, dout, (size_t)n64 * 16, hipMemcpyDeviceToHost) != hipSuccess)
     return done(fail(PYRO_E_HIP, "pyro_voxelize: download failed"));
   return done(0);
+}
+
+int pyro_voxelize_noalloc(const struct PyroShape *shapes, size_t count, const uint32_t dims[3], float dx, float t0,
+                          uint32_t seed, float *dst_device) {
+  if (!dims || !dst_device || (count && !shapes)) return fail(PYRO_E_INVALID, "pyro_voxelize_noalloc: null argument");
+  const uint64_t n64 = (uint64_t)dims[0] * dims[1] * dims[2];
+  if (!n64 || n64 * 16ull >= (1ull << 32)) return fail(PYRO_E_INVALID, "pyro_voxelize_noalloc: bad grid size");
+  if (count > VOX_NOALLOC_MAX_SHAPES)
+    return fail(PYRO_E_INVALID, "pyro_voxelize_noalloc: " + std::to_string(count) + " shapes, at most " +
+                                    std::to_string(VOX_NOALLOC_MAX_SHAPES));
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(PYRO_E_NODEVICE, "no HIP device available");
+  // the kernel writes through dst_device: it must be device memory
+  hipPointerAttribute_t at;
+  if (hipPointerGetAttributes(&at, dst_device) != hipSuccess || at.type != hipMemoryTypeDevice) {
+    (void)hipGetLastError();
+    return fail(PYRO_E_INVALID, "pyro_voxelize_noalloc: dst_device is not device memory");
+  }
+  void *sh = nullptr;
+  int rc = scratch_ptr(&sh);
+  if (rc) return rc;
+  if (count && hipMemcpy(sh, shapes, count * sizeof(PyroShape), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(PYRO_E_HIP, "pyro_voxelize_noalloc: shape upload failed");
+  if (launch_voxelize(static_cast<const PyroShape *>(sh), (int)count, dims, dx, t0, seed,
+                      reinterpret_cast<float4 *>(dst_device), nullptr) ||
+      hipStreamSynchronize(nullptr) != hipSuccess)
+    return fail(PYRO_E_HIP, "pyro_voxelize_noalloc: kernel failed");
+  return 0;
 }
 
 int pyro_sim_voxelize(PyroSim *s, const struct PyroShape *shapes, size_t count, float t0, uint32_t seed) {
