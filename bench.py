@@ -9,17 +9,31 @@ plus U[-1e-3, 1e-3] m/s velocity noise (seed 0).  ``value`` = cells x steps /
 wall seconds, inputs resident in HBM, timed between two device syncs.
 
 Extra fields:
-  roofline      -- the level-0 red-black sweep (``k_mg_sweep<true, true>``),
-                   the kernel with the most launches and bytes per step
-                   (16 per step): algorithmic bytes = 16 B per level-0 cell
-                   per sweep (read phi, b, flags and write phi once, in the
-                   reference's f32/u32 layout), divided by its average launch
-                   time (HIP events on the sim stream).  ``traffic`` is the
-                   HBM bytes per launch measured by rocprofv3 FETCH_SIZE +
-                   WRITE_SIZE (profiles/mg_sweep_l0_pool_2cm_traffic.json,
-                   scaled by the copy-kernel calibration), when present.
+  roofline      -- the kernel with the largest share of the step time (average
+                   launch time x launches per step), in the contract's form:
+                   algorithmic bytes per launch (the reference layout's
+                   compulsory bytes per cell, below, x the cells one launch
+                   processes) / average launch time (HIP events on the sim
+                   stream, repeated launches), against the 8 TB/s HBM peak;
+                   ``traffic`` is the HBM bytes per launch measured by
+                   rocprofv3 FETCH_SIZE + WRITE_SIZE (profiles/
+                   pool_2cm_kernel_traffic.json, scaled by the copy-kernel
+                   calibration), when present.
+  roofline_kernels -- the same for every kernel of the step, sorted by share:
+                   solid 52, curl 28, adv_vel1 24, adv_vel2 40, adv_s1 76,
+                   adv_s2 108, react 84, forces 76, divergence 36, project 32
+                   B per cell; the level-0 red-black sweep 24 B per cell
+                   (12 B per colour pass, two passes per launch, 16 launches
+                   per step); restriction 13.5 and prolongation 13 B per
+                   fine cell; the coarsest-level solve 80 x 24 B per coarse
+                   cell (80 SOR iterations in one launch, 4 per step).  Timed
+                   on a scratch simulation restored from a checkpoint of the
+                   burning pool, so the timed runs are not disturbed.
   cpu_baseline  -- the CPU restatement in oracle/ (kind "port") timed on this
-                   host on a bounded sample of the same workload.
+                   host on a bounded sample of the same workload, on the
+                   host's CPU share (OMP_NUM_THREADS, 16 on the GPU box,
+                   ``host_nproc`` beside it) and on one core.
+  pool_4cm      -- the small pool (80 x 160 x 80) on the GPU, 20 timed steps.
   stage_ms      -- per-stage time of one step (HIP events, direct launches).
   pressure      -- time to a relative residual of 1e-4 (sqrt of sum r^2 over
                    sum b^2 on fluid cells, FP64) on the next step's pressure
@@ -31,7 +45,8 @@ Extra fields:
                    V-cycles to 1e-4, FP64-residual refinement and MGPCG to
                    1e-6, with the error against an FP64 solution; the
                    bonfire at 10 cm stepped from one checkpointed mid-fire
-                   state with each pressure method to 1e-4.
+                   state with each pressure method to 1e-4, and the FP64
+                   mass balance of its fuel over those steps.
 """
 from __future__ import annotations
 
@@ -55,7 +70,20 @@ HBM_PEAK_GBS = 8000.0
 SWEEP_BYTES_PER_CELL = 24.0
 # renderer pack: read s0, s1, sol 48 + flags 4 + three velocity faces 12, write three half4 24
 RENDER_PACK_BYTES_PER_CELL = 88.0
-TRAFFIC_FILE = ROOT / "profiles" / "mg_sweep_l0_pool_2cm_traffic.json"
+TRAFFIC_FILE = ROOT / "profiles" / "pool_2cm_kernel_traffic.json"
+# compulsory bytes per cell of one launch in the reference layout, and
+# launches per step, of the single-kernel stages (kernel names as rocprof
+# shows them for a step without statistics)
+STAGE_KERNELS = {
+    "solid": ("k_solid<false>", 52.0), "curl": ("k_curl<false>", 28.0), "adv_vel1": ("k_adv_vel1", 24.0),
+    "adv_vel2": ("k_adv_vel2", 40.0), "adv_s1": ("k_adv_s1", 76.0), "adv_s2": ("k_adv_s2", 108.0),
+    "react": ("k_react<false>", 84.0), "forces": ("k_forces", 76.0), "divergence": ("k_divergence", 36.0),
+    "project": ("k_project", 32.0),
+}
+RESTRICT_BYTES_PER_FINE_CELL = 13.5
+PROLONG_BYTES_PER_FINE_CELL = 13.0
+COARSE_ITERS = 80
+COARSE_BYTES_PER_CELL_ITER = 24.0
 # U-Net FLOPs per level-0 voxel: 2*27*cin*cout per conv at its level's
 # resolution (1/8 per halving) plus the 1x1x1 head: level 0 2*27*(2*16 + 16*16
 # + 48*16) + 2*16, level 1 2*27*(16*32 + 32*32 + 96*32)/8, level 2
@@ -99,24 +127,6 @@ def gpu_run(args):
     log(f"[bench] timed {args.steps} steps: {wall:.3f} s")
     sim.check_finite()
     st = sim.read_stats()
-
-    # per-stage breakdown (direct launches, HIP events on the sim stream)
-    stage_ms = {}
-    sim.begin_step(args.dt, False)
-    for stg in pa.STAGES:
-        if stg == "resid":
-            continue
-        stage_ms[stg] = round(sim.time_stage(stg, 1), 4)
-    sim.end_step()
-    sim.sync()
-    # dominant kernel: fine-level colour pass, averaged over many launches
-    reps = 200
-    sim.time_kernel("mg_smooth", 0, 10)
-    smooth_ms = sim.time_kernel("mg_smooth", 0, reps)
-    coarse_ms = sim.time_kernel("mg_coarse", 0, 20)
-    algo_bytes = SWEEP_BYTES_PER_CELL * cells
-    achieved = algo_bytes / (smooth_ms * 1e-3) / 1e9
-    log(f"[bench] k_mg_sweep<true,true,true> avg {smooth_ms * 1e3:.1f} us -> {achieved:.0f} GB/s algorithmic")
     # achievable streaming bandwidth on this device (plain copy of 2 x 256 MiB)
     copy_ms = pa.sim.debug_copy(64 << 20, 20)
     copy_gbs = 2 * (64 << 20) * 4 / (copy_ms * 1e-3) / 1e9
@@ -148,6 +158,14 @@ def gpu_run(args):
                    "step_resid": sb.resid}
         log(f"[bench] pool from t = {burning['t_start_s']} s: {burning['ms_per_step']:.3f} ms/step, "
             f"HRR {sb.hrr / 1e3:.0f} kW")
+    # per-kernel times on a scratch copy of this state (the runs above and
+    # below are not disturbed)
+    import tempfile
+
+    ck = Path(tempfile.mkdtemp()) / "pool.ckpt"
+    sim.save(ck)
+    kern, stage_ms = kernel_rooflines(scene, ck, args)
+    ck.unlink()
     handoff = render_handoff(sim, args) if args.burn_time > 0 else None
     net, learn = (None, None) if args.no_pressure else learned_net(args)
     frame = None if args.no_pressure else pressure_frame(sim, args, p["dx"], net)
@@ -156,9 +174,101 @@ def gpu_run(args):
         learn["forward_tflops_pool_2cm"] = UNET_FLOP_PER_VOXEL * cells / (learn["forward_ms_pool_2cm"] * 1e-3) / 1e12
         log(f"[bench] U-Net forward on {nx}x{ny}x{nz}: {learn['forward_ms_pool_2cm']:.3f} ms, "
             f"{learn['forward_tflops_pool_2cm']:.0f} TFLOP/s")
-    return dict(cells=cells, wall=wall, stats=st, stage_ms=stage_ms, smooth_ms=smooth_ms,
-                coarse_ms=coarse_ms, achieved=achieved, algo_bytes=algo_bytes, dims=(nx, ny, nz),
+    return dict(cells=cells, wall=wall, stats=st, stage_ms=stage_ms, kernels=kern, dims=(nx, ny, nz),
                 copy_gbs=copy_gbs, frame=frame, learned=learn, net=net, burning=burning, handoff=handoff)
+
+
+def load_traffic():
+    """{rocprof kernel name: HBM bytes per launch} from the committed
+    FETCH_SIZE / WRITE_SIZE summary, when present."""
+    if not TRAFFIC_FILE.exists():
+        return {}
+    try:
+        d = json.loads(TRAFFIC_FILE.read_text())
+        return {k: v["hbm_bytes_per_launch"] for k, v in d.get("kernels", {}).items()}
+    except (ValueError, OSError, KeyError, TypeError):
+        return {}
+
+
+def kernel_rooflines(scene, ck, args):
+    """Average launch time of every kernel of the step, on a scratch
+    simulation restored from checkpoint `ck`, against its compulsory bytes.
+    Each stage is launched once to warm up, then `reps` times between two
+    HIP events on the simulation's stream (the average includes the short
+    gaps between back-to-back launches)."""
+    import pyro_amd as pa
+
+    tsim = pa.Sim(scene.params(), scene.voxelize(), None, 4, scene.get("ignition.until"))
+    tsim.load(ck)
+    nx, ny, nz = tsim.dims
+    cells = nx * ny * nz
+    reps = 5
+    stage_ms = {}
+    tsim.begin_step(args.dt, False)
+    for stg in pa.STAGES:
+        if stg == "resid":
+            continue
+        tsim.time_stage(stg, 1)
+        stage_ms[stg] = tsim.time_stage(stg, 1 if stg == "vcycles" else reps)
+    tsim.end_step()
+    tsim.sync()
+    sweep = tsim.time_kernel("mg_smooth", 0, 200)
+    restrict0 = tsim.time_kernel("mg_restrict", 0, 50)
+    prolong0 = tsim.time_kernel("mg_prolong", 0, 50)
+    coarse = tsim.time_kernel("mg_coarse", 0, 20)
+    cd = tsim.level_dims(tsim.levels - 1)
+    ncoarse = cd[0] * cd[1] * cd[2]
+    tsim.close()
+    traffic = load_traffic()
+    ents = []
+
+    def add(kernel, note, algo_bytes, avg_ms, per_step):
+        ach = algo_bytes / (avg_ms * 1e-3) / 1e9
+        tr = traffic.get(kernel)
+        ents.append({"kernel": kernel, "note": note, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": tr,
+                     "traffic_GBs": tr / (avg_ms * 1e-3) / 1e9 if tr else None,
+                     "algo_bytes_per_launch": algo_bytes, "avg_launch_us": avg_ms * 1e3,
+                     "launches_per_step": per_step, "ms_per_step": avg_ms * per_step})
+
+    for stg, (kname, bpc) in STAGE_KERNELS.items():
+        add(kname, f"stage {stg}, {bpc:g} B per cell", bpc * cells, stage_ms[stg], 1)
+    add("k_mg_sweep<true, true, true>", "level-0 red-black sweep, both colours, 24 B per cell", 24.0 * cells,
+        sweep, 16)
+    add("k_mg_restrict", "level 0 -> 1, 13.5 B per fine cell", RESTRICT_BYTES_PER_FINE_CELL * cells, restrict0, 4)
+    add("k_mg_prolong", "level 1 -> 0, 13 B per fine cell", PROLONG_BYTES_PER_FINE_CELL * cells, prolong0, 4)
+    coarse_name = next((k for k in traffic if k.startswith("k_mg_coarse_z")), "k_mg_coarse_z")
+    add(coarse_name, f"coarsest level {cd[0]}x{cd[1]}x{cd[2]}: {COARSE_ITERS} SOR iterations in one "
+        "workgroup (a dependency chain on one CU), 24 B per cell and iteration",
+        COARSE_ITERS * COARSE_BYTES_PER_CELL_ITER * ncoarse, coarse, 4)
+    ents.sort(key=lambda e: -e["ms_per_step"])
+    for e in ents:
+        log(f"[bench] {e['kernel']}: {e['avg_launch_us']:.1f} us x {e['launches_per_step']} = "
+            f"{e['ms_per_step']:.3f} ms/step, {e['achieved']:.0f} GB/s (frac {e['frac']:.3f})")
+    return ents, {k: round(v, 4) for k, v in stage_ms.items()}
+
+
+def pool_4cm(args):
+    """The small pool (80 x 160 x 80 at 4 cm): 20 hipGraph steps after 10."""
+    import pyro_amd as pa
+
+    scene = make_pool(0.04)
+    nx, ny, nz = scene.grid().dims
+    sim = pa.Sim(scene.params(), scene.voxelize(), None, 4, scene.get("ignition.until"))
+    sim.add_velocity_noise(1e-3, 0)
+    for k in range(10):
+        sim.step(args.dt, k % 10 == 9)
+    sim.sync()
+    t0 = time.perf_counter()
+    for k in range(20):
+        sim.step(args.dt, k % 10 == 9)
+    sim.sync()
+    wall = time.perf_counter() - t0
+    sim.close()
+    out = {"grid": [nx, ny, nz], "steps": 20, "ms_per_step": 1e3 * wall / 20,
+           "cell_updates_per_s": nx * ny * nz * 20 / wall}
+    log(f"[bench] pool 4 cm: {out}")
+    return out
 
 
 def render_handoff(sim, args):
@@ -317,7 +427,8 @@ def pressure_c3(args):
     if args.c3_cpu:
         import oracle
 
-        oracle.set_threads(min(16, os.cpu_count() or 1))
+        nproc = os.cpu_count() or 1
+        oracle.set_threads(max(1, min(int(os.environ.get("OMP_NUM_THREADS", nproc)), nproc)))
         t0 = time.perf_counter()
         ref, its, rel = oracle.poisson_cg64(flags, b, 1.0 / n, tol=1e-10, max_iter=20000)
         P.set(b=b, phi=zero)
@@ -334,7 +445,10 @@ def cpu_baseline(args, dims):
     """The oracle on this host: full steps of the same workload (bounded sample)."""
     import oracle
 
-    threads = min(16, os.cpu_count() or 1)
+    nproc = os.cpu_count() or 1
+    # this host's CPU share: OMP_NUM_THREADS where the box sets it (16 per
+    # GPU on the MI355X boxes, whatever nproc shows), else every core
+    threads = max(1, min(int(os.environ.get("OMP_NUM_THREADS", nproc)), nproc))
     oracle.set_threads(threads)
     scene = make_pool(args.cell)
     o = oracle.OSim(scene.params(), scene.voxelize(), None, 4, scene.get("ignition.until"))
@@ -350,9 +464,10 @@ def cpu_baseline(args, dims):
         if el > args.cpu_seconds or n >= 50:
             break
     cells = dims[0] * dims[1] * dims[2]
-    out = dict(value=cells * n / el, unit="cell-updates/s", cores=threads, kind="port",
+    out = dict(value=cells * n / el, unit="cell-updates/s", cores=threads, kind="port", host_nproc=nproc,
                sample=f"{n} full steps (after 1 warm step) of the same {dims[0]}x{dims[1]}x{dims[2]} pool "
-                      f"workload on the oracle (OpenMP, {threads} threads)")
+                      f"workload on the oracle (OpenMP, {threads} threads: the host's CPU share, "
+                      f"nproc {nproc})")
     # the small pool (4 cm) on all threads and on one
     del o
     c1 = make_pool(0.04)
@@ -445,7 +560,40 @@ def bonfire_c5(args, net):
         out[key] = d
         log(f"[bench] bonfire {key}: {d}")
     P.close()
+    out["fuel_balance"] = fuel_balance(sim, ck, s.params(), dt, args.c5_steps)
     ck.unlink()
+    return out
+
+
+def fuel_balance(sim, ck, p, dt, steps):
+    """FP64 mass balance of the fuel over `steps` reference steps from the
+    checkpoint, statistics on every step: the volatiles released by the
+    solid (slot 2, kg/s, times dt) against the change of gas-phase fuel
+    (slot 6), the fuel burnt (heat release / heat of combustion), and the
+    change of condensed smoke (slot 8, formed from unburnt vapour).  The
+    rest is fuel carried out of the domain or absorbed by the sponge layers
+    (and the advection's non-conservation).  On the solid side: the change
+    of solid mass (slot 7) against the volatiles; the rest is char burnt and
+    the mass of cells that burnt out."""
+    sim.load(ck)
+    sim.step(dt, True)
+    s0 = sim.read_stats()
+    released = burnt = 0.0
+    t0 = time.perf_counter()
+    for k in range(steps):
+        sim.step(dt, True)
+        st = sim.read_stats()
+        released += st.pyro * dt
+        burnt += st.hrr / p["dhc"] * dt
+    wall = time.perf_counter() - t0
+    d_fuel, d_smoke, d_solid = st.fuel - s0.fuel, st.smoke - s0.smoke, st.solid - s0.solid
+    rest = released - burnt - d_fuel - d_smoke
+    out = {"steps": steps, "seconds_simulated": steps * dt, "volatiles_released_kg": released,
+           "fuel_burnt_kg": burnt, "gas_fuel_change_kg": d_fuel, "smoke_change_kg": d_smoke,
+           "outflow_and_sponge_kg": rest, "outflow_frac_of_released": rest / released if released else None,
+           "solid_change_kg": d_solid, "char_and_burnout_kg": -d_solid - released,
+           "wall_s_with_stats_every_step": wall}
+    log(f"[bench] bonfire fuel balance: {out}")
     return out
 
 
@@ -521,18 +669,14 @@ def main():
     if int(os.environ.get("RANK", "0")) != 0:
         sys.exit(0)
     r = gpu_run(args)
+    c1 = pool_4cm(args)
     c3 = None if args.no_pressure else pressure_c3(args)
     c5 = None if (args.no_pressure or args.c5_steps <= 0) else bonfire_c5(args, r.get("net"))
     vox = voxelize_leg()
     cpu = None if args.no_cpu else cpu_baseline(args, r["dims"])
     nx, ny, nz = r["dims"]
     value = r["cells"] * args.steps / r["wall"]
-    traffic = None
-    if TRAFFIC_FILE.exists():
-        try:
-            traffic = json.loads(TRAFFIC_FILE.read_text()).get("hbm_bytes_per_launch")
-        except (ValueError, OSError):
-            traffic = None
+    top = dict(r["kernels"][0])
     out = {
         "metric": "cell-updates/s at 8.2M-cell step; time to 1e-4 pressure residual, learned vs plain MG",
         "value": value,
@@ -554,21 +698,11 @@ def main():
             "pressure_cycles": 4,
             "dt": args.dt,
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "k_mg_sweep<true,true,true> (level-0 red-black sweep, both colours)",
-            "achieved": r["achieved"],
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": r["achieved"] / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "traffic_GBs": (traffic / (r["smooth_ms"] * 1e-3) / 1e9) if traffic else None,
-            "algo_bytes_per_launch": r["algo_bytes"],
-            "avg_launch_us": r["smooth_ms"] * 1e3,
-        },
+        "roofline": top,
+        "roofline_kernels": r["kernels"],
         "cpu_baseline": cpu,
         "stage_ms": r["stage_ms"],
-        "coarse_solve_ms": r["coarse_ms"],
+        "pool_4cm": c1,
         "pressure": {"pool_frame": r["frame"], "standalone": c3, "learned_training": r["learned"],
                      "bonfire_10cm": c5},
         "pool_burning": r["burning"],
