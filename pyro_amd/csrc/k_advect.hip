@@ -114,7 +114,21 @@ static_assert(VT_X * VT_Y + VT_X + 1 < 256, "corner offsets fit a ds_read2 offse
 struct VTile {
   const float *s;  // component c at s + c * VT_N, x fastest, then y, then z
   int x0, y0, z0;  // global node index of the tile's first entry
+  int k0;          // x0 + VT_X y0 + VT_X VT_Y z0
 };
+// Tile entry of global node (x, y, z) (any node of the tile; x, y, z >= 0):
+// full-rate 24-bit multiplies (node indices are far below 2^24).
+__device__ __forceinline__ int tli(const VTile &T, int x, int y, int z) {
+  return (int)(__umul24((uint32_t)z, (uint32_t)(VT_X * VT_Y)) + __umul24((uint32_t)y, (uint32_t)VT_X) + (uint32_t)x) -
+         T.k0;
+}
+// Float bounds of the staggered samples, converted once per thread.
+struct Bnd {
+  float nx, ny, nz, nx1, ny1, nz1;
+};
+__device__ __forceinline__ Bnd bnd_of(const Geo &g) {
+  return Bnd{(float)g.nx, (float)g.ny, (float)g.nz, (float)(g.nx - 1), (float)(g.ny - 1), (float)(g.nz - 1)};
+}
 
 __device__ __forceinline__ void brick_origin(uint32_t b, const FastDiv &dbx, const FastDiv &dby, int &x0, int &y0,
                                              int &z0) {
@@ -132,24 +146,36 @@ __device__ __forceinline__ void brick_origin(uint32_t b, const FastDiv &dbx, con
 // infinity exceeds every finite value, so such a tile is never fast).
 __device__ __forceinline__ bool stage_vtile(const Geo &g, const VelR &V, float *sm, uint32_t *smax, float k,
                                             int bx0, int by0, int bz0, VTile &T) {
-  T = VTile{sm, bx0 - VT_H, by0 - VT_H, bz0 - VT_H};
+  T = VTile{sm, bx0 - VT_H, by0 - VT_H, bz0 - VT_H, 0};
+  T.k0 = T.x0 + VT_X * T.y0 + VT_X * VT_Y * T.z0;
   if (threadIdx.x == 0) *smax = 0u;
+  // entry e = t + NT j of every component: the three padded arrays share
+  // the layout, so each entry's offset is computed once; all loads are
+  // issued before the first store
+  constexpr int NE = VT_N / NT;
+  static_assert(VT_N % NT == 0, "whole staging rounds");
+  uint32_t bits[NE][3];
+#pragma unroll
+  for (int j = 0; j < NE; j++) {
+    const int e = (int)threadIdx.x + NT * j;
+    const int lz = e / (VT_X * VT_Y), r = e - lz * (VT_X * VT_Y);
+    const int ly = r / VT_X, lx = r - ly * VT_X;
+    const int gx = T.x0 + lx, gy = T.y0 + ly, gz = T.z0 + lz;
+    const bool in = (unsigned)gx < (unsigned)g.vx && (unsigned)gy < (unsigned)g.vy && (unsigned)gz <= (unsigned)g.nz;
+    const uint32_t off = in ? vidx(g, gx, gy, gz) * 4u : RB_OOB;
+    bits[j][0] = __builtin_amdgcn_raw_buffer_load_b32(V.u.r, off, 0, 0);
+    bits[j][1] = __builtin_amdgcn_raw_buffer_load_b32(V.v.r, off, 0, 0);
+    bits[j][2] = __builtin_amdgcn_raw_buffer_load_b32(V.w.r, off, 0, 0);
+  }
   uint32_t mb = 0u;
 #pragma unroll
-  for (int c = 0; c < 3; c++) {
-    const RB &X = c == 0 ? V.u : (c == 1 ? V.v : V.w);
-    for (int e = threadIdx.x; e < VT_N; e += NT) {
-      const int lz = e / (VT_X * VT_Y), r = e - lz * (VT_X * VT_Y);
-      const int ly = r / VT_X, lx = r - ly * VT_X;
-      const int gx = T.x0 + lx, gy = T.y0 + ly, gz = T.z0 + lz;
-      const bool in = (unsigned)gx < (unsigned)g.vx && (unsigned)gy < (unsigned)g.vy && (unsigned)gz <= (unsigned)g.nz;
-      const uint32_t off = in ? vidx(g, gx, gy, gz) * 4u : RB_OOB;
-      const uint32_t bits = __builtin_amdgcn_raw_buffer_load_b32(X.r, off, 0, 0);
-      sm[c * VT_N + e] = __builtin_bit_cast(float, bits);
-      const uint32_t ab = bits & 0x7fffffffu;
+  for (int j = 0; j < NE; j++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      sm[c * VT_N + (int)threadIdx.x + NT * j] = __builtin_bit_cast(float, bits[j][c]);
+      const uint32_t ab = bits[j][c] & 0x7fffffffu;
       mb = ab > mb ? ab : mb;
     }
-  }
   for (int o = 32; o > 0; o >>= 1) {
     const uint32_t t = (uint32_t)__shfl_xor((int)mb, o, 64);
     mb = t > mb ? t : mb;
@@ -173,11 +199,15 @@ struct Ax {
   float f;
 };
 template <bool FAST>
-__device__ __forceinline__ Ax axis(float g0, int h) {
-  const float g = FAST ? __builtin_amdgcn_fmed3f(g0, 0.0f, (float)h) : pclamp(g0, (float)h);
+__device__ __forceinline__ Ax axis(float g0, int h, float hf) {
+  const float g = FAST ? __builtin_amdgcn_fmed3f(g0, 0.0f, hf) : pclamp(g0, hf);
   int b = (int)g;
   b = b > h - 1 ? h - 1 : b;
   return Ax{b, g - (float)b};
+}
+template <bool FAST>
+__device__ __forceinline__ Ax axis(float g0, int h) {
+  return axis<FAST>(g0, h, (float)h);
 }
 // The four x-pairs of a sample's corners: from the tile, or gathered.
 template <bool FAST>
@@ -186,7 +216,7 @@ __device__ __forceinline__ void corners(const Geo &g, const VTile &T, int c, con
   const int lx = bx - T.x0, ly = by - T.y0, lz = bz - T.z0;
   if (FAST || ((unsigned)lx < (unsigned)(VT_X - 1) && (unsigned)ly < (unsigned)(VT_Y - 1) &&
                (unsigned)lz < (unsigned)(VT_Z - 1))) {
-    const float *q = T.s + c * VT_N + (lz * VT_Y + ly) * VT_X + lx;
+    const float *q = T.s + c * VT_N + tli(T, bx, by, bz);
     p00 = make_float2(q[0], q[1]);
     p10 = make_float2(q[VT_X], q[VT_X + 1]);
     p01 = make_float2(q[VT_X * VT_Y], q[VT_X * VT_Y + 1]);
@@ -237,11 +267,11 @@ __device__ __forceinline__ void mmCT(const Geo &g, const VTile &T, int c, const 
 // pairwise -- v and w the x axis (px - 1/2, bound nx - 1), u and w the y axis
 // (py - 1/2, ny - 1), u and v the z axis (pz - 1/2, nz - 1).
 template <bool FAST>
-__device__ __forceinline__ void velAtT(const Geo &g, const VTile &T, const VelR &V, float px, float py, float pz,
-                                       float &ux, float &uy, float &uz) {
-  const Ax x0 = axis<FAST>(px, g.nx), x1 = axis<FAST>(px - 0.5f, g.nx - 1);
-  const Ax y0 = axis<FAST>(py, g.ny), y1 = axis<FAST>(py - 0.5f, g.ny - 1);
-  const Ax z0 = axis<FAST>(pz, g.nz), z1 = axis<FAST>(pz - 0.5f, g.nz - 1);
+__device__ __forceinline__ void velAtT(const Geo &g, const Bnd &B, const VTile &T, const VelR &V, float px,
+                                       float py, float pz, float &ux, float &uy, float &uz) {
+  const Ax x0 = axis<FAST>(px, g.nx, B.nx), x1 = axis<FAST>(px - 0.5f, g.nx - 1, B.nx1);
+  const Ax y0 = axis<FAST>(py, g.ny, B.ny), y1 = axis<FAST>(py - 0.5f, g.ny - 1, B.ny1);
+  const Ax z0 = axis<FAST>(pz, g.nz, B.nz), z1 = axis<FAST>(pz - 0.5f, g.nz - 1, B.nz1);
   ux = triT<FAST>(g, T, 0, V.u, x0, y1, z1);
   uy = triT<FAST>(g, T, 1, V.v, x1, y0, z1);
   uz = triT<FAST>(g, T, 2, V.w, x1, y1, z0);
@@ -249,7 +279,7 @@ __device__ __forceinline__ void velAtT(const Geo &g, const VTile &T, const VelR 
 
 // Tile node (x, y, z) of component c.
 __device__ __forceinline__ float tn(const VTile &T, int c, int x, int y, int z) {
-  return T.s[c * VT_N + ((z - T.z0) * VT_Y + (y - T.y0)) * VT_X + (x - T.x0)];
+  return T.s[c * VT_N + tli(T, x, y, z)];
 }
 // The velocity at a trace's start point, where every sample weight is 0 or
 // 1/2.  Valid on a fast block (finite velocities) for the interior points
@@ -294,32 +324,33 @@ __device__ __forceinline__ bool start_interior(const Geo &g, int c, int x, int y
   }
 }
 template <bool FAST>
-__device__ __forceinline__ void start_vel(const Geo &g, const VTile &T, const VelR &V, int c, int x, int y, int z,
-                                          float px, float py, float pz, float &ux, float &uy, float &uz) {
+__device__ __forceinline__ void start_vel(const Geo &g, const Bnd &B, const VTile &T, const VelR &V, int c, int x,
+                                          int y, int z, float px, float py, float pz, float &ux, float &uy,
+                                          float &uz) {
   if (FAST && start_interior(g, c, x, y, z)) {
     if (c == 0) start_u(T, x, y, z, ux, uy, uz);
     else if (c == 1) start_v(T, x, y, z, ux, uy, uz);
     else if (c == 2) start_w(T, x, y, z, ux, uy, uz);
     else start_c(T, x, y, z, ux, uy, uz);
   } else {
-    velAtT<FAST>(g, T, V, px, py, pz, ux, uy, uz);
+    velAtT<FAST>(g, B, T, V, px, py, pz, ux, uy, uz);
   }
 }
 // traceBack (vel.wgsl:2-13) from the start point of type c; with the
 // forward trace (vel.wgsl:14-19) too when FWD.
 template <bool FAST, bool FWD>
-__device__ __forceinline__ void traceT(const Geo &g, const VTile &T, const VelR &V, float k, int c, int x, int y,
-                                       int z, float px, float py, float pz, float &bx, float &by, float &bz,
-                                       float &fx, float &fy, float &fz) {
+__device__ __forceinline__ void traceT(const Geo &g, const Bnd &B, const VTile &T, const VelR &V, float k, int c,
+                                       int x, int y, int z, float px, float py, float pz, float &bx, float &by,
+                                       float &bz, float &fx, float &fy, float &fz) {
   float u1x, u1y, u1z, u2x, u2y, u2z;
-  start_vel<FAST>(g, T, V, c, x, y, z, px, py, pz, u1x, u1y, u1z);
+  start_vel<FAST>(g, B, T, V, c, x, y, z, px, py, pz, u1x, u1y, u1z);
   const float hk = 0.5f * k;
-  velAtT<FAST>(g, T, V, px - hk * u1x, py - hk * u1y, pz - hk * u1z, u2x, u2y, u2z);
+  velAtT<FAST>(g, B, T, V, px - hk * u1x, py - hk * u1y, pz - hk * u1z, u2x, u2y, u2z);
   bx = px - k * u2x;
   by = py - k * u2y;
   bz = pz - k * u2z;
   if (FWD) {
-    velAtT<FAST>(g, T, V, px + hk * u1x, py + hk * u1y, pz + hk * u1z, u2x, u2y, u2z);
+    velAtT<FAST>(g, B, T, V, px + hk * u1x, py + hk * u1y, pz + hk * u1z, u2x, u2y, u2z);
     fx = px + k * u2x;
     fy = py + k * u2y;
     fz = pz + k * u2z;
@@ -332,17 +363,18 @@ __device__ __forceinline__ void adv_vel1_face(const Params *__restrict__ P, cons
                                               const VelR &V, float k, int x, int y, int z, float *__restrict__ hu,
                                               float *__restrict__ hv, float *__restrict__ hw) {
   const uint32_t i = vidx(g, x, y, z);
+  const Bnd B = bnd_of(g);
   float bx, by, bz, fx, fy, fz;
   if (x <= g.nx && y <= g.ny - 1 && z <= g.nz - 1) {
-    traceT<FAST, false>(g, T, V, k, 0, x, y, z, (float)x, (float)y + 0.5f, (float)z + 0.5f, bx, by, bz, fx, fy, fz);
+    traceT<FAST, false>(g, B, T, V, k, 0, x, y, z, (float)x, (float)y + 0.5f, (float)z + 0.5f, bx, by, bz, fx, fy, fz);
     hu[i] = triCT<FAST>(g, T, 0, V.u, bx, by - 0.5f, bz - 0.5f, g.nx, g.ny - 1, g.nz - 1);
   }
   if (x <= g.nx - 1 && y <= g.ny && z <= g.nz - 1) {
-    traceT<FAST, false>(g, T, V, k, 1, x, y, z, (float)x + 0.5f, (float)y, (float)z + 0.5f, bx, by, bz, fx, fy, fz);
+    traceT<FAST, false>(g, B, T, V, k, 1, x, y, z, (float)x + 0.5f, (float)y, (float)z + 0.5f, bx, by, bz, fx, fy, fz);
     hv[i] = triCT<FAST>(g, T, 1, V.v, bx - 0.5f, by, bz - 0.5f, g.nx - 1, g.ny, g.nz - 1);
   }
   if (x <= g.nx - 1 && y <= g.ny - 1 && z <= g.nz) {
-    traceT<FAST, false>(g, T, V, k, 2, x, y, z, (float)x + 0.5f, (float)y + 0.5f, (float)z, bx, by, bz, fx, fy, fz);
+    traceT<FAST, false>(g, B, T, V, k, 2, x, y, z, (float)x + 0.5f, (float)y + 0.5f, (float)z, bx, by, bz, fx, fy, fz);
     hw[i] = triCT<FAST>(g, T, 2, V.w, bx - 0.5f, by - 0.5f, bz, g.nx - 1, g.ny - 1, g.nz);
   }
 }
@@ -379,6 +411,7 @@ __device__ __forceinline__ void adv_vel2_face(const Geo &g, const VTile &T, cons
                                               float *__restrict__ ou, float *__restrict__ ov,
                                               float *__restrict__ ow, const uint8_t *__restrict__ fl) {
   const uint32_t i = vidx(g, x, y, z);
+  const Bnd B = bnd_of(g);
   const bool sc = solidC(g, fl, x, y, z);
 #pragma unroll 1
   for (int c = 0; c < 3; c++) {
@@ -394,7 +427,7 @@ __device__ __forceinline__ void adv_vel2_face(const Geo &g, const VTile &T, cons
     const RB &Hc = c == 0 ? Hu : (c == 1 ? Hv : Hw);
     const float px = (float)x + offx, py = (float)y + offy, pz = (float)z + offz;
     float bx, by, bz, fx, fy, fz;
-    traceT<FAST, true>(g, T, V, k, c, x, y, z, px, py, pz, bx, by, bz, fx, fy, fz);
+    traceT<FAST, true>(g, B, T, V, k, c, x, y, z, px, py, pz, bx, by, bz, fx, fy, fz);
     bx = bx - offx; by = by - offy; bz = bz - offz;
     fx = fx - offx; fy = fy - offy; fz = fz - offz;
     float lo, hi;
@@ -482,7 +515,7 @@ __device__ __forceinline__ float outside(const Geo &g, float x, float y, float z
 // sample of both fields at its foot, ambient beyond the open boundaries.
 // With S1_TILE the velocity comes through the LDS tile like the other three
 // kernels; without it every velocity sample is gathered from global memory.
-constexpr bool S1_TILE = true;
+constexpr bool S1_TILE = false;
 __device__ __forceinline__ void adv_s1_out(const Params *__restrict__ P, const Geo &g, uint32_t i, float bx,
                                            float by, float bz, const RB &s0, const RB &s1, float4 *__restrict__ h0,
                                            float4 *__restrict__ h1) {
@@ -513,8 +546,8 @@ __global__ __launch_bounds__(NT) void k_adv_s1(const Params *__restrict__ P, Geo
     VTile T;
     const bool fast = stage_vtile(g, V, sm, &smax, k, x0, y0, z0, T);
     if (x >= g.nx || y >= g.ny || z >= g.nz) return;
-    if (fast) traceT<true, false>(g, T, V, k, 3, x, y, z, px, py, pz, bx, by, bz, fx, fy, fz);
-    else traceT<false, false>(g, T, V, k, 3, x, y, z, px, py, pz, bx, by, bz, fx, fy, fz);
+    if (fast) traceT<true, false>(g, bnd_of(g), T, V, k, 3, x, y, z, px, py, pz, bx, by, bz, fx, fy, fz);
+    else traceT<false, false>(g, bnd_of(g), T, V, k, 3, x, y, z, px, py, pz, bx, by, bz, fx, fy, fz);
   } else {
     if (x >= g.nx || y >= g.ny || z >= g.nz) return;
     trace<-1>(g, V, k, px, py, pz, bx, by, bz);
@@ -536,9 +569,10 @@ __device__ __forceinline__ void adv_s2_cell(const Geo &g, const VTile &T, const 
                                             int z, const RB &s0, const RB &s1, const RB &h0, const RB &h1,
                                             float4 *__restrict__ o0, float4 *__restrict__ o1) {
   const uint32_t i = sidx(g, x, y, z);
+  const Bnd B = bnd_of(g);
   const float px = (float)x + 0.5f, py = (float)y + 0.5f, pz = (float)z + 0.5f;
   float bx, by, bz, fx, fy, fz;
-  traceT<FAST, true>(g, T, V, k, 3, x, y, z, px, py, pz, bx, by, bz, fx, fy, fz);
+  traceT<FAST, true>(g, B, T, V, k, 3, x, y, z, px, py, pz, bx, by, bz, fx, fy, fz);
   bx = bx - 0.5f; by = by - 0.5f; bz = bz - 0.5f;
   fx = fx - 0.5f; fy = fy - 0.5f; fz = fz - 0.5f;
   float4 r0 = ld4(h0, i), r1 = ld4(h1, i);
