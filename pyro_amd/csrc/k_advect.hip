@@ -93,10 +93,11 @@ __device__ __forceinline__ void trace(const Geo &g, const VelR &V, float k, floa
 // Each block stages the three padded velocity arrays over its brick plus a
 // halo of VT_H nodes in LDS.  A cell brick and a face brick index the same
 // padded grid (face (x, y, z) is the low face of cell (x, y, z)), so one
-// tile shape serves every kernel.  It pays where a thread samples the
-// velocity often (adv_vel1, adv_vel2, adv_s2); adv_s1 gathers directly.  Two
-// nodes (15 KB per 16x4x4 block) measured best against three (29.6 KB per
-// 16x2x8 block: fewer blocks in flight, longer staging).
+// tile shape serves every kernel, all four of which use it (adv_s1 with it
+// 278 -> 239 us at pool 2 cm: its velocity gathers kept the texture
+// addressers busy).  Two nodes (15 KB per 16x4x4 block) measured best
+// against three (29.6 KB per 16x2x8 block: fewer blocks in flight, longer
+// staging).
 //
 // Staging also finds the largest |velocity| M of the tile.  Every velocity
 // sample is a convex combination of tile nodes, so no trace point moves more
@@ -413,7 +414,7 @@ __device__ __forceinline__ void adv_vel2_face(const Geo &g, const VTile &T, cons
   const uint32_t i = vidx(g, x, y, z);
   const Bnd B = bnd_of(g);
   const bool sc = solidC(g, fl, x, y, z);
-#pragma unroll 1
+#pragma unroll
   for (int c = 0; c < 3; c++) {
     const int hx = g.nx - 1 + (c == 0), hy = g.ny - 1 + (c == 1), hz = g.nz - 1 + (c == 2);
     if (x > hx || y > hy || z > hz) continue;
@@ -436,7 +437,10 @@ __device__ __forceinline__ void adv_vel2_face(const Geo &g, const VTile &T, cons
     O[i] = clampf_(ldf(Hc, i) + 0.5f * (ldf(Vc, i) - t), lo, hi);
   }
 }
-__global__ __launch_bounds__(NT) void k_adv_vel2(const Params *__restrict__ P, Geo g, VelPtrs VP,
+// The component loop is unrolled (each start type and offset known at
+// compile time) and the kernel held to 5 waves per SIMD (at most 102 VGPRs;
+// unbounded it takes 105 and runs 4): 466 -> 347 us at pool 2 cm.
+__global__ __launch_bounds__(NT, 5) void k_adv_vel2(const Params *__restrict__ P, Geo g, VelPtrs VP,
                                                   const float *__restrict__ hu, const float *__restrict__ hv,
                                                   const float *__restrict__ hw, float *__restrict__ ou,
                                                   float *__restrict__ ov, float *__restrict__ ow,
@@ -515,7 +519,7 @@ __device__ __forceinline__ float outside(const Geo &g, float x, float y, float z
 // sample of both fields at its foot, ambient beyond the open boundaries.
 // With S1_TILE the velocity comes through the LDS tile like the other three
 // kernels; without it every velocity sample is gathered from global memory.
-constexpr bool S1_TILE = false;
+constexpr bool S1_TILE = true;
 __device__ __forceinline__ void adv_s1_out(const Params *__restrict__ P, const Geo &g, uint32_t i, float bx,
                                            float by, float bz, const RB &s0, const RB &s1, float4 *__restrict__ h0,
                                            float4 *__restrict__ h1) {

@@ -6,7 +6,8 @@ red-black sweep and R launches of the plain copy kernel over 64 Mi floats
 (the calibration for FETCH_SIZE / WRITE_SIZE counters).  Optional phases:
 ``--solves K`` K MGPCG solves to 1e-4 of the next step's pressure problem
 (warm start); ``--unet F`` F U-Net forward passes on the same grid;
-``--train T`` T training steps of the U-Net on 8 pairs of the pool at 4 cm.
+``--train T`` T training steps of the U-Net on 8 pairs of the pool at 4 cm;
+``--noise A`` the velocity noise amplitude of the start state (m/s).
 Prints one line per phase so a profiled run is never silent.
 """
 from __future__ import annotations
@@ -31,11 +32,13 @@ def main() -> None:
     ap.add_argument("--solves", type=int, default=0)
     ap.add_argument("--unet", type=int, default=0)
     ap.add_argument("--train", type=int, default=0)
+    ap.add_argument("--noise", type=float, default=1e-3,
+                    help="amplitude (m/s) of the uniform velocity noise the run starts from")
     args = ap.parse_args()
     s = pa.Scene.load(str(ROOT / "pyro_amd" / "scenes" / "pool.toml"))
     s.set("domain.cell", args.cell)
     sim = pa.Sim(s.params(), s.voxelize(), None, 4, s.get("ignition.until"))
-    sim.add_velocity_noise(1e-3, 0)
+    sim.add_velocity_noise(args.noise, 0)
     for k in range(args.steps):
         sim.step(0.002, k == args.steps - 1)
     sim.sync()
