@@ -360,32 +360,66 @@ constexpr int SW_BY = SW_TY + 2, SW_BZ = SW_TZ + 2;                      // ring
 constexpr int SW_PN = SW_PX * SW_PY * SW_PZ;
 constexpr int SW_Q = SW_PX / 4;  // float4 per footprint row
 
-// One cell of a colour pass at LDS index pi (h^2 b at bi), without branches:
-// the update is computed for every cell and stored only for cells with a
-// counted neighbour that are not solid.  ICNT (outside weight 1): the count
-// is the integer fluid + outside and ct[code & 63] holds (count,
-// RN(1/count)) for div_count (code 0x40 / 0x80 reads (0, 0): a finite,
-// discarded result).
-template <bool ICNT>
-__device__ __forceinline__ void sweep_cell(float *sp, const uint8_t *sc, float hb, const float2 *ct, int pi,
-                                           float omega, float wdir) {
-  const uint32_t code = sc[pi];
-  float sum = 0.0f;
-  sum += sp[pi + 1];
-  sum += sp[pi - 1];
-  sum += sp[pi + SW_PX];
-  sum += sp[pi - SW_PX];
-  sum += sp[pi + SW_PX * SW_PY];
-  sum += sp[pi - SW_PX * SW_PY];
-  float q;
-  if (ICNT) {
-    const float2 cr = ct[code & 63u];
-    q = div_count(sum - hb, cr.x, cr.y);
-  } else {
-    q = (sum - hb) / nc_count(code, wdir);
+// The N cells of one colour that a thread updates in a pass (mg.wgsl:25-35
+// per cell).  ICNT (outside weight 1): the count is the integer fluid +
+// outside and ct[code & 63] holds (count, RN(1/count)) for div_count (code
+// 0x40 / 0x80 reads (0, 0): a finite, discarded result).  The cells of a
+// colour are independent (their neighbours are of the other colour, which
+// the pass does not write), so all of the pass's LDS reads are issued
+// first, the quotients are formed branch-free with one branch per pass for a
+// subnormal quotient (div_count's IEEE fallback), and every cell is stored:
+// one that is not updated (solid, or no counted neighbour) gets its own
+// value back.  act[q] = false: slot q is not this thread's (no read, no
+// store).
+template <bool ICNT, int N>
+__device__ __forceinline__ void sweep_cells(float *sp, const uint8_t *sc, const float *hb, const float2 *ct,
+                                            const int *pi, const bool *act, float omega, float wdir) {
+  uint32_t code[N];
+  float nb[N][6], cen[N];
+#pragma unroll
+  for (int q = 0; q < N; q++) {
+    const int p = act[q] ? pi[q] : SW_PX * SW_PY + SW_PX + 1;  // any interior entry
+    code[q] = sc[p];
+    nb[q][0] = sp[p + 1];
+    nb[q][1] = sp[p - 1];
+    nb[q][2] = sp[p + SW_PX];
+    nb[q][3] = sp[p - SW_PX];
+    nb[q][4] = sp[p + SW_PX * SW_PY];
+    nb[q][5] = sp[p - SW_PX * SW_PY];
+    cen[q] = sp[p];
   }
-  const float nv = mixf_(sp[pi], q, omega);
-  if (!(code & (NC_SOLID | NC_NONE))) sp[pi] = nv;
+  float xs[N], qs[N];
+  bool redo = false;
+#pragma unroll
+  for (int q = 0; q < N; q++) {
+    float sum = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 6; k++) sum += nb[q][k];
+    xs[q] = sum - hb[q];
+    if (ICNT) {
+      const float2 cr = ct[code[q] & 63u];
+      const float q0 = xs[q] * cr.y;
+      const float e = __builtin_fmaf(-q0, cr.x, xs[q]);
+      qs[q] = __builtin_fmaf(e, cr.y, q0);
+      redo |= __builtin_amdgcn_classf(qs[q], 0x90);  // -subnormal | +subnormal
+    } else {
+      qs[q] = xs[q] / nc_count(code[q], wdir);
+    }
+  }
+  if (ICNT && redo) {
+#pragma unroll
+    for (int q = 0; q < N; q++)
+      if (__builtin_amdgcn_classf(qs[q], 0x90)) {
+        float x = xs[q];
+        asm volatile("" : "+v"(x));
+        qs[q] = x / ct[code[q] & 63u].x;
+      }
+  }
+#pragma unroll
+  for (int q = 0; q < N; q++) {
+    const float nv = mixf_(cen[q], qs[q], omega);
+    if (act[q]) sp[pi[q]] = (code[q] & (NC_SOLID | NC_NONE)) ? cen[q] : nv;
+  }
 }
 
 // h^2 b of the cell at grid (gx, gy, gz), read from global memory (0 outside
@@ -396,8 +430,15 @@ template <bool VEC>
 __device__ __forceinline__ float sweep_hb(const LvArgs &a, const RB &rbb, const float *__restrict__ b, int gx, int gy,
                                           int gz) {
   const bool ok = (uint32_t)gx < (uint32_t)a.d0 && (uint32_t)gy < (uint32_t)a.d1 && (uint32_t)gz < (uint32_t)a.d2;
+  if (VEC) {
+    // 24-bit multiplies (full rate): d0, d1, z and the row index y + d1 z of
+    // a cell inside the level are below 2^24 (launch_mg_sweep takes VEC only
+    // for d1 d2 <= 2^24); the offset of a cell outside is replaced by the
+    // out-of-range one
+    const uint32_t g = (uint32_t)gx + __umul24((uint32_t)a.d0, (uint32_t)gy + __umul24((uint32_t)a.d1, (uint32_t)gz));
+    return a.hh * __builtin_bit_cast(float, rb_ld32(rbb, ok ? 4u * g : RB_OOB));
+  }
   const uint32_t g = (uint32_t)gx + (uint32_t)a.d0 * ((uint32_t)gy + (uint32_t)a.d1 * (uint32_t)gz);
-  if (VEC) return a.hh * __builtin_bit_cast(float, rb_ld32(rbb, ok ? 4u * g : RB_OOB));
   return ok ? a.hh * b[g] : 0.0f;
 }
 
@@ -434,12 +475,13 @@ __global__ __launch_bounds__(SW_NT) void k_mg_sweep(LvArgs a, int ntx, int nty, 
       const int q = t % SW_Q, r0 = t / SW_Q, ly = r0 % SW_PY, lz0 = r0 / SW_PY;
       const int gx = x0 + 4 * q, gy = y0 + ly;
       const bool okxy = t < SW_PT && (uint32_t)gx < (uint32_t)a.d0 && (uint32_t)gy < (uint32_t)a.d1;
-      const uint32_t gxy = (uint32_t)gx + (uint32_t)sy * (uint32_t)gy;
+      // offsets by 24-bit multiplies (the VEC bounds of sweep_hb)
 #pragma unroll
       for (int it = 0; it < 4; it++) {
         const int gz = z0 + lz0 + 2 * it;
         const bool ok = okxy && (uint32_t)gz < (uint32_t)a.d2;
-        const uint32_t g = gxy + (uint32_t)sz * (uint32_t)gz;
+        const uint32_t g =
+            (uint32_t)gx + __umul24((uint32_t)sy, (uint32_t)gy + __umul24((uint32_t)a.d1, (uint32_t)gz));
         pv[it] = rb_ld4(rp, ok ? 4u * g : RB_OOB);
         const uint32_t c = rb_ld32(rcd, ok ? g : RB_OOB);
         cv[it] = ok ? c : 0x40404040u;
@@ -510,15 +552,18 @@ __global__ __launch_bounds__(SW_NT) void k_mg_sweep(LvArgs a, int ntx, int nty, 
   // (ly, lz) of the 10 x 6 ring-1 box (LDS offset 1 in y and z); thread
   // t < 255 takes x-pair t % 17 of rows r0 + 15 it, r0 = t / 17, it = 0..3
   static_assert((SW_TX + 2) / 2 == 17 && SW_BY * SW_BZ == 60, "colour-0 map");
-  if (t < 255) {
+  {
     const int px = t % 17, r0 = t / 17;
+    int pi[4];
+    bool act[4];
 #pragma unroll
     for (int it = 0; it < 4; it++) {
       const int r = r0 + 15 * it, lz = r / SW_BY, ly = r - SW_BY * lz;
       const int lx = 3 + 2 * px + ((K + ly + lz) & 1);
-      const int pi = ((lz + 1) * SW_PY + (ly + 1)) * SW_PX + lx;
-      sweep_cell<ICNT>(sp, sc, hb0[it], ct, pi, omega, a.wdir);
+      pi[it] = (int)__umul24((uint32_t)((lz + 1) * SW_PY + (ly + 1)), (uint32_t)SW_PX) + lx;
+      act[it] = t < 255;
     }
+    sweep_cells<ICNT, 4>(sp, sc, hb0, ct, pi, act, omega, a.wdir);
   }
   __syncthreads();
   // colour 1 on the tile: x-pairs from lx = 4 (16), rows of the 8 x 4 tile
@@ -526,13 +571,15 @@ __global__ __launch_bounds__(SW_NT) void k_mg_sweep(LvArgs a, int ntx, int nty, 
   static_assert(SW_TX / 2 == 16 && SW_TY == 8 && SW_TZ == 4, "colour-1 map");
   {
     const int px = t % 16, r0 = t / 16;
+    int pi[2];
+    const bool act[2] = {true, true};
 #pragma unroll
     for (int it = 0; it < 2; it++) {
       const int r = r0 + 16 * it, ly = r % SW_TY, lz = r / SW_TY;
       const int lx = 4 + 2 * px + ((K + ly + lz) & 1);
-      const int pi = ((lz + 2) * SW_PY + (ly + 2)) * SW_PX + lx;
-      sweep_cell<ICNT>(sp, sc, hb1[it], ct, pi, omega, a.wdir);
+      pi[it] = (int)__umul24((uint32_t)((lz + 2) * SW_PY + (ly + 2)), (uint32_t)SW_PX) + lx;
     }
+    sweep_cells<ICNT, 2>(sp, sc, hb1, ct, pi, act, omega, a.wdir);
   }
   __syncthreads();
   if (VEC) {
@@ -541,7 +588,7 @@ __global__ __launch_bounds__(SW_NT) void k_mg_sweep(LvArgs a, int ntx, int nty, 
     const int q = t % (SW_TX / 4), row = t / (SW_TX / 4), ly = row % SW_TY, lz = row / SW_TY;
     const int gx = x0 + 4 + 4 * q, gy = y0 + 2 + ly, gz = z0 + 2 + lz;
     if (gx < a.d0 && gy < a.d1 && gz < a.d2) {
-      const int g = gx + sy * gy + sz * gz;
+      const uint32_t g = (uint32_t)gx + __umul24((uint32_t)sy, (uint32_t)gy + __umul24((uint32_t)a.d1, (uint32_t)gz));
       const int pi = ((lz + 2) * SW_PY + (ly + 2)) * SW_PX + 4 + 4 * q;
       float4 v = *reinterpret_cast<const float4 *>(sp + pi);
       const uint32_t cv = *reinterpret_cast<const uint32_t *>(sc + pi);
@@ -571,8 +618,9 @@ void launch_mg_sweep(const MgLevel &L, const float *pin, float *pout, float omeg
   LvArgs a = lv_args(L);
   int ntx = (L.d[0] + SW_TX - 1) / SW_TX, nty = (L.d[1] + SW_TY - 1) / SW_TY, ntz = (L.d[2] + SW_TZ - 1) / SW_TZ;
   int nb = ntx * nty * ntz;
-  // VEC: whole float4 rows, buffer loads with 32-bit byte offsets below 2 GiB
-  bool vec = (L.d[0] % 4) == 0 && (uint64_t)L.n * 4u < (1ull << 31);
+  // VEC: whole float4 rows, buffer loads with 32-bit byte offsets below 2 GiB,
+  // row indices y + d1 z below 2^24 (sweep_hb's 24-bit multiplies)
+  bool vec = (L.d[0] % 4) == 0 && (uint64_t)L.n * 4u < (1ull << 31) && (uint64_t)L.d[1] * L.d[2] <= (1ull << 24);
   if (L.wdir == 1.0f) {
     if (fine && vec)
       k_mg_sweep<true, true, true><<<nb, SW_NT, 0, s>>>(a, ntx, nty, L.b, L.nc, pin, pout, omega);
