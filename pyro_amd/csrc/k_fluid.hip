@@ -65,41 +65,6 @@ void launch_curl(const Fields &f, StatParts *st, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------- forces
-__device__ __forceinline__ bool solidAt(const Geo &g, const uint8_t *__restrict__ fl, int x, int y, int z) {
-  return inb(g, x, y, z) && fl[sidx(g, x, y, z)] == 1;
-}
-__device__ __forceinline__ float cmag(const Geo &g, const float4 *__restrict__ curl, int x, int y, int z) {
-  return curl[sidx(g, clampi(x, 0, g.nx - 1), clampi(y, 0, g.ny - 1), clampi(z, 0, g.nz - 1))].w;
-}
-// Vorticity confinement at a cell centre (fluid.wgsl:167-181).
-__device__ void confine(const Params *__restrict__ P, const Geo &g, const float4 *__restrict__ s0,
-                        const float4 *__restrict__ s1, const float4 *__restrict__ curl, int x, int y, int z,
-                        float &fx, float &fy, float &fz) {
-  fx = fy = fz = 0.0f;
-  uint32_t i = sidx(g, x, y, z);
-  float4 a = s0[i];
-  float T = a.x;
-  float smoke = a.w + s1[i].x;
-  float gate = fmax_(smoothstep_(P->vort_t, P->vort_t + 500.0f, T), 0.35f * smoothstep_(0.0f, 2e-4f, smoke));
-  if (gate <= 0.0f) return;
-  float gx = cmag(g, curl, x + 1, y, z) - cmag(g, curl, x - 1, y, z);
-  float gy = cmag(g, curl, x, y + 1, z) - cmag(g, curl, x, y - 1, z);
-  float gz = cmag(g, curl, x, y, z + 1) - cmag(g, curl, x, y, z - 1);
-  float gl = len3(gx, gy, gz);
-  if (gl < 1e-6f) return;
-  float nx = gx / gl, ny = gy / gl, nz = gz / gl;
-  float4 w = curl[i];
-  float cx = ny * w.z - nz * w.y, cy = nz * w.x - nx * w.z, cz = nx * w.y - ny * w.x;
-  float vd = P->vort * P->dx;
-  fx = vd * cx * gate;
-  fy = vd * cy * gate;
-  fz = vd * cz * gate;
-  float fl = len3(fx, fy, fz);
-  if (fl > 60.0f) {
-    float sc = 60.0f / fl;
-    fx *= sc; fy *= sc; fz *= sc;
-  }
-}
 __device__ __forceinline__ float spongeSide(const Params *__restrict__ P, const Geo &g, int x, int z) {
   int e = x < g.nx - 1 - x ? x : g.nx - 1 - x;
   int e2 = z < g.nz - 1 - z ? z : g.nz - 1 - z;
@@ -111,30 +76,145 @@ __device__ __forceinline__ float spongeTop(const Params *__restrict__ P, const G
   return smoothstep_(h - 1.5f * P->sponge, h - 1.0f, (float)y);
 }
 
+// A block takes a 16 x 4 x 4 brick of face triples (brick_cell over the
+// padded faces).  Every face reads its own cell b = (x, y, z) and, for
+// component k, the cell a one step down along k, so the block's cells are
+// the brick grown by one on the low side of each axis: C_X x C_Y x C_Z =
+// 17 x 5 x 5.  Their confinement needs |curl| one cell further out on both
+// sides (19 x 7 x 7, coordinates clamped to the grid as fluid.wgsl's
+// curl lookups are).  Both tiles are
+// staged in LDS with all global loads in flight together, the confinement
+// of each tile cell is evaluated once (a face triple used to evaluate four),
+// and the faces then read T, smoke, solid flags and forces from LDS.
+constexpr int FC_X = BR_X + 1, FC_Y = BR_Y + 1, FC_Z = BR_Z + 1, FC_N = FC_X * FC_Y * FC_Z;
+constexpr int FW_X = BR_X + 3, FW_Y = BR_Y + 3, FW_Z = BR_Z + 3, FW_N = FW_X * FW_Y * FW_Z;
+constexpr int FC_R = (FC_N + NT - 1) / NT, FW_R = (FW_N + NT - 1) / NT;
+
+// Vorticity confinement at a cell centre (fluid.wgsl:167-181) from its T,
+// smoke and curl w; m points at the cell's |curl| in the |curl| tile, whose
+// entries m[+-1], m[+-FW_X], m[+-FW_X FW_Y] are the clamped neighbours.
+__device__ __forceinline__ void confine_tile(const Params *__restrict__ P, float T, float smoke, float4 w,
+                                             const float *m, float &fx, float &fy, float &fz) {
+  fx = fy = fz = 0.0f;
+  float gate = fmax_(smoothstep_(P->vort_t, P->vort_t + 500.0f, T), 0.35f * smoothstep_(0.0f, 2e-4f, smoke));
+  if (gate <= 0.0f) return;
+  float gx = m[1] - m[-1];
+  float gy = m[FW_X] - m[-FW_X];
+  float gz = m[FW_X * FW_Y] - m[-FW_X * FW_Y];
+  float gl = len3(gx, gy, gz);
+  if (gl < 1e-6f) return;
+  float nx = gx / gl, ny = gy / gl, nz = gz / gl;
+  float cx = ny * w.z - nz * w.y, cy = nz * w.x - nx * w.z, cz = nx * w.y - ny * w.x;
+  float vd = P->vort * P->dx;
+  fx = vd * cx * gate;
+  fy = vd * cy * gate;
+  fz = vd * cz * gate;
+  float fl = len3(fx, fy, fz);
+  if (fl > 60.0f) {
+    float sc = 60.0f / fl;
+    fx *= sc; fy *= sc; fz *= sc;
+  }
+}
+
 __global__ __launch_bounds__(NT) void k_forces(const Params *__restrict__ P, Geo g, float *__restrict__ U,
                                                 float *__restrict__ V, float *__restrict__ W,
                                                 const float4 *__restrict__ s0, const float4 *__restrict__ s1,
                                                 const float4 *__restrict__ curl, const uint8_t *__restrict__ fl) {
-  uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
-  if (i >= g.nv) return;
-  int x, y, z;
-  decode_face(g, i, x, y, z);
+  __shared__ float sw[FW_N];                           // |curl|
+  __shared__ float sf[3][FC_N], st[FC_N], sm[FC_N];    // confinement force, T, smoke
+  __shared__ uint8_t sc[FC_N];                         // bit 0 inside the grid, bit 1 solid
+  const uint32_t blk = (uint32_t)xcd_tile(blockIdx.x, gridDim.x);
+  int x, y, z, x0, y0, z0;
+  brick_cell(blk, threadIdx.x, g.dfbx, g.dfby, x, y, z);
+  {
+    const uint32_t r = fdiv(blk, g.dfbx), bz = fdiv(r, g.dfby);
+    x0 = (int)((blk - r * g.dfbx.d) * BR_X);
+    y0 = (int)((r - bz * g.dfby.d) * BR_Y);
+    z0 = (int)(bz * BR_Z);
+  }
+  // every load of the block is issued before the first LDS store: |curl| at
+  // clamped coordinates, the tile cells' state at clamped coordinates (a cell
+  // outside the grid is flagged and never read), the face's velocities
+  float wl[FW_R];
+#pragma unroll
+  for (int j = 0; j < FW_R; j++) {
+    const int e = (int)threadIdx.x + NT * j;
+    wl[j] = 0.0f;
+    if (e < FW_N) {
+      const int lz = e / (FW_X * FW_Y), r = e - lz * (FW_X * FW_Y), ly = r / FW_X, lx = r - ly * FW_X;
+      const int gx = clampi(x0 - 2 + lx, 0, g.nx - 1), gy = clampi(y0 - 2 + ly, 0, g.ny - 1),
+                gz = clampi(z0 - 2 + lz, 0, g.nz - 1);
+      wl[j] = curl[sidx(g, gx, gy, gz)].w;
+    }
+  }
+  float4 ca[FC_R], cw[FC_R];
+  float cs1[FC_R];
+  uint8_t cf[FC_R];
+  bool cin[FC_R];
+#pragma unroll
+  for (int j = 0; j < FC_R; j++) {
+    const int e = (int)threadIdx.x + NT * j;
+    cin[j] = false;
+    ca[j] = cw[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    cs1[j] = 0.0f;
+    cf[j] = 0;
+    if (e < FC_N) {
+      const int lz = e / (FC_X * FC_Y), r = e - lz * (FC_X * FC_Y), ly = r / FC_X, lx = r - ly * FC_X;
+      const int gx = x0 - 1 + lx, gy = y0 - 1 + ly, gz = z0 - 1 + lz;
+      cin[j] = inb(g, gx, gy, gz);
+      const uint32_t c = sidx(g, clampi(gx, 0, g.nx - 1), clampi(gy, 0, g.ny - 1), clampi(gz, 0, g.nz - 1));
+      ca[j] = s0[c];
+      cs1[j] = s1[c].x;
+      cw[j] = curl[c];
+      cf[j] = fl[c];
+    }
+  }
+  const bool face = x < g.vx && y < g.vy && z <= g.nz;
+  const uint32_t i = face ? vidx(g, x, y, z) : 0u;
+  float uvw[3];
+  uvw[0] = U[i];
+  uvw[1] = V[i];
+  uvw[2] = W[i];
+#pragma unroll
+  for (int j = 0; j < FW_R; j++) {
+    const int e = (int)threadIdx.x + NT * j;
+    if (e < FW_N) sw[e] = wl[j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < FC_R; j++) {
+    const int e = (int)threadIdx.x + NT * j;
+    if (e >= FC_N) continue;
+    const int lz = e / (FC_X * FC_Y), r = e - lz * (FC_X * FC_Y), ly = r / FC_X, lx = r - ly * FC_X;
+    const float T = ca[j].x, smoke = ca[j].w + cs1[j];
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    if (cin[j]) confine_tile(P, T, smoke, cw[j], sw + (lx + 1) + FW_X * ((ly + 1) + FW_Y * (lz + 1)), fx, fy, fz);
+    sf[0][e] = fx;
+    sf[1][e] = fy;
+    sf[2][e] = fz;
+    st[e] = T;
+    sm[e] = smoke;
+    sc[e] = (uint8_t)((cin[j] ? 1u : 0u) | (cin[j] && cf[j] == 1 ? 2u : 0u));
+  }
+  __syncthreads();
+  if (!face) return;
   const float dt = P->dt;
-  // the face's own cell (x, y, z) is the b side of all three components:
-  // its confinement force is evaluated once
-  float cbx = 0.0f, cby = 0.0f, cbz = 0.0f;
-  if (inb(g, x, y, z)) confine(P, g, s0, s1, curl, x, y, z, cbx, cby, cbz);
+  const int cb = (x - x0 + 1) + FC_X * ((y - y0 + 1) + FC_Y * (z - z0 + 1));
+  const bool ib = sc[cb] & 1u;
+  // the face's own cell (x, y, z) is the b side of all three components
+  const float cbx = sf[0][cb], cby = sf[1][cb], cbz = sf[2][cb];
+#pragma unroll
   for (int k = 0; k < 3; k++) {
     int hx = g.nx - 1 + (k == 0), hy = g.ny - 1 + (k == 1), hz = g.nz - 1 + (k == 2);
     if (x > hx || y > hy || z > hz) continue;
-    int ax = x - (k == 0), ay = y - (k == 1), az = z - (k == 2);
-    if ((k == 1 && y == 0) || solidAt(g, fl, x, y, z) || solidAt(g, fl, ax, ay, az)) continue;
-    bool ia = inb(g, ax, ay, az), ib = inb(g, x, y, z);
+    const int ca_ = cb - (k == 0 ? 1 : (k == 1 ? FC_X : FC_X * FC_Y));
+    if ((k == 1 && y == 0) || (sc[cb] & 2u) || (sc[ca_] & 2u)) continue;
+    const bool ia = sc[ca_] & 1u;
     float f = 0.0f;
     if (k == 1) {
       float Ta = P->t0, Tb = P->t0, ma = 0.0f, mb = 0.0f;
-      if (ia) { uint32_t j = sidx(g, ax, ay, az); float4 q = s0[j]; Ta = q.x; ma = q.w + s1[j].x; }
-      if (ib) { uint32_t j = sidx(g, x, y, z); float4 q = s0[j]; Tb = q.x; mb = q.w + s1[j].x; }
+      if (ia) { Ta = st[ca_]; ma = sm[ca_]; }
+      if (ib) { Tb = st[cb]; mb = sm[cb]; }
       if (!ia) { Ta = Tb; ma = mb; }
       if (!ib) { Tb = Ta; mb = ma; }
       float Tf = 0.5f * (Ta + Tb);
@@ -142,22 +222,20 @@ __global__ __launch_bounds__(NT) void k_forces(const Params *__restrict__ P, Geo
     }
     float fcx = 0.0f, fcy = 0.0f, fcz = 0.0f, n = 0.0f;
     if (ia) {
-      float cx, cy, cz;
-      confine(P, g, s0, s1, curl, ax, ay, az, cx, cy, cz);
-      fcx += cx; fcy += cy; fcz += cz; n += 1.0f;
+      fcx += sf[0][ca_]; fcy += sf[1][ca_]; fcz += sf[2][ca_]; n += 1.0f;
     }
     if (ib) {
       fcx += cbx; fcy += cby; fcz += cbz; n += 1.0f;
     }
     if (n > 0.0f) f += (k == 0 ? fcx : (k == 1 ? fcy : fcz)) / n;
     float *C = k == 0 ? U : (k == 1 ? V : W);
-    float u = C[i];
+    float u = uvw[k];
     u += f * dt;
     int cx = clampi(x, 0, g.nx - 1), cy = clampi(y, 0, g.ny - 1), cz = clampi(z, 0, g.nz - 1);
     float ss = spongeSide(P, g, cx, cz);
     if (k != 1) {
       float Tl = P->t0;
-      if (ib) Tl = s0[sidx(g, x, y, z)].x;
+      if (ib) Tl = st[cb];
       float hot = smoothstep_(350.0f, 900.0f, Tl);
       float kr = P->wind_relax * (1.0f - 0.85f * hot) + 8.0f * ss;
       float wv = k == 0 ? P->wind_x : P->wind_z;
@@ -172,7 +250,7 @@ __global__ __launch_bounds__(NT) void k_forces(const Params *__restrict__ P, Geo
 }
 
 void launch_forces(const Fields &f, hipStream_t s) {
-  k_forces<<<nblocks(f.g.nv), NT, 0, s>>>(f.P, f.g, f.vb[0], f.vb[1], f.vb[2], f.s0, f.s1, f.curl, f.flags);
+  k_forces<<<f.g.fbricks, NT, 0, s>>>(f.P, f.g, f.vb[0], f.vb[1], f.vb[2], f.s0, f.s1, f.curl, f.flags);
 }
 
 // ---------------------------------------------------------------- divergence
