@@ -246,10 +246,11 @@ __device__ __forceinline__ float triCT(const Geo &g, const VTile &T, int c, cons
                                        float gz, int hx, int hy, int hz) {
   return triT<FAST>(g, T, c, X, axis<FAST>(gx, hx), axis<FAST>(gy, hy), axis<FAST>(gz, hz));
 }
-// mmC (tmpl_comp.wgsl:16-25) through the tile.
+// triC and mmC (tmpl_comp.wgsl:2-25) of one point from the same corners:
+// the sample is returned, (min, max) of the eight corners stored in *mm.
 template <bool FAST>
-__device__ __forceinline__ void mmCT(const Geo &g, const VTile &T, int c, const RB &X, float gx, float gy, float gz,
-                                     int hx, int hy, int hz, float &lo, float &hi) {
+__device__ __forceinline__ float triMMCT(const Geo &g, const VTile &T, int c, const RB &X, float gx, float gy,
+                                         float gz, int hx, int hy, int hz, float2 *mm) {
   const Ax ax = axis<FAST>(gx, hx), ay = axis<FAST>(gy, hy), az = axis<FAST>(gz, hz);
   float2 p[4];
   corners<FAST>(g, T, c, X, ax.b, ay.b, az.b, p[0], p[1], p[2], p[3]);
@@ -261,8 +262,12 @@ __device__ __forceinline__ void mmCT(const Geo &g, const VTile &T, int c, const 
     l = fmin_(l, p[j].y);
     h = fmax_(h, p[j].y);
   }
-  lo = l;
-  hi = h;
+  *mm = make_float2(l, h);
+  const float c00 = mixf_(p[0].x, p[0].y, ax.f);
+  const float c10 = mixf_(p[1].x, p[1].y, ax.f);
+  const float c01 = mixf_(p[2].x, p[2].y, ax.f);
+  const float c11 = mixf_(p[3].x, p[3].y, ax.f);
+  return mixf_(mixf_(c00, c10, ay.f), mixf_(c01, c11, ay.f), az.f);
 }
 // velAt (vel.wgsl:2-13): the three components' samples share their axes
 // pairwise -- v and w the x axis (px - 1/2, bound nx - 1), u and w the y axis
@@ -337,19 +342,21 @@ __device__ __forceinline__ void start_vel(const Geo &g, const Bnd &B, const VTil
     velAtT<FAST>(g, B, T, V, px, py, pz, ux, uy, uz);
   }
 }
-// traceBack (vel.wgsl:2-13) from the start point of type c; with the
-// forward trace (vel.wgsl:14-19) too when FWD.
-template <bool FAST, bool FWD>
+// traceBack (vel.wgsl:2-13) from the start point of type c when BACK, the
+// forward trace (vel.wgsl:14-19) when FWD; both start from the same sample.
+template <bool FAST, bool BACK, bool FWD>
 __device__ __forceinline__ void traceT(const Geo &g, const Bnd &B, const VTile &T, const VelR &V, float k, int c,
                                        int x, int y, int z, float px, float py, float pz, float &bx, float &by,
                                        float &bz, float &fx, float &fy, float &fz) {
   float u1x, u1y, u1z, u2x, u2y, u2z;
   start_vel<FAST>(g, B, T, V, c, x, y, z, px, py, pz, u1x, u1y, u1z);
   const float hk = 0.5f * k;
-  velAtT<FAST>(g, B, T, V, px - hk * u1x, py - hk * u1y, pz - hk * u1z, u2x, u2y, u2z);
-  bx = px - k * u2x;
-  by = py - k * u2y;
-  bz = pz - k * u2z;
+  if (BACK) {
+    velAtT<FAST>(g, B, T, V, px - hk * u1x, py - hk * u1y, pz - hk * u1z, u2x, u2y, u2z);
+    bx = px - k * u2x;
+    by = py - k * u2y;
+    bz = pz - k * u2z;
+  }
   if (FWD) {
     velAtT<FAST>(g, B, T, V, px + hk * u1x, py + hk * u1y, pz + hk * u1z, u2x, u2y, u2z);
     fx = px + k * u2x;
@@ -359,29 +366,43 @@ __device__ __forceinline__ void traceT(const Geo &g, const Bnd &B, const VTile &
 }
 
 // ---------------------------------------------------------------- adv_vel1
+// Besides the sample at the back-trace foot, each face stores (min, max) of
+// that sample's eight corners: adv_vel2's limiter (advect.wgsl:22-70, mmC at
+// the same foot of the same field) then reads them instead of tracing back
+// again.
 template <bool FAST>
 __device__ __forceinline__ void adv_vel1_face(const Params *__restrict__ P, const Geo &g, const VTile &T,
                                               const VelR &V, float k, int x, int y, int z, float *__restrict__ hu,
-                                              float *__restrict__ hv, float *__restrict__ hw) {
+                                              float *__restrict__ hv, float *__restrict__ hw,
+                                              float2 *__restrict__ mu, float2 *__restrict__ mv,
+                                              float2 *__restrict__ mw) {
   const uint32_t i = vidx(g, x, y, z);
   const Bnd B = bnd_of(g);
   float bx, by, bz, fx, fy, fz;
+  float2 mm;
   if (x <= g.nx && y <= g.ny - 1 && z <= g.nz - 1) {
-    traceT<FAST, false>(g, B, T, V, k, 0, x, y, z, (float)x, (float)y + 0.5f, (float)z + 0.5f, bx, by, bz, fx, fy, fz);
-    hu[i] = triCT<FAST>(g, T, 0, V.u, bx, by - 0.5f, bz - 0.5f, g.nx, g.ny - 1, g.nz - 1);
+    traceT<FAST, true, false>(g, B, T, V, k, 0, x, y, z, (float)x, (float)y + 0.5f, (float)z + 0.5f, bx, by, bz, fx,
+                              fy, fz);
+    hu[i] = triMMCT<FAST>(g, T, 0, V.u, bx, by - 0.5f, bz - 0.5f, g.nx, g.ny - 1, g.nz - 1, &mm);
+    mu[i] = mm;
   }
   if (x <= g.nx - 1 && y <= g.ny && z <= g.nz - 1) {
-    traceT<FAST, false>(g, B, T, V, k, 1, x, y, z, (float)x + 0.5f, (float)y, (float)z + 0.5f, bx, by, bz, fx, fy, fz);
-    hv[i] = triCT<FAST>(g, T, 1, V.v, bx - 0.5f, by, bz - 0.5f, g.nx - 1, g.ny, g.nz - 1);
+    traceT<FAST, true, false>(g, B, T, V, k, 1, x, y, z, (float)x + 0.5f, (float)y, (float)z + 0.5f, bx, by, bz, fx,
+                              fy, fz);
+    hv[i] = triMMCT<FAST>(g, T, 1, V.v, bx - 0.5f, by, bz - 0.5f, g.nx - 1, g.ny, g.nz - 1, &mm);
+    mv[i] = mm;
   }
   if (x <= g.nx - 1 && y <= g.ny - 1 && z <= g.nz) {
-    traceT<FAST, false>(g, B, T, V, k, 2, x, y, z, (float)x + 0.5f, (float)y + 0.5f, (float)z, bx, by, bz, fx, fy, fz);
-    hw[i] = triCT<FAST>(g, T, 2, V.w, bx - 0.5f, by - 0.5f, bz, g.nx - 1, g.ny - 1, g.nz);
+    traceT<FAST, true, false>(g, B, T, V, k, 2, x, y, z, (float)x + 0.5f, (float)y + 0.5f, (float)z, bx, by, bz, fx,
+                              fy, fz);
+    hw[i] = triMMCT<FAST>(g, T, 2, V.w, bx - 0.5f, by - 0.5f, bz, g.nx - 1, g.ny - 1, g.nz, &mm);
+    mw[i] = mm;
   }
 }
 __global__ __launch_bounds__(NT) void k_adv_vel1(const Params *__restrict__ P, Geo g, VelPtrs VP,
                                                   float *__restrict__ hu, float *__restrict__ hv,
-                                                  float *__restrict__ hw) {
+                                                  float *__restrict__ hw, float2 *__restrict__ mu,
+                                                  float2 *__restrict__ mv, float2 *__restrict__ mw) {
   __shared__ float sm[3 * VT_N];
   __shared__ uint32_t smax;
   const uint32_t blk = (uint32_t)xcd_tile(blockIdx.x, gridDim.x);
@@ -393,12 +414,12 @@ __global__ __launch_bounds__(NT) void k_adv_vel1(const Params *__restrict__ P, G
   VTile T;
   const bool fast = stage_vtile(g, V, sm, &smax, k, x0, y0, z0, T);
   if (x >= g.vx || y >= g.vy || z > g.nz) return;
-  if (fast) adv_vel1_face<true>(P, g, T, V, k, x, y, z, hu, hv, hw);
-  else adv_vel1_face<false>(P, g, T, V, k, x, y, z, hu, hv, hw);
+  if (fast) adv_vel1_face<true>(P, g, T, V, k, x, y, z, hu, hv, hw, mu, mv, mw);
+  else adv_vel1_face<false>(P, g, T, V, k, x, y, z, hu, hv, hw, mu, mv, mw);
 }
 void launch_adv_vel1(const Fields &f, hipStream_t s) {
   VelPtrs V{f.va[0], f.va[1], f.va[2]};
-  k_adv_vel1<<<f.g.fbricks, NT, 0, s>>>(f.P, f.g, V, f.hat[0], f.hat[1], f.hat[2]);
+  k_adv_vel1<<<f.g.fbricks, NT, 0, s>>>(f.P, f.g, V, f.hat[0], f.hat[1], f.hat[2], f.mm[0], f.mm[1], f.mm[2]);
 }
 
 // ---------------------------------------------------------------- adv_vel2
@@ -409,8 +430,10 @@ __device__ __forceinline__ bool solidC(const Geo &g, const uint8_t *__restrict__
 template <bool FAST>
 __device__ __forceinline__ void adv_vel2_face(const Geo &g, const VTile &T, const VelR &V, float k, int x, int y,
                                               int z, const RB &Hu, const RB &Hv, const RB &Hw,
-                                              float *__restrict__ ou, float *__restrict__ ov,
-                                              float *__restrict__ ow, const uint8_t *__restrict__ fl) {
+                                              const float2 *__restrict__ mu, const float2 *__restrict__ mv,
+                                              const float2 *__restrict__ mw, float *__restrict__ ou,
+                                              float *__restrict__ ov, float *__restrict__ ow,
+                                              const uint8_t *__restrict__ fl) {
   const uint32_t i = vidx(g, x, y, z);
   const Bnd B = bnd_of(g);
   const bool sc = solidC(g, fl, x, y, z);
@@ -426,25 +449,26 @@ __device__ __forceinline__ void adv_vel2_face(const Geo &g, const VTile &T, cons
     const float offx = c == 0 ? 0.0f : 0.5f, offy = c == 1 ? 0.0f : 0.5f, offz = c == 2 ? 0.0f : 0.5f;
     const RB &Vc = c == 0 ? V.u : (c == 1 ? V.v : V.w);
     const RB &Hc = c == 0 ? Hu : (c == 1 ? Hv : Hw);
+    // limiter bounds at the back-trace foot, from adv_vel1
+    const float2 mm = (c == 0 ? mu : (c == 1 ? mv : mw))[i];
     const float px = (float)x + offx, py = (float)y + offy, pz = (float)z + offz;
     float bx, by, bz, fx, fy, fz;
-    traceT<FAST, true>(g, B, T, V, k, c, x, y, z, px, py, pz, bx, by, bz, fx, fy, fz);
-    bx = bx - offx; by = by - offy; bz = bz - offz;
+    traceT<FAST, false, true>(g, B, T, V, k, c, x, y, z, px, py, pz, bx, by, bz, fx, fy, fz);
     fx = fx - offx; fy = fy - offy; fz = fz - offz;
-    float lo, hi;
-    mmCT<FAST>(g, T, c, Vc, bx, by, bz, hx, hy, hz, lo, hi);
     const float t = triC(g, Hc, fx, fy, fz, hx, hy, hz);
-    O[i] = clampf_(ldf(Hc, i) + 0.5f * (ldf(Vc, i) - t), lo, hi);
+    O[i] = clampf_(ldf(Hc, i) + 0.5f * (ldf(Vc, i) - t), mm.x, mm.y);
   }
 }
 // The component loop is unrolled (each start type and offset known at
-// compile time) and the kernel held to 5 waves per SIMD (at most 102 VGPRs;
-// unbounded it takes 105 and runs 4): 466 -> 347 us at pool 2 cm.
+// compile time) and the kernel held to at least 5 waves per SIMD (at most 102
+// VGPRs; with the back trace still in it, unbounded it took 105 and ran 4
+// waves): 466 -> 347 us at pool 2 cm, and 289 us without the back trace.
 __global__ __launch_bounds__(NT, 5) void k_adv_vel2(const Params *__restrict__ P, Geo g, VelPtrs VP,
                                                   const float *__restrict__ hu, const float *__restrict__ hv,
-                                                  const float *__restrict__ hw, float *__restrict__ ou,
-                                                  float *__restrict__ ov, float *__restrict__ ow,
-                                                  const uint8_t *__restrict__ fl) {
+                                                  const float *__restrict__ hw, const float2 *__restrict__ mu,
+                                                  const float2 *__restrict__ mv, const float2 *__restrict__ mw,
+                                                  float *__restrict__ ou, float *__restrict__ ov,
+                                                  float *__restrict__ ow, const uint8_t *__restrict__ fl) {
   __shared__ float sm[3 * VT_N];
   __shared__ uint32_t smax;
   const uint32_t blk = (uint32_t)xcd_tile(blockIdx.x, gridDim.x);
@@ -457,13 +481,13 @@ __global__ __launch_bounds__(NT, 5) void k_adv_vel2(const Params *__restrict__ P
   const bool fast = stage_vtile(g, V, sm, &smax, k, x0, y0, z0, T);
   if (x >= g.vx || y >= g.vy || z > g.nz) return;
   const RB Hu = rb_of(hu, g.nv * 4u), Hv = rb_of(hv, g.nv * 4u), Hw = rb_of(hw, g.nv * 4u);
-  if (fast) adv_vel2_face<true>(g, T, V, k, x, y, z, Hu, Hv, Hw, ou, ov, ow, fl);
-  else adv_vel2_face<false>(g, T, V, k, x, y, z, Hu, Hv, Hw, ou, ov, ow, fl);
+  if (fast) adv_vel2_face<true>(g, T, V, k, x, y, z, Hu, Hv, Hw, mu, mv, mw, ou, ov, ow, fl);
+  else adv_vel2_face<false>(g, T, V, k, x, y, z, Hu, Hv, Hw, mu, mv, mw, ou, ov, ow, fl);
 }
 void launch_adv_vel2(const Fields &f, hipStream_t s) {
   VelPtrs V{f.va[0], f.va[1], f.va[2]};
-  k_adv_vel2<<<f.g.fbricks, NT, 0, s>>>(f.P, f.g, V, f.hat[0], f.hat[1], f.hat[2], f.vb[0], f.vb[1], f.vb[2],
-                                            f.flags);
+  k_adv_vel2<<<f.g.fbricks, NT, 0, s>>>(f.P, f.g, V, f.hat[0], f.hat[1], f.hat[2], f.mm[0], f.mm[1], f.mm[2],
+                                            f.vb[0], f.vb[1], f.vb[2], f.flags);
 }
 
 // ---------------------------------------------------------------- scalars
@@ -550,8 +574,8 @@ __global__ __launch_bounds__(NT) void k_adv_s1(const Params *__restrict__ P, Geo
     VTile T;
     const bool fast = stage_vtile(g, V, sm, &smax, k, x0, y0, z0, T);
     if (x >= g.nx || y >= g.ny || z >= g.nz) return;
-    if (fast) traceT<true, false>(g, bnd_of(g), T, V, k, 3, x, y, z, px, py, pz, bx, by, bz, fx, fy, fz);
-    else traceT<false, false>(g, bnd_of(g), T, V, k, 3, x, y, z, px, py, pz, bx, by, bz, fx, fy, fz);
+    if (fast) traceT<true, true, false>(g, bnd_of(g), T, V, k, 3, x, y, z, px, py, pz, bx, by, bz, fx, fy, fz);
+    else traceT<false, true, false>(g, bnd_of(g), T, V, k, 3, x, y, z, px, py, pz, bx, by, bz, fx, fy, fz);
   } else {
     if (x >= g.nx || y >= g.ny || z >= g.nz) return;
     trace<-1>(g, V, k, px, py, pz, bx, by, bz);
@@ -576,7 +600,7 @@ __device__ __forceinline__ void adv_s2_cell(const Geo &g, const VTile &T, const 
   const Bnd B = bnd_of(g);
   const float px = (float)x + 0.5f, py = (float)y + 0.5f, pz = (float)z + 0.5f;
   float bx, by, bz, fx, fy, fz;
-  traceT<FAST, true>(g, B, T, V, k, 3, x, y, z, px, py, pz, bx, by, bz, fx, fy, fz);
+  traceT<FAST, true, true>(g, B, T, V, k, 3, x, y, z, px, py, pz, bx, by, bz, fx, fy, fz);
   bx = bx - 0.5f; by = by - 0.5f; bz = bz - 0.5f;
   fx = fx - 0.5f; fy = fy - 0.5f; fz = fz - 0.5f;
   float4 r0 = ld4(h0, i), r1 = ld4(h1, i);

@@ -13,6 +13,7 @@ struct Fields {
   Geo g;
   float *va[3], *vb[3];     // velocity ping-pong: va = vel[cur], vb = vel[1-cur]
   float *hat[3];
+  float2 *mm[3];            // adv_vel1 -> adv_vel2: (min, max) of each face's back-trace corners
   float4 *s0, *s1, *s0b, *s1b, *h0, *h1, *sol_in, *sol_out, *pyr, *curl;
   uint8_t *flags;           // 1 = solid (u8 copy of the reference u32 flags)
   float *phi0, *b0;         // multigrid level 0

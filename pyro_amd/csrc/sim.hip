@@ -43,6 +43,7 @@ struct PyroSim {
   Geo g;
   float *vel[2][3] = {};
   float *hat[3] = {};
+  float2 *mm[3] = {};
   float4 *s0 = nullptr, *s1 = nullptr, *s0b = nullptr, *s1b = nullptr, *h0 = nullptr, *h1 = nullptr;
   float4 *sol[2] = {}, *pyr = nullptr, *curl = nullptr;
   uint8_t *flags = nullptr;
@@ -91,6 +92,7 @@ Fields fields_of(PyroSim *s) {
     f.va[k] = s->vel[s->cur][k];
     f.vb[k] = s->vel[1 - s->cur][k];
     f.hat[k] = s->hat[k];
+    f.mm[k] = s->mm[k];
   }
   f.s0 = s->s0; f.s1 = s->s1; f.s0b = s->s0b; f.s1b = s->s1b; f.h0 = s->h0; f.h1 = s->h1;
   f.sol_in = s->sol[s->sol_cur];
@@ -320,6 +322,7 @@ int pyro_sim_create(const void *params, size_t params_bytes, const float *sol_in
   for (int c = 0; c < 2 && !rc; c++)
     for (int k = 0; k < 3 && !rc; k++) rc |= dalloc_t(s, &s->vel[c][k], g.nv);
   for (int k = 0; k < 3 && !rc; k++) rc |= dalloc_t(s, &s->hat[k], g.nv);
+  for (int k = 0; k < 3 && !rc; k++) rc |= dalloc_t(s, &s->mm[k], g.nv);
   float4 **f4[] = {&s->s0, &s->s1, &s->s0b, &s->s1b, &s->h0, &s->h1, &s->sol[0], &s->sol[1], &s->pyr, &s->curl};
   for (auto p : f4)
     if (!rc) rc |= dalloc_t(s, p, g.n);
