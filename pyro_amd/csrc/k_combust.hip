@@ -98,10 +98,31 @@ __device__ float faceIrr(const Params *__restrict__ P, const Geo &g, const float
   return 0.2f * ((((r0 + r1) + r2) + r3) + r4);
 }
 
+// Faces of a solid cell whose irradiance the cell needs (combust.wgsl:60-75):
+// bit k set for a neighbour in direction k (dirOf) that is inside the grid,
+// not below the ground and not solid.  Inert cells (kind 2) need none.
+__device__ __forceinline__ uint32_t ray_faces(const Params *__restrict__ P, const Geo &g,
+                                              const float4 *__restrict__ solIn, int x, int y, int z, float4 so) {
+  if (kindOf(P, so) == 2u) return 0u;
+  uint32_t mask = 0u;
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    int dx_, dy_, dz_;
+    dirOf(k, dx_, dy_, dz_);
+    const int nx_ = x + dx_, ny_ = y + dy_, nz_ = z + dz_;
+    if (ny_ < 0 || !inb(g, nx_, ny_, nz_)) continue;
+    if (!isSolidS(P, solIn[sidx(g, nx_, ny_, nz_)])) mask |= 1u << k;
+  }
+  return mask;
+}
+
 // Surface energy balance and pyrolysis of one solid cell (combust.wgsl:60-128).
+// The irradiance of the cell's ray faces (ray_faces) is given: face k at
+// irr[popc(mask & ((1 << k) - 1))].
 __device__ __forceinline__ void solid_cell(const Params *__restrict__ P, const Geo &g, const float4 *__restrict__ solIn,
-                                        const float4 *__restrict__ s0, int x, int y, int z, float4 so, float4 &o,
-                                        float4 &po, uint32_t &fl, float &stPyro, float &stMass, float &stArea) {
+                                        const float4 *__restrict__ s0, int x, int y, int z, float4 so,
+                                        const float *irr, uint32_t mask, float4 &o, float4 &po, uint32_t &fl,
+                                        float &stPyro, float &stMass, float &stArea) {
   fl = 1;
   const Mat &m = P->mats[slotOf(so)];
   uint32_t kind = kindOf(P, so);
@@ -126,7 +147,7 @@ __device__ __forceinline__ void solid_cell(const Params *__restrict__ P, const G
         float4 q = s0[sidx(g, nx_, ny_, nz_)];
         Tg += q.x;
         Yo += q.z;
-        qin += faceIrr(P, g, solIn, s0, nx_, ny_, nz_, k);
+        qin += irr[__popc(mask & ((1u << k) - 1u))];
       } else {
         Tg += P->t0;
         Yo += P->yo2;
@@ -192,26 +213,71 @@ __device__ __forceinline__ void solid_cell(const Params *__restrict__ P, const G
 // the other buffer and the flag becomes 0.
 constexpr uint8_t FLAG_BURNT = 2;
 
+// The ray marches of a block's solid cells are dealt out one face per
+// thread: each thread lists its cell's ray faces in LDS, the block's threads
+// take the listed faces in turn (faceIrr: 5 rays of 10 samples), and each
+// cell then reads its faces' irradiance back in its own order.  A face's
+// value goes to its own slot, so the listing order does not matter.  (One
+// thread per cell marched up to 6 faces in turn, and its wave every face any
+// of its cells needed.)
 template <bool STATS>
 __global__ __launch_bounds__(NT) void k_solid(const Params *__restrict__ P, Geo g, const float4 *__restrict__ solIn,
                                                float4 *__restrict__ solOut, float4 *__restrict__ pyr,
                                                uint8_t *__restrict__ flags, const float4 *__restrict__ s0,
                                                double *__restrict__ part) {
-  uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
+  __shared__ uint16_t sfm[6 * NT];  // listed face: thread << 3 | k
+  __shared__ float sfi[6 * NT];     // its irradiance
+  __shared__ int snf;
+  const uint32_t c0 = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT;
+  const uint32_t i = c0 + threadIdx.x;
   float stPyro = 0.0f, stMass = 0.0f, stArea = 0.0f;
+  if (threadIdx.x == 0) snf = 0;
+  const uint8_t fprev = i < g.n ? flags[i] : 0;
+  int x = 0, y = 0, z = 0;
+  float4 so = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  uint32_t mask = 0u;
+  if (fprev == 1) {
+    so = solIn[i];
+    decode_cell(g, i, x, y, z);
+    mask = ray_faces(P, g, solIn, x, y, z, so);
+  }
+  // snf = 0 is visible after this barrier; a block without solid or burnt
+  // cells has nothing to do (its statistics partials are +0)
+  if (!__syncthreads_or(fprev != 0)) {
+    if (STATS && threadIdx.x == 0) {
+      part[blockIdx.x * 3 + 0] = 0.0;
+      part[blockIdx.x * 3 + 1] = 0.0;
+      part[blockIdx.x * 3 + 2] = 0.0;
+    }
+    return;
+  }
+  int base = 0;
+  if (mask) {
+    base = atomicAdd(&snf, __popc(mask));
+    int s = base;
+#pragma unroll
+    for (int k = 0; k < 6; k++)
+      if (mask & (1u << k)) sfm[s++] = (uint16_t)((threadIdx.x << 3) | k);
+  }
+  __syncthreads();
+  const int nf = snf;
+  for (int f = threadIdx.x; f < nf; f += NT) {
+    const int owner = sfm[f] >> 3, k = sfm[f] & 7;
+    int ox, oy, oz, dx_, dy_, dz_;
+    decode_cell(g, c0 + (uint32_t)owner, ox, oy, oz);
+    dirOf(k, dx_, dy_, dz_);
+    sfi[f] = faceIrr(P, g, solIn, s0, ox + dx_, oy + dy_, oz + dz_, k);
+  }
+  __syncthreads();
   if (i < g.n) {
-    const uint8_t fprev = flags[i];
     if (fprev == FLAG_BURNT) {
       solOut[i] = solIn[i];
       pyr[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       flags[i] = 0;
     } else if (fprev == 1) {
-      float4 so = solIn[i];
       float4 o = so, po = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       uint32_t fl = 0;
-      int x, y, z;
-      decode_cell(g, i, x, y, z);
-      solid_cell(P, g, solIn, s0, x, y, z, so, o, po, fl, stPyro, stMass, stArea);
+      solid_cell(P, g, solIn, s0, x, y, z, so, sfi + base, mask, o, po, fl, stPyro, stMass, stArea);
       solOut[i] = o;
       pyr[i] = po;
       flags[i] = fl ? 1 : FLAG_BURNT;
