@@ -82,22 +82,20 @@ __device__ __forceinline__ void dirOf(int k, int &x, int &y, int &z) {
   y = k == 2 ? 1 : (k == 3 ? -1 : 0);
   z = k == 4 ? 1 : (k == 5 ? -1 : 0);
 }
-// Hemispherical irradiance: normal ray plus 4 rays at 45 degrees (combust.wgsl:41-48).
-__device__ float faceIrr(const Params *__restrict__ P, const Geo &g, const float4 *__restrict__ solIn,
-                         const float4 *__restrict__ s0, int x, int y, int z, int k) {
-  int dx_, dy_, dz_;
+// Direction of ray r of face k (combust.wgsl:41-48): the normal (r = 0),
+// then the normal plus and minus the first tangent (r = 1, 2) and plus and
+// minus the second (r = 3, 4).
+__device__ __forceinline__ void rayDir(int k, int r, int &dx_, int &dy_, int &dz_) {
   dirOf(k, dx_, dy_, dz_);
   int t1x = 0, t1y = 1, t1z = 0, t2x = 0, t2y = 0, t2z = 1;
   if (dy_ != 0) { t1x = 1; t1y = 0; t1z = 0; }
   if (dz_ != 0) { t2x = 1; t2y = 0; t2z = 0; }
-  float r0 = rayIrr(P, g, solIn, s0, x, y, z, dx_, dy_, dz_);
-  float r1 = rayIrr(P, g, solIn, s0, x, y, z, dx_ + t1x, dy_ + t1y, dz_ + t1z);
-  float r2 = rayIrr(P, g, solIn, s0, x, y, z, dx_ - t1x, dy_ - t1y, dz_ - t1z);
-  float r3 = rayIrr(P, g, solIn, s0, x, y, z, dx_ + t2x, dy_ + t2y, dz_ + t2z);
-  float r4 = rayIrr(P, g, solIn, s0, x, y, z, dx_ - t2x, dy_ - t2y, dz_ - t2z);
-  return 0.2f * ((((r0 + r1) + r2) + r3) + r4);
+  const int s = (r == 1 || r == 3) ? 1 : ((r == 2 || r == 4) ? -1 : 0);
+  const bool second = r >= 3;
+  dx_ += s * (second ? t2x : t1x);
+  dy_ += s * (second ? t2y : t1y);
+  dz_ += s * (second ? t2z : t1z);
 }
-
 // Faces of a solid cell whose irradiance the cell needs (combust.wgsl:60-75):
 // bit k set for a neighbour in direction k (dirOf) that is inside the grid,
 // not below the ground and not solid.  Inert cells (kind 2) need none.
@@ -213,13 +211,14 @@ __device__ __forceinline__ void solid_cell(const Params *__restrict__ P, const G
 // the other buffer and the flag becomes 0.
 constexpr uint8_t FLAG_BURNT = 2;
 
-// The ray marches of a block's solid cells are dealt out one face per
+// The ray marches of a block's solid cells are dealt out one ray per
 // thread: each thread lists its cell's ray faces in LDS, the block's threads
-// take the listed faces in turn (faceIrr: 5 rays of 10 samples), and each
-// cell then reads its faces' irradiance back in its own order.  A face's
-// value goes to its own slot, so the listing order does not matter.  (One
-// thread per cell marched up to 6 faces in turn, and its wave every face any
-// of its cells needed.)
+// take the listed faces' rays in turn (5 rays of 10 samples per face), each
+// face sums its rays as faceIrr does (combust.wgsl:41-48), and each cell then
+// reads its faces' irradiance back in its own order.  A face's value goes to
+// its own slot, so the listing order does not matter.  (One thread per cell
+// marched up to 6 faces in turn, and its wave every face any of its cells
+// needed.)
 template <bool STATS>
 __global__ __launch_bounds__(NT) void k_solid(const Params *__restrict__ P, Geo g, const float4 *__restrict__ solIn,
                                                float4 *__restrict__ solOut, float4 *__restrict__ pyr,
@@ -227,6 +226,7 @@ __global__ __launch_bounds__(NT) void k_solid(const Params *__restrict__ P, Geo 
                                                double *__restrict__ part) {
   __shared__ uint16_t sfm[6 * NT];  // listed face: thread << 3 | k
   __shared__ float sfi[6 * NT];     // its irradiance
+  __shared__ float sray[5 * NT];    // the rays of a batch of NT faces
   __shared__ int snf;
   const uint32_t c0 = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT;
   const uint32_t i = c0 + threadIdx.x;
@@ -260,15 +260,27 @@ __global__ __launch_bounds__(NT) void k_solid(const Params *__restrict__ P, Geo 
       if (mask & (1u << k)) sfm[s++] = (uint16_t)((threadIdx.x << 3) | k);
   }
   __syncthreads();
+  // the listed faces in batches of NT: the batch's rays one per thread, then
+  // each face's five rays summed in faceIrr's order
   const int nf = snf;
-  for (int f = threadIdx.x; f < nf; f += NT) {
-    const int owner = sfm[f] >> 3, k = sfm[f] & 7;
-    int ox, oy, oz, dx_, dy_, dz_;
-    decode_cell(g, c0 + (uint32_t)owner, ox, oy, oz);
-    dirOf(k, dx_, dy_, dz_);
-    sfi[f] = faceIrr(P, g, solIn, s0, ox + dx_, oy + dy_, oz + dz_, k);
+  for (int f0 = 0; f0 < nf; f0 += NT) {
+    const int nr = 5 * (nf - f0 < NT ? nf - f0 : NT);
+    for (int j = threadIdx.x; j < nr; j += NT) {
+      const int f = f0 + j / 5, r = j - 5 * (j / 5);
+      const int owner = sfm[f] >> 3, k = sfm[f] & 7;
+      int ox, oy, oz, nx_, ny_, nz_, dx_, dy_, dz_;
+      decode_cell(g, c0 + (uint32_t)owner, ox, oy, oz);
+      dirOf(k, nx_, ny_, nz_);
+      rayDir(k, r, dx_, dy_, dz_);
+      sray[j] = rayIrr(P, g, solIn, s0, ox + nx_, oy + ny_, oz + nz_, dx_, dy_, dz_);
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < nr / 5; j += NT) {
+      const float *q = sray + 5 * j;
+      sfi[f0 + j] = 0.2f * ((((q[0] + q[1]) + q[2]) + q[3]) + q[4]);
+    }
+    __syncthreads();
   }
-  __syncthreads();
   if (i < g.n) {
     if (fprev == FLAG_BURNT) {
       solOut[i] = solIn[i];
