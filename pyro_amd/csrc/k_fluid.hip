@@ -82,9 +82,9 @@ __device__ __forceinline__ float spongeTop(const Params *__restrict__ P, const G
 // the brick grown by one on the low side of each axis: C_X x C_Y x C_Z =
 // 17 x 5 x 5.  Their confinement needs |curl| one cell further out on both
 // sides (19 x 7 x 7, coordinates clamped to the grid as fluid.wgsl's
-// curl lookups are).  Both tiles are
-// staged in LDS with all global loads in flight together, the confinement
-// of each tile cell is evaluated once (a face triple used to evaluate four),
+// curl lookups are).  Both tiles are staged in LDS (the |curl| tile only by
+// a block where some cell's confinement gate is open), the confinement of
+// each tile cell is evaluated once (a face triple used to evaluate four),
 // and the faces then read T, smoke, solid flags and forces from LDS.
 constexpr int FC_X = BR_X + 1, FC_Y = BR_Y + 1, FC_Z = BR_Z + 1, FC_N = FC_X * FC_Y * FC_Z;
 constexpr int FW_X = BR_X + 3, FW_Y = BR_Y + 3, FW_Z = BR_Z + 3, FW_N = FW_X * FW_Y * FW_Z;
@@ -93,10 +93,12 @@ constexpr int FC_R = (FC_N + NT - 1) / NT, FW_R = (FW_N + NT - 1) / NT;
 // Vorticity confinement at a cell centre (fluid.wgsl:167-181) from its T,
 // smoke and curl w; m points at the cell's |curl| in the |curl| tile, whose
 // entries m[+-1], m[+-FW_X], m[+-FW_X FW_Y] are the clamped neighbours.
-__device__ __forceinline__ void confine_tile(const Params *__restrict__ P, float T, float smoke, float4 w,
-                                             const float *m, float &fx, float &fy, float &fz) {
+__device__ __forceinline__ float confine_gate(const Params *__restrict__ P, float T, float smoke) {
+  return fmax_(smoothstep_(P->vort_t, P->vort_t + 500.0f, T), 0.35f * smoothstep_(0.0f, 2e-4f, smoke));
+}
+__device__ __forceinline__ void confine_tile(const Params *__restrict__ P, float gate, float4 w, const float *m,
+                                             float &fx, float &fy, float &fz) {
   fx = fy = fz = 0.0f;
-  float gate = fmax_(smoothstep_(P->vort_t, P->vort_t + 500.0f, T), 0.35f * smoothstep_(0.0f, 2e-4f, smoke));
   if (gate <= 0.0f) return;
   float gx = m[1] - m[-1];
   float gy = m[FW_X] - m[-FW_X];
@@ -132,41 +134,31 @@ __global__ __launch_bounds__(NT) void k_forces(const Params *__restrict__ P, Geo
     y0 = (int)((r - bz * g.dfby.d) * BR_Y);
     z0 = (int)(bz * BR_Z);
   }
-  // every load of the block is issued before the first LDS store: |curl| at
-  // clamped coordinates, the tile cells' state at clamped coordinates (a cell
-  // outside the grid is flagged and never read), the face's velocities
-  float wl[FW_R];
-#pragma unroll
-  for (int j = 0; j < FW_R; j++) {
-    const int e = (int)threadIdx.x + NT * j;
-    wl[j] = 0.0f;
-    if (e < FW_N) {
-      const int lz = e / (FW_X * FW_Y), r = e - lz * (FW_X * FW_Y), ly = r / FW_X, lx = r - ly * FW_X;
-      const int gx = clampi(x0 - 2 + lx, 0, g.nx - 1), gy = clampi(y0 - 2 + ly, 0, g.ny - 1),
-                gz = clampi(z0 - 2 + lz, 0, g.nz - 1);
-      wl[j] = curl[sidx(g, gx, gy, gz)].w;
-    }
-  }
-  float4 ca[FC_R], cw[FC_R];
-  float cs1[FC_R];
+  // the tile cells' state at clamped coordinates (a cell outside the grid is
+  // flagged and never read) and the face's velocities, all loads in flight
+  // together
+  float4 ca[FC_R];
+  float cs1[FC_R], gate[FC_R];
   uint8_t cf[FC_R];
   bool cin[FC_R];
+  uint32_t cidx[FC_R];
+  bool open = false;
 #pragma unroll
   for (int j = 0; j < FC_R; j++) {
     const int e = (int)threadIdx.x + NT * j;
     cin[j] = false;
-    ca[j] = cw[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    ca[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     cs1[j] = 0.0f;
     cf[j] = 0;
+    cidx[j] = 0u;
     if (e < FC_N) {
       const int lz = e / (FC_X * FC_Y), r = e - lz * (FC_X * FC_Y), ly = r / FC_X, lx = r - ly * FC_X;
       const int gx = x0 - 1 + lx, gy = y0 - 1 + ly, gz = z0 - 1 + lz;
       cin[j] = inb(g, gx, gy, gz);
-      const uint32_t c = sidx(g, clampi(gx, 0, g.nx - 1), clampi(gy, 0, g.ny - 1), clampi(gz, 0, g.nz - 1));
-      ca[j] = s0[c];
-      cs1[j] = s1[c].x;
-      cw[j] = curl[c];
-      cf[j] = fl[c];
+      cidx[j] = sidx(g, clampi(gx, 0, g.nx - 1), clampi(gy, 0, g.ny - 1), clampi(gz, 0, g.nz - 1));
+      ca[j] = s0[cidx[j]];
+      cs1[j] = s1[cidx[j]].x;
+      cf[j] = fl[cidx[j]];
     }
   }
   const bool face = x < g.vx && y < g.vy && z <= g.nz;
@@ -176,24 +168,50 @@ __global__ __launch_bounds__(NT) void k_forces(const Params *__restrict__ P, Geo
   uvw[1] = V[i];
   uvw[2] = W[i];
 #pragma unroll
-  for (int j = 0; j < FW_R; j++) {
-    const int e = (int)threadIdx.x + NT * j;
-    if (e < FW_N) sw[e] = wl[j];
+  for (int j = 0; j < FC_R; j++) {
+    gate[j] = confine_gate(P, ca[j].x, ca[j].w + cs1[j]);
+    open |= cin[j] && gate[j] > 0.0f;
   }
-  __syncthreads();
+  // confinement acts only where the gate is open (fluid.wgsl:167-181 returns
+  // 0 otherwise): a block none of whose cells has it open stages no |curl|
+  float4 cw[FC_R];
+#pragma unroll
+  for (int j = 0; j < FC_R; j++) cw[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (__syncthreads_or(open)) {
+    float wl[FW_R];
+#pragma unroll
+    for (int j = 0; j < FW_R; j++) {
+      const int e = (int)threadIdx.x + NT * j;
+      wl[j] = 0.0f;
+      if (e < FW_N) {
+        const int lz = e / (FW_X * FW_Y), r = e - lz * (FW_X * FW_Y), ly = r / FW_X, lx = r - ly * FW_X;
+        const int gx = clampi(x0 - 2 + lx, 0, g.nx - 1), gy = clampi(y0 - 2 + ly, 0, g.ny - 1),
+                  gz = clampi(z0 - 2 + lz, 0, g.nz - 1);
+        wl[j] = curl[sidx(g, gx, gy, gz)].w;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < FC_R; j++)
+      if (cin[j] && gate[j] > 0.0f) cw[j] = curl[cidx[j]];
+#pragma unroll
+    for (int j = 0; j < FW_R; j++) {
+      const int e = (int)threadIdx.x + NT * j;
+      if (e < FW_N) sw[e] = wl[j];
+    }
+    __syncthreads();
+  }
 #pragma unroll
   for (int j = 0; j < FC_R; j++) {
     const int e = (int)threadIdx.x + NT * j;
     if (e >= FC_N) continue;
     const int lz = e / (FC_X * FC_Y), r = e - lz * (FC_X * FC_Y), ly = r / FC_X, lx = r - ly * FC_X;
-    const float T = ca[j].x, smoke = ca[j].w + cs1[j];
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    if (cin[j]) confine_tile(P, T, smoke, cw[j], sw + (lx + 1) + FW_X * ((ly + 1) + FW_Y * (lz + 1)), fx, fy, fz);
+    if (cin[j]) confine_tile(P, gate[j], cw[j], sw + (lx + 1) + FW_X * ((ly + 1) + FW_Y * (lz + 1)), fx, fy, fz);
     sf[0][e] = fx;
     sf[1][e] = fy;
     sf[2][e] = fz;
-    st[e] = T;
-    sm[e] = smoke;
+    st[e] = ca[j].x;
+    sm[e] = ca[j].w + cs1[j];
     sc[e] = (uint8_t)((cin[j] ? 1u : 0u) | (cin[j] && cf[j] == 1 ? 2u : 0u));
   }
   __syncthreads();
