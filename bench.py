@@ -373,7 +373,8 @@ def pressure_frame(sim, args, dx, net=None):
     the reference V-cycles (plain MG as the reference runs it), FP64-residual
     refinement (plain MG iteration on the consistent hierarchy), MGPCG, and
     MGPCG with the learned network (every preconditioner application, or
-    the first only)."""
+    the first only; on the fine grid, or on level 1 of the preconditioner
+    hierarchy: "coarse")."""
     import pyro_amd as pa
     from pyro_amd.poisson import Poisson
 
@@ -386,11 +387,12 @@ def pressure_frame(sim, args, dx, net=None):
            "initial_rel_resid": P.residual()}
     runs = [("vcycle", None, 400), ("refine", None, 100), ("mgpcg", None, 100)]
     if net is not None:
-        runs += [("learned", "every", 100), ("learned", "first", 100)]
+        runs += [("learned", "every", 100), ("learned", "first", 100), ("learned", "coarse-every", 100),
+                 ("learned", "coarse-first", 100)]
     for m, mode, cap in runs:
         if mode:
             P.attach_net(net, mode)
-        key = m if mode is None else f"{m}_{mode}"
+        key = m if mode is None else f"{m}_{mode.replace('-', '_')}"
         P.set_from_sim(sim, warm=True, flags=False)
         P.solve(m, 1e-4, min(cap, 20))  # first call: allocation / code-object warm-up
         P.set_from_sim(sim, warm=True, flags=False)
@@ -508,8 +510,8 @@ def bonfire_c5(args, net):
     (fixed 4 V-cycles), and pressure solves to 1e-4 by the reference V-cycles
     (at most 50) on b as the step forms it and on the consistent b (mean
     removed on sealed pockets inside the log pile, where heat release makes
-    the system inconsistent), refinement, MGPCG and learned MGPCG (consistent
-    b); wall time per step, solver iterations and the FP64 statistics (HRR,
+    the system inconsistent), refinement, MGPCG and learned MGPCG (network in the first application
+    on the fine grid, or in every application on level 1; consistent b); wall time per step, solver iterations and the FP64 statistics (HRR,
     pyrolysis, solid mass)."""
     import tempfile
 
@@ -535,9 +537,10 @@ def bonfire_c5(args, net):
     runs = [("reference_step", None, None), ("vcycle", "vcycle", 50), ("vcycle_consistent", "vcycle", 50),
             ("refine", "refine", 100), ("mgpcg", "mgpcg", 100)]
     if net is not None:
-        runs.append(("learned_first", "learned", 100))
-        P.attach_net(net, "first")
+        runs += [("learned_first", "learned", 100), ("learned_coarse_every", "learned", 100)]
     for key, method, cap in runs:
+        if key.startswith("learned"):
+            P.attach_net(net, key[len("learned_"):].replace("_", "-"))
         # every method but the reference V-cycles solves the consistent
         # problem (b minus its mean on sealed pockets); vcycle_consistent is
         # the reference V-cycle on that same problem

@@ -294,7 +294,9 @@ int pyro_net_eval(PyroNet *net, int first, int count, double *mean_loss);
 /* Average device time (HIP events) of `reps` forward passes on a grid. */
 int pyro_net_time_forward(PyroNet *net, uint32_t nx, uint32_t ny, uint32_t nz, int reps, float *avg_ms);
 /* mode 0: the network in every preconditioner application; 1: in the first
- * only (a learned warm start).  net = NULL detaches. */
+ * only (a learned warm start); 2 and 3: the same, with the network applied to
+ * the residual restricted to level 1 of the preconditioner hierarchy and its
+ * output prolonged back (1/8 of the voxels).  net = NULL detaches. */
 int pyro_poisson_attach_net(PyroPoisson *p, PyroNet *net, int mode);
 
 /* Test hooks of the network kernels.  mfma: one v_mfma_f32_16x16x32_bf16 of

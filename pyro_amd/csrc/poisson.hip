@@ -58,7 +58,10 @@ struct PyroPoisson {
   std::vector<void *> allocs;
   int coarse_iters_pc = 80;
   PyroNet *net = nullptr;
-  int learned_mode = 0;  // 0: network in every preconditioner application, 1: first only
+  // bit 0: the network in the first preconditioner application only (else in
+  // every one); bit 1: the network on level 1 of the preconditioner hierarchy
+  int learned_mode = 0;
+  double nfluid1 = -1.0;  // fluid cells of level 1, counted when first needed (-1: not yet)
   // sealed fluid components (pure-Neumann pockets): b is made consistent by
   // subtracting its mean over each (segment k of seal_idx: cells
   // seal_off[k] .. seal_off[k+1]-1), once per right-hand side
@@ -124,29 +127,6 @@ __global__ __launch_bounds__(256) void k_flags_from_sim(uint32_t n, const uint8_
   if (i < n) f[i] = src[i] == 1 ? 1 : 0;
 }
 
-// s * N(r) + V(r - s A N(r)) with s = (N(r).r) / (N(r).A N(r)), the step that
-// minimises the energy norm of the error along N(r) (A is negative definite,
-// so both dots are negative for a useful direction and s > 0).  ok = false,
-// z untouched, if the direction is unusable (s not positive and finite).
-int learned_precond(PyroPoisson *P, const double *r, double rr, double *z, bool *ok) {
-  *ok = false;
-  const G3 g{P->d[0], P->d[1], P->d[2]};
-  int rc = net_apply64(P->net, g, r, P->pc[0].f, rr, P->nfluid, P->h, P->w, P->stream);
-  if (rc) return pfail(rc, std::string("learned preconditioner: ") + net_error());
-  launch_apply64(P->pc[0], P->w, P->t, P->stream);
-  double d[2];
-  rc = dots(P, {{P->w, r}, {P->w, P->t}}, d);
-  if (rc) return rc;
-  const double sc = d[1] != 0.0 ? d[0] / d[1] : 0.0;
-  if (!(sc > 0.0) || !std::isfinite(sc)) return 0;
-  // t = r - s A w, then z = V(t) + s w
-  launch_xpby64(P->n, r, -sc, P->t, P->stream);
-  precond(P, P->t, z);
-  launch_axpy64(P->n, sc, P->w, z, P->stream);
-  *ok = true;
-  return 0;
-}
-
 // number of fluid cells: per-wave ballots summed in LDS, one integer atomic
 // per block (exact and order-independent)
 constexpr int COUNT_BLOCKS = 512;
@@ -163,6 +143,69 @@ __global__ __launch_bounds__(256) void k_count_fluid(uint32_t n, const uint8_t *
   __syncthreads();
   if (threadIdx.x == 0) atomicAdd(cnt, sh[0] + sh[1] + sh[2] + sh[3]);
 }
+
+// s * N(r) + V(r - s A N(r)) with s = (N(r).r) / (N(r).A N(r)), the step that
+// minimises the energy norm of the error along N(r) (A is negative definite,
+// so both dots are negative for a useful direction and s > 0).  ok = false,
+// z untouched, if the direction is unusable (s not positive and finite).
+// With bit 1 of the mode the direction is Pr N(R r) instead of N(r): R the
+// restriction to level 1 of the preconditioner hierarchy (the mean of r over
+// each coarse cell's fluid children), N applied there (spacing 2h, 1/8 of the
+// voxels), Pr the V-cycle's masked trilinear prolongation.
+int learned_precond(PyroPoisson *P, const double *r, double rr, double *z, bool *ok) {
+  *ok = false;
+  int rc = 0;
+  if ((P->learned_mode & 2) && P->nl_pc > 1) {
+    MgLevel &L0 = P->pc[0], &L1 = P->pc[1];
+    const G3 g1{L1.d[0], L1.d[1], L1.d[2]};
+    if (P->nfluid1 < 0.0) {
+      unsigned int c = 0;
+      PCHK(hipMemsetAsync(P->count, 0, sizeof(unsigned int), P->stream));
+      k_count_fluid<<<COUNT_BLOCKS, 256, 0, P->stream>>>(L1.n, L1.f, P->count);
+      PCHK(hipMemcpyAsync(&c, P->count, sizeof(c), hipMemcpyDeviceToHost, P->stream));
+      rc = sync(P);
+      if (rc) return rc;
+      P->nfluid1 = (double)c;
+    }
+    // L1.b = R r (restriction of the residual of phi = 0, b = r), into t as FP64
+    launch_to_f32_zero(P->n, r, L0.b, L0.phi, P->stream);
+    launch_mg_restrict(L0, L1, P->stream);
+    launch_to_f64(L1.n, L1.b, P->t, P->stream);
+    double rr1 = 0.0;
+    {
+      const double *a[1] = {P->t}, *b[1] = {P->t};
+      launch_dots(a, b, 1, L1.n, P->part, P->dots, P->stream);
+      PCHK(hipMemcpyAsync(&rr1, P->dots, sizeof(double), hipMemcpyDeviceToHost, P->stream));
+      rc = sync(P);
+      if (rc) return rc;
+    }
+    if (!(rr1 > 0.0) || !(P->nfluid1 > 0.0)) return 0;
+    rc = net_apply64(P->net, g1, P->t, L1.f, rr1, P->nfluid1, (double)L1.h, P->w, P->stream);
+    if (rc) return pfail(rc, std::string("learned preconditioner: ") + net_error());
+    // w = Pr N(R r): the level-1 direction through FP32 phi and the prolongation
+    launch_to_f32(L1.n, P->w, L1.phi, P->stream);
+    launch_zero32(P->n, L0.phi, P->stream);
+    launch_mg_prolong(L1, L0, P->stream);
+    launch_to_f64(P->n, L0.phi, P->w, P->stream);
+  } else {
+    const G3 g{P->d[0], P->d[1], P->d[2]};
+    rc = net_apply64(P->net, g, r, P->pc[0].f, rr, P->nfluid, P->h, P->w, P->stream);
+    if (rc) return pfail(rc, std::string("learned preconditioner: ") + net_error());
+  }
+  launch_apply64(P->pc[0], P->w, P->t, P->stream);
+  double d[2];
+  rc = dots(P, {{P->w, r}, {P->w, P->t}}, d);
+  if (rc) return rc;
+  const double sc = d[1] != 0.0 ? d[0] / d[1] : 0.0;
+  if (!(sc > 0.0) || !std::isfinite(sc)) return 0;
+  // t = r - s A w, then z = V(t) + s w
+  launch_xpby64(P->n, r, -sc, P->t, P->stream);
+  precond(P, P->t, z);
+  launch_axpy64(P->n, sc, P->w, z, P->stream);
+  *ok = true;
+  return 0;
+}
+
 
 // Sealed components of the level-0 fluid cells (launch_cc_label), sorted on
 // the host into per-component segments of ascending cell index.  The union-
@@ -225,6 +268,7 @@ int set_flags_device(PyroPoisson *P, const uint8_t *sim_flags) {
   int rc = sync(P);
   if (rc) return rc;
   P->nfluid = (double)c[0];
+  P->nfluid1 = -1.0;
   if (c[1] || !P->labelled) return relabel(P);
   return 0;
 }
@@ -440,7 +484,7 @@ int pyro_poisson_solve(PyroPoisson *P, int method, double tol, int max_iters, Py
         int e = learned_precond(P, P->r, rr_now, P->z, &ok);
         if (e) return e;
         if (ok) {
-          if (P->learned_mode == 1) use_net = false;
+          if (P->learned_mode & 1) use_net = false;
           if (with_dot) {
             const double *a[1] = {P->z}, *b[1] = {P->q};
             launch_dots(a, b, 1, n, P->part, P->dev + 3, P->stream);
@@ -524,7 +568,7 @@ int pyro_poisson_get_rhs(PyroPoisson *P, double *b) {
 }
 
 int pyro_poisson_attach_net(PyroPoisson *P, PyroNet *net, int mode) {
-  if (!P || (mode != 0 && mode != 1)) return pfail(PYRO_E_INVALID, "bad argument");
+  if (!P || mode < 0 || mode > 3) return pfail(PYRO_E_INVALID, "bad argument");
   P->net = net;
   P->learned_mode = mode;
   return 0;

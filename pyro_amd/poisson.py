@@ -149,12 +149,16 @@ class Poisson:
         _check(_lib().pyro_poisson_solve(self._h, METHODS[method], float(tol), int(max_iters), C.byref(info)))
         return SolveInfo._of(info)
 
+    LEARNED_MODES = {"every": 0, "first": 1, "coarse-every": 2, "coarse-first": 3}
+
     def attach_net(self, net, mode: str = "every") -> None:
         """Use `net` (a pyro_amd.net.Net) in the "learned" method: in every
-        preconditioner application ("every") or only the first ("first")."""
+        preconditioner application ("every") or only the first ("first");
+        "coarse-every" / "coarse-first" apply it to the residual restricted to
+        level 1 of the preconditioner hierarchy and prolong its output."""
         self._net = net
         _check(_lib().pyro_poisson_attach_net(self._h, None if net is None else net._h,
-                                              {"every": 0, "first": 1}[mode]))
+                                              self.LEARNED_MODES[mode]))
 
     def set_from_sim(self, sim, warm: bool = True, flags: bool = True) -> None:
         """b = the simulation's current level-0 right-hand side; phi = its

@@ -174,6 +174,14 @@ def test_training_lowers_loss_and_learned_solver_converges():
     assert li.iterations < mi.iterations, (li, mi)
     fluid = flags == 0
     assert rel(P.get()[fluid], phi[fluid]) < 1e-6
+    # the network on level 1 of the preconditioner hierarchy (restricted
+    # residual in, prolonged direction out) reaches the same tolerance
+    P.attach_net(net, "coarse-every")
+    P.set(phi=np.zeros_like(b))
+    ci = P.solve("learned", 1e-8, 100)
+    print("learned on level 1", ci)
+    assert ci.rel_resid <= 1e-8
+    assert rel(P.get()[fluid], phi[fluid]) < 1e-6
 
 
 def test_training_is_deterministic():
@@ -207,7 +215,8 @@ def test_learned_solver_falls_back_to_vcycles():
     net.set_params(np.zeros(net.n, np.float32))
     net.output_scale = 1.0
     P = Poisson(flags, 1.0 / n)
-    P.attach_net(net)
-    P.set(b=b, phi=np.zeros_like(b))
-    info = P.solve("learned", 1e-8, 100)
-    assert info.fallbacks == 1 and info.rel_resid <= 1e-8
+    for mode in ("every", "coarse-every"):
+        P.attach_net(net, mode)
+        P.set(b=b, phi=np.zeros_like(b))
+        info = P.solve("learned", 1e-8, 100)
+        assert info.fallbacks == 1 and info.rel_resid <= 1e-8, (mode, info)
