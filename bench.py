@@ -31,7 +31,7 @@ Extra fields:
                    burning pool, so the timed runs are not disturbed.
   cpu_baseline  -- the CPU restatement in oracle/ (kind "port") timed on this
                    host on a bounded sample of the same workload, on the
-                   host's CPU share (OMP_NUM_THREADS, 16 on the GPU box,
+                   host's CPU share (OMP_NUM_THREADS, 16 on the MI355X host,
                    ``host_nproc`` beside it) and on one core.
   pool_4cm      -- the small pool (80 x 160 x 80) on the GPU, 20 timed steps.
   stage_ms      -- per-stage time of one step (HIP events, direct launches).
@@ -454,8 +454,8 @@ def cpu_baseline(args, dims):
     import oracle
 
     nproc = os.cpu_count() or 1
-    # this host's CPU share: OMP_NUM_THREADS where the box sets it (16 per
-    # GPU on the MI355X boxes, whatever nproc shows), else every core
+    # this host's CPU share: OMP_NUM_THREADS where the host sets it (16 per
+    # GPU on the MI355X hosts, whatever nproc shows), else every core
     threads = max(1, min(int(os.environ.get("OMP_NUM_THREADS", nproc)), nproc))
     oracle.set_threads(threads)
     scene = make_pool(args.cell)
