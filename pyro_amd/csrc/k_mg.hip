@@ -105,15 +105,25 @@ __device__ __forceinline__ float div_count(float x, float c, float r) {
 constexpr int RS_TX = 16, RS_TY = 4, RS_TZ = 4;
 constexpr int RS_FX = 2 * RS_TX + 2, RS_FY = 2 * RS_TY + 2, RS_FZ = 2 * RS_TZ + 2;
 constexpr int RS_FN = RS_FX * RS_FY * RS_FZ;
+// VEC (fine x-size a multiple of 4, arrays below 2 GiB): the footprint's x
+// range is widened to 16-byte alignment, [2 x0 - 4, 2 x0 + 36), and staged
+// as whole float4 / uchar4 groups: thread t < 250 loads group t % 10 of the
+// rows t / 10 + 25 it, it = 0..3 (a group lies entirely inside or outside
+// the level; outside reads 0 and codes 0x40).
+constexpr int RS_VX = 2 * RS_TX + 8, RS_VQ = RS_VX / 4, RS_VN = RS_VX * RS_FY * RS_FZ;
+static_assert(RS_VQ * 25 == 250 && RS_FY * RS_FZ == 100, "restriction staging map");
+template <bool VEC>
 __global__ __launch_bounds__(NT) void k_mg_restrict(LvArgs F, int ntx, int nty, const float *__restrict__ pf,
                                                      const float *__restrict__ bf, const uint8_t *__restrict__ ncf,
                                                      int c0, int c1, int c2, float *__restrict__ bc,
                                                      float *__restrict__ pc) {
-  __shared__ float sp[RS_FN];
-  __shared__ uint8_t sc[RS_FN];
+  constexpr int FX = VEC ? RS_VX : RS_FX;
+  __shared__ __align__(16) float sp[VEC ? RS_VN : RS_FN];
+  __shared__ __align__(16) uint8_t sc[VEC ? RS_VN : RS_FN];
   const int bid = xcd_tile(blockIdx.x, gridDim.x);
   const int tx = bid % ntx, rr = bid / ntx, ty = rr % nty, tz = rr / nty;
-  const int fx0 = 2 * RS_TX * tx - 1, fy0 = 2 * RS_TY * ty - 1, fz0 = 2 * RS_TZ * tz - 1;  // footprint origin
+  // footprint origin
+  const int fx0 = 2 * RS_TX * tx - (VEC ? 4 : 1), fy0 = 2 * RS_TY * ty - 1, fz0 = 2 * RS_TZ * tz - 1;
   const int sy = F.d0, sz = F.d0 * F.d1;
   const int lcx = threadIdx.x % RS_TX, lr = threadIdx.x / RS_TX, lcy = lr % RS_TY, lcz = lr / RS_TY;
   const int x = RS_TX * tx + lcx, y = RS_TY * ty + lcy, z = RS_TZ * tz + lcz;
@@ -127,21 +137,56 @@ __global__ __launch_bounds__(NT) void k_mg_restrict(LvArgs F, int ntx, int nty, 
     const bool in = a < F.d0 && bb < F.d1 && c < F.d2;
     bv[k] = bf[in ? a + sy * bb + sz * c : 0];
   }
-  // staging: every load is unconditional (outside the level at element 0)
-  // and selected afterwards, so all of a thread's loads are in flight at once
+  if (VEC) {
+    const RB rp = rb_of(pf, F.n * 4u), rcd = rb_of(ncf, F.n);
+    const int t = threadIdx.x, q = t % RS_VQ, r0 = t / RS_VQ;
+    const int gx = fx0 + 4 * q;
+    float4 pv[4];
+    uint32_t cv[4];
 #pragma unroll
-  for (int q = 0; q < (RS_FN + NT - 1) / NT; q++) {
-    const int s = threadIdx.x + q * NT;
-    const int se = s < RS_FN ? s : RS_FN - 1;
-    const int lx = se % RS_FX, r2 = se / RS_FX, ly = r2 % RS_FY, lz = r2 / RS_FY;
-    const int gx = fx0 + lx, gy = fy0 + ly, gz = fz0 + lz;
-    const bool in = gx >= 0 && gx < F.d0 && gy >= 0 && gy < F.d1 && gz >= 0 && gz < F.d2;
-    const int g = in ? gx + sy * gy + sz * gz : 0;
-    const uint8_t c = ncf[g];
-    const float p = pf[g];
-    if (s < RS_FN) {
-      sp[s] = (in && !(c & NC_SOLID)) ? p : 0.0f;
-      sc[s] = in ? c : NC_NONE;
+    for (int it = 0; it < 4; it++) {
+      const int r = r0 + 25 * it, lz = r / RS_FY, ly = r - RS_FY * lz;
+      const int gy = fy0 + ly, gz = fz0 + lz;
+      const bool ok = t < 250 && (uint32_t)gx < (uint32_t)F.d0 && (uint32_t)gy < (uint32_t)F.d1 &&
+                      (uint32_t)gz < (uint32_t)F.d2;
+      const uint32_t g = (uint32_t)gx + (uint32_t)sy * (uint32_t)gy + (uint32_t)sz * (uint32_t)gz;
+      pv[it] = rb_ld4(rp, ok ? 4u * g : RB_OOB);
+      const uint32_t c = rb_ld32(rcd, ok ? g : RB_OOB);
+      cv[it] = ok ? c : 0x40404040u;
+    }
+    if (t < 250) {
+#pragma unroll
+      for (int it = 0; it < 4; it++) {
+        float4 v = pv[it];
+        const uint32_t c = cv[it];
+        if (c & 0x80808080u) {  // solids read 0 in the neighbour sums
+          if (c & 0x80u) v.x = 0.0f;
+          if (c & 0x8000u) v.y = 0.0f;
+          if (c & 0x800000u) v.z = 0.0f;
+          if (c & 0x80000000u) v.w = 0.0f;
+        }
+        const int e = 4 * (q + RS_VQ * (r0 + 25 * it));
+        *reinterpret_cast<float4 *>(sp + e) = v;
+        *reinterpret_cast<uint32_t *>(sc + e) = c;
+      }
+    }
+  } else {
+    // every load is unconditional (outside the level at element 0) and
+    // selected afterwards, so all of a thread's loads are in flight at once
+#pragma unroll
+    for (int q = 0; q < (RS_FN + NT - 1) / NT; q++) {
+      const int s = threadIdx.x + q * NT;
+      const int se = s < RS_FN ? s : RS_FN - 1;
+      const int lx = se % RS_FX, r2 = se / RS_FX, ly = r2 % RS_FY, lz = r2 / RS_FY;
+      const int gx = fx0 + lx, gy = fy0 + ly, gz = fz0 + lz;
+      const bool in = gx >= 0 && gx < F.d0 && gy >= 0 && gy < F.d1 && gz >= 0 && gz < F.d2;
+      const int g = in ? gx + sy * gy + sz * gz : 0;
+      const uint8_t c = ncf[g];
+      const float p = pf[g];
+      if (s < RS_FN) {
+        sp[s] = (in && !(c & NC_SOLID)) ? p : 0.0f;
+        sc[s] = in ? c : NC_NONE;
+      }
     }
   }
   __syncthreads();
@@ -151,17 +196,17 @@ __global__ __launch_bounds__(NT) void k_mg_restrict(LvArgs F, int ntx, int nty, 
   for (int k = 0; k < 8; k++) {
     const int a = 2 * x + (k & 1), bb = 2 * y + ((k >> 1) & 1), c = 2 * z + ((k >> 2) & 1);
     if (a >= F.d0 || bb >= F.d1 || c >= F.d2) continue;
-    const int pi = ((c - fz0) * RS_FY + (bb - fy0)) * RS_FX + (a - fx0);
+    const int pi = ((c - fz0) * RS_FY + (bb - fy0)) * FX + (a - fx0);
     const uint8_t code = sc[pi];
     if (code & NC_SOLID) continue;
     nfl += 1.0f;
     float sum = 0.0f;
     sum += sp[pi + 1];
     sum += sp[pi - 1];
-    sum += sp[pi + RS_FX];
-    sum += sp[pi - RS_FX];
-    sum += sp[pi + RS_FX * RS_FY];
-    sum += sp[pi - RS_FX * RS_FY];
+    sum += sp[pi + FX];
+    sum += sp[pi - FX];
+    sum += sp[pi + FX * RS_FY];
+    sum += sp[pi - FX * RS_FY];
     const float cnt = nc_count(code, F.wdir);
     acc += bv[k] - (sum - cnt * sp[pi]) / F.hh;
   }
@@ -172,7 +217,13 @@ __global__ __launch_bounds__(NT) void k_mg_restrict(LvArgs F, int ntx, int nty, 
 void launch_mg_restrict(const MgLevel &Fl, const MgLevel &C, hipStream_t s) {
   LvArgs F = lv_args(Fl);
   const int ntx = (C.d[0] + RS_TX - 1) / RS_TX, nty = (C.d[1] + RS_TY - 1) / RS_TY, ntz = (C.d[2] + RS_TZ - 1) / RS_TZ;
-  k_mg_restrict<<<ntx * nty * ntz, NT, 0, s>>>(F, ntx, nty, Fl.phi, Fl.b, Fl.nc, C.d[0], C.d[1], C.d[2], C.b, C.phi);
+  const bool vec = (Fl.d[0] % 4) == 0 && (uint64_t)Fl.n * 4u < (1ull << 31);
+  if (vec)
+    k_mg_restrict<true><<<ntx * nty * ntz, NT, 0, s>>>(F, ntx, nty, Fl.phi, Fl.b, Fl.nc, C.d[0], C.d[1], C.d[2], C.b,
+                                                       C.phi);
+  else
+    k_mg_restrict<false><<<ntx * nty * ntz, NT, 0, s>>>(F, ntx, nty, Fl.phi, Fl.b, Fl.nc, C.d[0], C.d[1], C.d[2],
+                                                        C.b, C.phi);
 }
 
 // Masked, renormalised trilinear correction (mg.wgsl:97-120).  A block takes
@@ -242,13 +293,15 @@ __global__ __launch_bounds__(NT) void k_mg_prolong(int f0, int f1, int f2, int n
     const float pz = ((float)z + 0.5f) * 0.5f - 0.5f;
     const int i0z = (int)floorf(pz);
     const float frz = (q & 1) ? 0.25f : 0.75f;  // = pz - i0z
+    // a solid corner adds weight 0 (its coarse phi is 0, never updated):
+    // +-0 added to these sums, which start at +0 and are never -0, leaves
+    // them unchanged, so this is the per-cell loop that skips solid corners
     float acc = 0.0f, ws = 0.0f;
 #pragma unroll
     for (int k = 0; k < 8; k++) {
       const int ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
       const int si = (i0z + oz - cz0) * (PL_CY * PL_CX) + sxy + oy * PL_CX + ox;
-      if (sf[si] == 1) continue;
-      const float wgt = wxy[k & 3] * (oz == 1 ? frz : 1.0f - frz);
+      const float wgt = sf[si] == 1 ? 0.0f : wxy[k & 3] * (oz == 1 ? frz : 1.0f - frz);
       acc += wgt * sp[si];
       ws += wgt;
     }
