@@ -361,6 +361,54 @@ __global__ __launch_bounds__(NT) void k_mg_counts(LvArgs a, FastDiv d0, const ui
   if (nm) nm[i] = (uint8_t)m;
 }
 
+// The same codes four cells at a time along x (level x-size a multiple of
+// 4): a thread reads the flag words of its cells and of their y and z
+// neighbours and the two bytes beside them in x, and writes one code word.
+__device__ __forceinline__ uint32_t byte_of(uint32_t w, int j) { return (w >> (8 * j)) & 0xffu; }
+__global__ __launch_bounds__(NT) void k_mg_counts4(LvArgs a, FastDiv d0, const uint8_t *__restrict__ f,
+                                                    uint8_t *__restrict__ nc, uint8_t *__restrict__ nm) {
+  const uint32_t w = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;  // word index
+  if (w >= a.n / 4u) return;
+  const uint32_t i = 4u * w;
+  uint32_t r = fdiv(i, d0);
+  const int x0 = (int)(i - r * (uint32_t)a.d0);
+  uint32_t zq = fdiv(r, a.dny);
+  const int y = (int)(r - zq * (uint32_t)a.d1), z = (int)zq;
+  const uint32_t sy = (uint32_t)a.d0, sz = (uint32_t)a.d0 * (uint32_t)a.d1;
+  const bool hyp = y + 1 < a.d1, hym = y > 0, hzp = z + 1 < a.d2, hzm = z > 0;
+  const bool hx0 = x0 > 0, hx3 = x0 + 4 < a.d0;
+  const uint32_t *f4 = reinterpret_cast<const uint32_t *>(f);
+  // a neighbour outside the level reads the cell's own word or byte and is not used
+  const uint32_t C = f4[w];
+  const uint32_t YP = f4[hyp ? w + sy / 4u : w], YM = f4[hym ? w - sy / 4u : w];
+  const uint32_t ZP = f4[hzp ? w + sz / 4u : w], ZM = f4[hzm ? w - sz / 4u : w];
+  const uint32_t XM = f[hx0 ? i - 1u : i], XP = f[hx3 ? i + 4u : i];
+  uint32_t codes = 0u, masks = 0u;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int x = x0 + j;
+    const bool hxp = x + 1 < a.d0, hxm = x > 0;
+    const uint32_t fc = byte_of(C, j);
+    const uint32_t fxp = j < 3 ? byte_of(C, j + 1) : XP, fxm = j > 0 ? byte_of(C, j - 1) : XM;
+    uint32_t code, m = 0u;
+    if (fc == 1u) {
+      code = NC_SOLID;
+    } else {
+      m = (hxp && fxp != 1u ? 1u : 0u) | (hxm && fxm != 1u ? 2u : 0u) | (hyp && byte_of(YP, j) != 1u ? 4u : 0u) |
+          (hym && byte_of(YM, j) != 1u ? 8u : 0u) | (hzp && byte_of(ZP, j) != 1u ? 16u : 0u) |
+          (hzm && byte_of(ZM, j) != 1u ? 32u : 0u);
+      // neighbours outside the box on a side or the top; the ground (y - 1) is not counted
+      const int out = (hxp ? 0 : 1) + (hxm ? 0 : 1) + (hyp ? 0 : 1) + (hzp ? 0 : 1) + (hzm ? 0 : 1);
+      const int fl = __builtin_popcount(m);
+      code = (fl + out) == 0 ? NC_NONE : (uint32_t)(fl | (out << 3));
+    }
+    codes |= code << (8 * j);
+    masks |= m << (8 * j);
+  }
+  reinterpret_cast<uint32_t *>(nc)[w] = codes;
+  if (nm) reinterpret_cast<uint32_t *>(nm)[w] = masks;
+}
+
 // Coarse cell is solid iff every child inside the fine box is solid (mg.wgsl:74-82).
 __global__ __launch_bounds__(NT) void k_mg_coarse_flags(LvArgs F, const uint8_t *__restrict__ ff, int c0, int c1,
                                                          int c2, FastDiv dc0, FastDiv dc1, uint8_t *__restrict__ fc) {
@@ -387,8 +435,12 @@ void launch_mg_setup(const MgLevel *L, int nl, hipStream_t s) {
       k_mg_coarse_flags<<<nblocks(C.n), NT, 0, s>>>(lv_args(F), F.f, C.d[0], C.d[1], C.d[2],
                                                    make_fastdiv((uint32_t)C.d[0]), make_fastdiv((uint32_t)C.d[1]), C.f);
     }
-    k_mg_counts<<<nblocks(L[l].n), NT, 0, s>>>(lv_args(L[l]), make_fastdiv((uint32_t)L[l].d[0]), L[l].f, L[l].nc,
-                                               L[l].nm);
+    if (L[l].d[0] % 4 == 0)
+      k_mg_counts4<<<nblocks(L[l].n / 4u), NT, 0, s>>>(lv_args(L[l]), make_fastdiv((uint32_t)L[l].d[0]), L[l].f,
+                                                        L[l].nc, L[l].nm);
+    else
+      k_mg_counts<<<nblocks(L[l].n), NT, 0, s>>>(lv_args(L[l]), make_fastdiv((uint32_t)L[l].d[0]), L[l].f, L[l].nc,
+                                                 L[l].nm);
   }
 }
 
