@@ -644,4 +644,82 @@ void launch_adv_s2(const Fields &f, hipStream_t s) {
   k_adv_s2<<<f.g.cbricks, NT, 0, s>>>(f.P, f.g, V, f.s0, f.s1, f.h0, f.h1, f.s0b, f.s1b);
 }
 
+// ---------------------------------------------------------------- fused passes
+// The step runs adv_vel1 with adv_s1, then adv_vel2 with adv_s2: each pair
+// needs the same velocity tile and neither output feeds the other, so one
+// block per face brick stages the tile once and computes its face triples'
+// velocity samples, then its cells' scalar samples (the cells of a brick are
+// its faces inside the grid).  The arithmetic of each output is that of the
+// single kernels above; adv_vel1's velocity-bound (VALU) work and adv_s1's
+// gather-bound work then share the CU.
+__global__ __launch_bounds__(NT) void k_adv_pass1(const Params *__restrict__ P, Geo g, VelPtrs VP,
+                                                   float *__restrict__ hu, float *__restrict__ hv,
+                                                   float *__restrict__ hw, float2 *__restrict__ mu,
+                                                   float2 *__restrict__ mv, float2 *__restrict__ mw,
+                                                   const float4 *__restrict__ s0p, const float4 *__restrict__ s1p,
+                                                   float4 *__restrict__ h0, float4 *__restrict__ h1) {
+  __shared__ float sm[3 * VT_N];
+  __shared__ uint32_t smax;
+  const uint32_t blk = (uint32_t)xcd_tile(blockIdx.x, gridDim.x);
+  int x, y, z, x0, y0, z0;
+  brick_cell(blk, threadIdx.x, g.dfbx, g.dfby, x, y, z);
+  brick_origin(blk, g.dfbx, g.dfby, x0, y0, z0);
+  const VelR V = velr_of(g, VP);
+  const float k = P->dt / P->dx;
+  VTile T;
+  const bool fast = stage_vtile(g, V, sm, &smax, k, x0, y0, z0, T);
+  if (x >= g.vx || y >= g.vy || z > g.nz) return;
+  if (fast) adv_vel1_face<true>(P, g, T, V, k, x, y, z, hu, hv, hw, mu, mv, mw);
+  else adv_vel1_face<false>(P, g, T, V, k, x, y, z, hu, hv, hw, mu, mv, mw);
+  if (x >= g.nx || y >= g.ny || z >= g.nz) return;
+  const RB s0 = rb_of(s0p, g.n * 16u), s1 = rb_of(s1p, g.n * 16u);
+  const float px = (float)x + 0.5f, py = (float)y + 0.5f, pz = (float)z + 0.5f;
+  float bx, by, bz, fx, fy, fz;
+  if (fast) traceT<true, true, false>(g, bnd_of(g), T, V, k, 3, x, y, z, px, py, pz, bx, by, bz, fx, fy, fz);
+  else traceT<false, true, false>(g, bnd_of(g), T, V, k, 3, x, y, z, px, py, pz, bx, by, bz, fx, fy, fz);
+  adv_s1_out(P, g, sidx(g, x, y, z), bx, by, bz, s0, s1, h0, h1);
+}
+void launch_adv_pass1(const Fields &f, hipStream_t s) {
+  VelPtrs V{f.va[0], f.va[1], f.va[2]};
+  k_adv_pass1<<<f.g.fbricks, NT, 0, s>>>(f.P, f.g, V, f.hat[0], f.hat[1], f.hat[2], f.mm[0], f.mm[1], f.mm[2],
+                                           f.s0, f.s1, f.h0, f.h1);
+}
+
+__global__ __launch_bounds__(NT) void k_adv_pass2(const Params *__restrict__ P, Geo g, VelPtrs VP,
+                                                   const float *__restrict__ hu, const float *__restrict__ hv,
+                                                   const float *__restrict__ hw, const float2 *__restrict__ mu,
+                                                   const float2 *__restrict__ mv, const float2 *__restrict__ mw,
+                                                   float *__restrict__ ou, float *__restrict__ ov,
+                                                   float *__restrict__ ow, const uint8_t *__restrict__ fl,
+                                                   const float4 *__restrict__ s0p, const float4 *__restrict__ s1p,
+                                                   const float4 *__restrict__ h0p, const float4 *__restrict__ h1p,
+                                                   float4 *__restrict__ o0, float4 *__restrict__ o1) {
+  __shared__ float sm[3 * VT_N];
+  __shared__ uint32_t smax;
+  const uint32_t blk = (uint32_t)xcd_tile(blockIdx.x, gridDim.x);
+  int x, y, z, x0, y0, z0;
+  brick_cell(blk, threadIdx.x, g.dfbx, g.dfby, x, y, z);
+  brick_origin(blk, g.dfbx, g.dfby, x0, y0, z0);
+  const VelR V = velr_of(g, VP);
+  const float k = P->dt / P->dx;
+  VTile T;
+  const bool fast = stage_vtile(g, V, sm, &smax, k, x0, y0, z0, T);
+  if (x >= g.vx || y >= g.vy || z > g.nz) return;
+  {
+    const RB Hu = rb_of(hu, g.nv * 4u), Hv = rb_of(hv, g.nv * 4u), Hw = rb_of(hw, g.nv * 4u);
+    if (fast) adv_vel2_face<true>(g, T, V, k, x, y, z, Hu, Hv, Hw, mu, mv, mw, ou, ov, ow, fl);
+    else adv_vel2_face<false>(g, T, V, k, x, y, z, Hu, Hv, Hw, mu, mv, mw, ou, ov, ow, fl);
+  }
+  if (x >= g.nx || y >= g.ny || z >= g.nz) return;
+  const RB s0 = rb_of(s0p, g.n * 16u), s1 = rb_of(s1p, g.n * 16u), h0 = rb_of(h0p, g.n * 16u),
+           h1 = rb_of(h1p, g.n * 16u);
+  if (fast) adv_s2_cell<true>(g, T, V, k, x, y, z, s0, s1, h0, h1, o0, o1);
+  else adv_s2_cell<false>(g, T, V, k, x, y, z, s0, s1, h0, h1, o0, o1);
+}
+void launch_adv_pass2(const Fields &f, hipStream_t s) {
+  VelPtrs V{f.va[0], f.va[1], f.va[2]};
+  k_adv_pass2<<<f.g.fbricks, NT, 0, s>>>(f.P, f.g, V, f.hat[0], f.hat[1], f.hat[2], f.mm[0], f.mm[1], f.mm[2],
+                                           f.vb[0], f.vb[1], f.vb[2], f.flags, f.s0, f.s1, f.h0, f.h1, f.s0b, f.s1b);
+}
+
 }  // namespace pyro

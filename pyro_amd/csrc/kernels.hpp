@@ -46,6 +46,9 @@ void launch_adv_vel1(const Fields &f, hipStream_t s);
 void launch_adv_vel2(const Fields &f, hipStream_t s);
 void launch_adv_s1(const Fields &f, hipStream_t s);
 void launch_adv_s2(const Fields &f, hipStream_t s);
+// adv_vel1 with adv_s1, and adv_vel2 with adv_s2, one launch each (the step's order)
+void launch_adv_pass1(const Fields &f, hipStream_t s);
+void launch_adv_pass2(const Fields &f, hipStream_t s);
 void launch_react(const Fields &f, StatParts *st, hipStream_t s);
 void launch_forces(const Fields &f, hipStream_t s);
 void launch_divergence(const Fields &f, hipStream_t s);

@@ -128,9 +128,21 @@ void enqueue_stage(PyroSim *s, int stage, bool stats, hipStream_t st) {
   }
 }
 
-// sim.rs:394-414, plus the FP64 stats combination on stats steps.
+// sim.rs:394-414, plus the FP64 stats combination on stats steps.  The four
+// advection stages run as two fused launches (adv_vel1 with adv_s1, then
+// adv_vel2 with adv_s2): adv_s1 reads nothing adv_vel2 writes and the reverse,
+// so the outputs are those of the reference's order.
 void enqueue_step_body(PyroSim *s, bool stats, hipStream_t st) {
-  for (int k = PYRO_STAGE_SOLID; k <= PYRO_STAGE_VCYCLES; k++) enqueue_stage(s, k, stats, st);
+  for (int k = PYRO_STAGE_SOLID; k <= PYRO_STAGE_VCYCLES; k++) {
+    if (k == PYRO_STAGE_ADV_VEL1) {
+      const Fields f = fields_of(s);
+      launch_adv_pass1(f, st);
+      launch_adv_pass2(f, st);
+      k = PYRO_STAGE_ADV_S2;
+      continue;
+    }
+    enqueue_stage(s, k, stats, st);
+  }
   if (stats) enqueue_stage(s, PYRO_STAGE_RESID, true, st);
   enqueue_stage(s, PYRO_STAGE_PROJECT, stats, st);
   if (stats) launch_stats_finalize(s->g, s->st, STATS_RAN_ALL, st);
