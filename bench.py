@@ -20,8 +20,9 @@ Extra fields:
                    pool_2cm_kernel_traffic.json, scaled by the copy-kernel
                    calibration), when present.
   roofline_kernels -- the same for every kernel of the step, sorted by share:
-                   solid 52, curl 28, adv_vel1 24, adv_vel2 40, adv_s1 76,
-                   adv_s2 108, react 84, forces 76, divergence 36, project 32
+                   solid 52, curl 28, react 84, forces 76, divergence 36,
+                   project 32 B per cell; the two fused advection launches
+                   adv_vel1 + adv_s1 24 + 76 and adv_vel2 + adv_s2 40 + 108
                    B per cell; the level-0 red-black sweep 24 B per cell
                    (12 B per colour pass, two passes per launch, 16 launches
                    per step); restriction 13.5 and prolongation 13 B per
@@ -34,7 +35,9 @@ Extra fields:
                    host's CPU share (OMP_NUM_THREADS, 16 on the MI355X host,
                    ``host_nproc`` beside it) and on one core.
   pool_4cm      -- the small pool (80 x 160 x 80) on the GPU, 20 timed steps.
-  stage_ms      -- per-stage time of one step (HIP events, direct launches).
+  stage_ms      -- per-stage time of one step (HIP events, direct launches;
+                   the four advection stages singly, and the two fused
+                   launches the step uses, adv_pass1 and adv_pass2).
   pressure      -- time to a relative residual of 1e-4 (sqrt of sum r^2 over
                    sum b^2 on fluid cells, FP64) on the next step's pressure
                    problem of this run, warm-started as the step would be,
@@ -212,6 +215,14 @@ def kernel_rooflines(scene, ck, args):
         stage_ms[stg] = tsim.time_stage(stg, 1 if stg == "vcycles" else reps)
     tsim.end_step()
     tsim.sync()
+    # the step runs the four advection stages as two fused launches
+    pass_ms = {}
+    for kn in ("adv_pass1", "adv_pass2"):
+        tsim.time_kernel(kn, 0, 1)  # warm-up
+        pass_ms[kn] = tsim.time_kernel(kn, 0, reps)
+    passes = {"k_adv_pass1": ("adv_vel1 24 + adv_s1 76 B per cell, one launch", 100.0, pass_ms["adv_pass1"]),
+              "k_adv_pass2": ("adv_vel2 40 + adv_s2 108 B per cell, one launch", 148.0, pass_ms["adv_pass2"])}
+    stage_ms.update(pass_ms)
     sweep = tsim.time_kernel("mg_smooth", 0, 200)
     restrict0 = tsim.time_kernel("mg_restrict", 0, 50)
     prolong0 = tsim.time_kernel("mg_prolong", 0, 50)
@@ -231,7 +242,11 @@ def kernel_rooflines(scene, ck, args):
                      "algo_bytes_per_launch": algo_bytes, "avg_launch_us": avg_ms * 1e3,
                      "launches_per_step": per_step, "ms_per_step": avg_ms * per_step})
 
+    for kname, (note, bpc, ms) in passes.items():
+        add(kname, note, bpc * cells, ms, 1)
     for stg, (kname, bpc) in STAGE_KERNELS.items():
+        if stg.startswith("adv_"):
+            continue  # in the passes above
         note = f"stage {stg}, {bpc:g} B per cell"
         if stg == "solid":
             # the reference's pass reads and writes every cell; this one skips
@@ -241,7 +256,8 @@ def kernel_rooflines(scene, ck, args):
         add(kname, note, bpc * cells, stage_ms[stg], 1)
     add("k_mg_sweep<true, true, true>", "level-0 red-black sweep, both colours, 24 B per cell", 24.0 * cells,
         sweep, 16)
-    add("k_mg_restrict", "level 0 -> 1, 13.5 B per fine cell", RESTRICT_BYTES_PER_FINE_CELL * cells, restrict0, 4)
+    add("k_mg_restrict<true>" if nx % 4 == 0 else "k_mg_restrict<false>", "level 0 -> 1, 13.5 B per fine cell",
+        RESTRICT_BYTES_PER_FINE_CELL * cells, restrict0, 4)
     add("k_mg_prolong", "level 1 -> 0, 13 B per fine cell", PROLONG_BYTES_PER_FINE_CELL * cells, prolong0, 4)
     coarse_name = next((k for k in traffic if k.startswith("k_mg_coarse_z")), "k_mg_coarse_z")
     add(coarse_name, f"coarsest level {cd[0]}x{cd[1]}x{cd[2]}: {COARSE_ITERS} SOR iterations in one "
