@@ -5,6 +5,7 @@
 // reduced per block in FP64 in a fixed order and combined by
 // k_stats_finalize, so every stats step is bit-reproducible.
 #include "kernels.hpp"
+#include "curl.hpp"
 
 namespace pyro {
 
@@ -220,10 +221,10 @@ constexpr uint8_t FLAG_BURNT = 2;
 // marched up to 6 faces in turn, and its wave every face any of its cells
 // needed.)
 template <bool STATS>
-__global__ __launch_bounds__(NT) void k_solid(const Params *__restrict__ P, Geo g, const float4 *__restrict__ solIn,
-                                               float4 *__restrict__ solOut, float4 *__restrict__ pyr,
-                                               uint8_t *__restrict__ flags, const float4 *__restrict__ s0,
-                                               double *__restrict__ part) {
+__device__ __forceinline__ void solid_block(const Params *__restrict__ P, const Geo &g,
+                                            const float4 *__restrict__ solIn, float4 *__restrict__ solOut,
+                                            float4 *__restrict__ pyr, uint8_t *__restrict__ flags,
+                                            const float4 *__restrict__ s0, double *__restrict__ part) {
   __shared__ uint16_t sfm[6 * NT];  // listed face: thread << 3 | k
   __shared__ float sfi[6 * NT];     // its irradiance
   __shared__ float sray[5 * NT];    // the rays of a batch of NT faces
@@ -306,6 +307,47 @@ __global__ __launch_bounds__(NT) void k_solid(const Params *__restrict__ P, Geo 
       part[blockIdx.x * 3 + 2] = c;
     }
   }
+}
+
+template <bool STATS>
+__global__ __launch_bounds__(NT) void k_solid(const Params *__restrict__ P, Geo g, const float4 *__restrict__ solIn,
+                                               float4 *__restrict__ solOut, float4 *__restrict__ pyr,
+                                               uint8_t *__restrict__ flags, const float4 *__restrict__ s0,
+                                               double *__restrict__ part) {
+  solid_block<STATS>(P, g, solIn, solOut, pyr, flags, s0, part);
+}
+
+// The step's solid and curl stages in one launch (the same block and cell
+// mapping): curl reads only the velocity and writes only curl, which solid
+// neither reads nor writes, so each block computes its cells' vorticity and
+// then runs the solid pass; the dense, memory-bound curl fills the CUs that
+// wait on the few blocks with ray marches.
+template <bool STATS>
+__global__ __launch_bounds__(NT) void k_solid_curl(const Params *__restrict__ P, Geo g,
+                                                    const float4 *__restrict__ solIn, float4 *__restrict__ solOut,
+                                                    float4 *__restrict__ pyr, uint8_t *__restrict__ flags,
+                                                    const float4 *__restrict__ s0, double *__restrict__ part,
+                                                    const float *__restrict__ U, const float *__restrict__ V,
+                                                    const float *__restrict__ W, float4 *__restrict__ curl,
+                                                    float *__restrict__ cpart) {
+  const uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
+  float spd = 0.0f;
+  if (i < g.n) spd = curl_cell<STATS>(P, g, U, V, W, curl, i);
+  if (STATS) {
+    __shared__ float sh[NT / 64];
+    const float m = block_max<NT>(spd, sh);
+    if (threadIdx.x == 0) cpart[blockIdx.x] = m;
+  }
+  solid_block<STATS>(P, g, solIn, solOut, pyr, flags, s0, part);
+}
+void launch_solid_curl(const Fields &f, StatParts *st, hipStream_t s) {
+  const int nb = nblocks(f.g.n);
+  if (st)
+    k_solid_curl<true><<<nb, NT, 0, s>>>(f.P, f.g, f.sol_in, f.sol_out, f.pyr, f.flags, f.s0, st->solid, f.va[0],
+                                         f.va[1], f.va[2], f.curl, st->curl);
+  else
+    k_solid_curl<false><<<nb, NT, 0, s>>>(f.P, f.g, f.sol_in, f.sol_out, f.pyr, f.flags, f.s0, nullptr, f.va[0],
+                                          f.va[1], f.va[2], f.curl, nullptr);
 }
 
 void launch_solid(const Fields &f, StatParts *st, hipStream_t s) {

@@ -5,50 +5,18 @@
 // per padded face triple (forces, project); the velocity components live in
 // the reference's padded (nx+1)(ny+1)(nz+1) layout, x fastest.
 #include "kernels.hpp"
+#include "curl.hpp"
 
 namespace pyro {
 
 // ---------------------------------------------------------------- curl
-// Face-averaged velocity at the clamped cell c (fluid.wgsl:134-138).
-__device__ __forceinline__ void uc(const Geo &g, const float *__restrict__ U, const float *__restrict__ V,
-                                   const float *__restrict__ W, int x, int y, int z, float &ox, float &oy,
-                                   float &oz) {
-  x = clampi(x, 0, g.nx - 1);
-  y = clampi(y, 0, g.ny - 1);
-  z = clampi(z, 0, g.nz - 1);
-  uint32_t i = vidx(g, x, y, z);
-  ox = 0.5f * (U[i] + U[i + 1]);
-  oy = 0.5f * (V[i] + V[i + g.vx]);
-  oz = 0.5f * (W[i] + W[i + (uint32_t)g.vx * (uint32_t)g.vy]);
-}
-
 template <bool STATS>
 __global__ __launch_bounds__(NT) void k_curl(const Params *__restrict__ P, Geo g, const float *__restrict__ U,
                                               const float *__restrict__ V, const float *__restrict__ W,
                                               float4 *__restrict__ curl, float *__restrict__ part) {
   uint32_t i = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   float spd = 0.0f;
-  if (i < g.n) {
-    int x, y, z;
-    decode_cell(g, i, x, y, z);
-    float tdx = 2.0f * P->dx;
-    float pyx, pyy, pyz, myx, myy, myz, pzx, pzy, pzz, mzx, mzy, mzz, pxx, pxy, pxz, mxx, mxy, mxz;
-    uc(g, U, V, W, x, y + 1, z, pyx, pyy, pyz);
-    uc(g, U, V, W, x, y - 1, z, myx, myy, myz);
-    uc(g, U, V, W, x, y, z + 1, pzx, pzy, pzz);
-    uc(g, U, V, W, x, y, z - 1, mzx, mzy, mzz);
-    uc(g, U, V, W, x + 1, y, z, pxx, pxy, pxz);
-    uc(g, U, V, W, x - 1, y, z, mxx, mxy, mxz);
-    float wx = ((pyz - myz) - (pzy - mzy)) / tdx;
-    float wy = ((pzx - mzx) - (pxz - mxz)) / tdx;
-    float wz = ((pxy - mxy) - (pyx - myx)) / tdx;
-    curl[i] = make_float4(wx, wy, wz, len3(wx, wy, wz));
-    if (STATS) {
-      float cx, cy, cz;
-      uc(g, U, V, W, x, y, z, cx, cy, cz);
-      spd = len3(cx, cy, cz);
-    }
-  }
+  if (i < g.n) spd = curl_cell<STATS>(P, g, U, V, W, curl, i);
   if (STATS) {
     __shared__ float sh[NT / 64];
     float m = block_max<NT>(spd, sh);

@@ -34,6 +34,8 @@ constexpr int NT = 256;  // threads per block of the element-wise kernels
 
 void launch_solid(const Fields &f, StatParts *st, hipStream_t s);
 void launch_curl(const Fields &f, StatParts *st, hipStream_t s);
+// solid and curl in one launch (the step's order; same statistics partials)
+void launch_solid_curl(const Fields &f, StatParts *st, hipStream_t s);
 // flags[i] = 1 where sol[i] is solid (at creation and after SOL is written).
 void launch_init_flags(const Params *P, const Geo &g, const float4 *sol, uint8_t *flags, hipStream_t s);
 // Scene::voxelize on the device (voxel.hip): SOL of dims from the expanded

@@ -128,12 +128,17 @@ void enqueue_stage(PyroSim *s, int stage, bool stats, hipStream_t st) {
   }
 }
 
-// sim.rs:394-414, plus the FP64 stats combination on stats steps.  The four
-// advection stages run as two fused launches (adv_vel1 with adv_s1, then
-// adv_vel2 with adv_s2): adv_s1 reads nothing adv_vel2 writes and the reverse,
-// so the outputs are those of the reference's order.
+// sim.rs:394-414, plus the FP64 stats combination on stats steps.  Solid and
+// curl run as one launch, and the four advection stages as two (adv_vel1 with
+// adv_s1, then adv_vel2 with adv_s2): in each group no stage reads what
+// another writes, so the outputs are those of the reference's order.
 void enqueue_step_body(PyroSim *s, bool stats, hipStream_t st) {
   for (int k = PYRO_STAGE_SOLID; k <= PYRO_STAGE_VCYCLES; k++) {
+    if (k == PYRO_STAGE_SOLID) {
+      launch_solid_curl(fields_of(s), stats ? &s->st : nullptr, st);
+      k = PYRO_STAGE_CURL;
+      continue;
+    }
     if (k == PYRO_STAGE_ADV_VEL1) {
       const Fields f = fields_of(s);
       launch_adv_pass1(f, st);
