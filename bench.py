@@ -20,10 +20,11 @@ Extra fields:
                    pool_2cm_kernel_traffic.json, scaled by the copy-kernel
                    calibration), when present.
   roofline_kernels -- the same for every kernel of the step, sorted by share:
-                   solid 52, curl 28, react 84, forces 76, divergence 36,
-                   project 32 B per cell; the two fused advection launches
-                   adv_vel1 + adv_s1 24 + 76 and adv_vel2 + adv_s2 40 + 108
-                   B per cell; the level-0 red-black sweep 24 B per cell
+                   react 84, forces 76, divergence 36, project 32 B per
+                   cell; the fused launches solid + curl 52 + 28 (solid's
+                   bytes are the reference's dense pass; this one is
+                   sparse), adv_vel1 + adv_s1 24 + 76 and adv_vel2 + adv_s2
+                   40 + 108 B per cell; the level-0 red-black sweep 24 B per cell
                    (12 B per colour pass, two passes per launch, 16 launches
                    per step); restriction 13.5 and prolongation 13 B per
                    fine cell; the coarsest-level solve 80 x 24 B per coarse
@@ -36,8 +37,8 @@ Extra fields:
                    ``host_nproc`` beside it) and on one core.
   pool_4cm      -- the small pool (80 x 160 x 80) on the GPU, 20 timed steps.
   stage_ms      -- per-stage time of one step (HIP events, direct launches;
-                   the four advection stages singly, and the two fused
-                   launches the step uses, adv_pass1 and adv_pass2).
+                   the stages singly, and the fused launches the step
+                   uses: solid_curl, adv_pass1 and adv_pass2).
   pressure      -- time to a relative residual of 1e-4 (sqrt of sum r^2 over
                    sum b^2 on fluid cells, FP64) on the next step's pressure
                    problem of this run, warm-started as the step would be,
@@ -216,12 +217,16 @@ def kernel_rooflines(scene, ck, args):
     tsim.end_step()
     tsim.sync()
     # the step runs the four advection stages as two fused launches
+    # and solid with curl as one
     pass_ms = {}
-    for kn in ("adv_pass1", "adv_pass2"):
+    for kn in ("adv_pass1", "adv_pass2", "solid_curl"):
         tsim.time_kernel(kn, 0, 1)  # warm-up
         pass_ms[kn] = tsim.time_kernel(kn, 0, reps)
     passes = {"k_adv_pass1": ("adv_vel1 24 + adv_s1 76 B per cell, one launch", 100.0, pass_ms["adv_pass1"]),
-              "k_adv_pass2": ("adv_vel2 40 + adv_s2 108 B per cell, one launch", 148.0, pass_ms["adv_pass2"])}
+              "k_adv_pass2": ("adv_vel2 40 + adv_s2 108 B per cell, one launch", 148.0, pass_ms["adv_pass2"]),
+              "k_solid_curl<false>": ("solid 52 (the reference's dense pass; sparse here: only solid cells move "
+                                      "data, so frac can exceed 1) + curl 28 B per cell, one launch", 80.0,
+                                      pass_ms["solid_curl"])}
     stage_ms.update(pass_ms)
     sweep = tsim.time_kernel("mg_smooth", 0, 200)
     restrict0 = tsim.time_kernel("mg_restrict", 0, 50)
@@ -245,15 +250,9 @@ def kernel_rooflines(scene, ck, args):
     for kname, (note, bpc, ms) in passes.items():
         add(kname, note, bpc * cells, ms, 1)
     for stg, (kname, bpc) in STAGE_KERNELS.items():
-        if stg.startswith("adv_"):
-            continue  # in the passes above
-        note = f"stage {stg}, {bpc:g} B per cell"
-        if stg == "solid":
-            # the reference's pass reads and writes every cell; this one skips
-            # the cells that are neither solid nor burnt out (their outputs are
-            # already in place), so frac can exceed 1
-            note += " of the reference's dense pass; sparse here: only solid cells move data"
-        add(kname, note, bpc * cells, stage_ms[stg], 1)
+        if stg.startswith("adv_") or stg in ("solid", "curl"):
+            continue  # in the fused launches above
+        add(kname, f"stage {stg}, {bpc:g} B per cell", bpc * cells, stage_ms[stg], 1)
     add("k_mg_sweep<true, true, true>", "level-0 red-black sweep, both colours, 24 B per cell", 24.0 * cells,
         sweep, 16)
     add("k_mg_restrict<true>" if nx % 4 == 0 else "k_mg_restrict<false>", "level 0 -> 1, 13.5 B per fine cell",

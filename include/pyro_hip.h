@@ -180,6 +180,7 @@ int pyro_sim_time_stage(PyroSim *sim, int stage, int reps, float *avg_ms);
 #define PYRO_KERNEL_RENDER_PACK 4 /* the renderer pack (after one pyro_sim_render_pack) */
 #define PYRO_KERNEL_ADV_PASS1 5   /* the step's fused adv_vel1 + adv_s1 launch (level ignored) */
 #define PYRO_KERNEL_ADV_PASS2 6   /* the step's fused adv_vel2 + adv_s2 launch (level ignored) */
+#define PYRO_KERNEL_SOLID_CURL 7  /* the step's fused solid + curl launch (level ignored) */
 /* Time `reps` launches of one multigrid kernel (HIP events on the sim
  * stream); the pressure state is modified. */
 int pyro_sim_time_kernel(PyroSim *sim, int kernel, int level, int reps, float *avg_ms);

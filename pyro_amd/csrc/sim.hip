@@ -828,7 +828,7 @@ int pyro_sim_time_kernel(PyroSim *s, int kernel, int level, int reps, float *avg
   if (level < 0 || level >= s->nl) return fail(PYRO_E_INVALID, "bad level");
   if ((kernel == PYRO_KERNEL_MG_RESTRICT || kernel == PYRO_KERNEL_MG_PROLONG) && level >= s->nl - 1)
     return fail(PYRO_E_INVALID, "restrict/prolong need a coarser level");
-  if (kernel < 0 || kernel > PYRO_KERNEL_ADV_PASS2) return fail(PYRO_E_INVALID, "bad kernel id");
+  if (kernel < 0 || kernel > PYRO_KERNEL_SOLID_CURL) return fail(PYRO_E_INVALID, "bad kernel id");
   if (kernel == PYRO_KERNEL_RENDER_PACK && !s->render)
     return fail(PYRO_E_STATE, "render_pack timing needs a pyro_sim_render_pack call first");
   if (kernel == PYRO_KERNEL_MG_SMOOTH && (reps & 1)) reps++;  // even: phi ends in its own buffer
@@ -849,6 +849,7 @@ int pyro_sim_time_kernel(PyroSim *s, int kernel, int level, int reps, float *avg
       case PYRO_KERNEL_RENDER_PACK: launch_render_pack(fields_of(s), s->render, s->stream); break;
       case PYRO_KERNEL_ADV_PASS1: launch_adv_pass1(fields_of(s), s->stream); break;
       case PYRO_KERNEL_ADV_PASS2: launch_adv_pass2(fields_of(s), s->stream); break;
+      case PYRO_KERNEL_SOLID_CURL: launch_solid_curl(fields_of(s), nullptr, s->stream); break;
       default: launch_mg_coarse(s->lv[s->nl - 1], COARSE_ITERS, 1.6f, s->stream); break;
     }
   }

@@ -233,7 +233,7 @@ class Sim:
 
     def time_kernel(self, kernel: str, level: int, reps: int) -> float:
         kid = {"mg_smooth": 0, "mg_restrict": 1, "mg_prolong": 2, "mg_coarse": 3, "render_pack": 4, "adv_pass1": 5,
-               "adv_pass2": 6}[kernel]
+               "adv_pass2": 6, "solid_curl": 7}[kernel]
         ms = C.c_float()
         _native.check_hip(_native.hip().pyro_sim_time_kernel(self._h, kid, int(level), int(reps), C.byref(ms)))
         return ms.value
