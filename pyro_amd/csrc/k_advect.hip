@@ -68,27 +68,6 @@ __device__ __forceinline__ float triC(const Geo &g, const RB &X, float gx0, floa
   float c11 = mixf_(p11.x, p11.y, fx);
   return mixf_(mixf_(c00, c10, fy), mixf_(c01, c11, fy), fz);
 }
-// velAt / traceBack / traceFwd (vel.wgsl:2-19) in cell units.
-__device__ __forceinline__ void velAt(const Geo &g, const VelR &V, float px, float py, float pz, float &ux,
-                                      float &uy, float &uz) {
-  ux = triC(g, V.u, px, py - 0.5f, pz - 0.5f, g.nx, g.ny - 1, g.nz - 1);
-  uy = triC(g, V.v, px - 0.5f, py, pz - 0.5f, g.nx - 1, g.ny, g.nz - 1);
-  uz = triC(g, V.w, px - 0.5f, py - 0.5f, pz, g.nx - 1, g.ny - 1, g.nz);
-}
-template <int SIGN>
-__device__ __forceinline__ void trace(const Geo &g, const VelR &V, float k, float x, float y, float z, float &ox,
-                                      float &oy, float &oz) {
-  float u1x, u1y, u1z, u2x, u2y, u2z;
-  velAt(g, V, x, y, z, u1x, u1y, u1z);
-  float hk = 0.5f * k;
-  if (SIGN < 0) {
-    velAt(g, V, x - hk * u1x, y - hk * u1y, z - hk * u1z, u2x, u2y, u2z);
-    ox = x - k * u2x; oy = y - k * u2y; oz = z - k * u2z;
-  } else {
-    velAt(g, V, x + hk * u1x, y + hk * u1y, z + hk * u1z, u2x, u2y, u2z);
-    ox = x + k * u2x; oy = y + k * u2y; oz = z + k * u2z;
-  }
-}
 // ---------------------------------------------------------------- velocity tile
 // Each block stages the three padded velocity arrays over its brick plus a
 // halo of VT_H nodes in LDS.  A cell brick and a face brick index the same
@@ -539,11 +518,9 @@ __device__ __forceinline__ float outside(const Geo &g, float x, float y, float z
                fmax_(fmax_(-0.5f - z, z - ((float)g.nz - 0.5f)), y - ((float)g.ny - 0.5f)));
 }
 
-// adv_s1 (advect.wgsl:72-91): the back trace from the cell centre and the
-// sample of both fields at its foot, ambient beyond the open boundaries.
-// With S1_TILE the velocity comes through the LDS tile like the other three
-// kernels; without it every velocity sample is gathered from global memory.
-constexpr bool S1_TILE = true;
+// adv_s1 (advect.wgsl:72-91): the back trace from the cell centre (through
+// the velocity tile) and the sample of both fields at its foot, ambient
+// beyond the open boundaries.
 __device__ __forceinline__ void adv_s1_out(const Params *__restrict__ P, const Geo &g, uint32_t i, float bx,
                                            float by, float bz, const RB &s0, const RB &s1, float4 *__restrict__ h0,
                                            float4 *__restrict__ h1) {
@@ -566,20 +543,15 @@ __global__ __launch_bounds__(NT) void k_adv_s1(const Params *__restrict__ P, Geo
   const float k = P->dt / P->dx;
   const float px = (float)x + 0.5f, py = (float)y + 0.5f, pz = (float)z + 0.5f;
   float bx, by, bz, fx, fy, fz;
-  if (S1_TILE) {
-    __shared__ float sm[3 * VT_N];
-    __shared__ uint32_t smax;
-    int x0, y0, z0;
-    brick_origin(blk, g.dcbx, g.dcby, x0, y0, z0);
-    VTile T;
-    const bool fast = stage_vtile(g, V, sm, &smax, k, x0, y0, z0, T);
-    if (x >= g.nx || y >= g.ny || z >= g.nz) return;
-    if (fast) traceT<true, true, false>(g, bnd_of(g), T, V, k, 3, x, y, z, px, py, pz, bx, by, bz, fx, fy, fz);
-    else traceT<false, true, false>(g, bnd_of(g), T, V, k, 3, x, y, z, px, py, pz, bx, by, bz, fx, fy, fz);
-  } else {
-    if (x >= g.nx || y >= g.ny || z >= g.nz) return;
-    trace<-1>(g, V, k, px, py, pz, bx, by, bz);
-  }
+  __shared__ float sm[3 * VT_N];
+  __shared__ uint32_t smax;
+  int x0, y0, z0;
+  brick_origin(blk, g.dcbx, g.dcby, x0, y0, z0);
+  VTile T;
+  const bool fast = stage_vtile(g, V, sm, &smax, k, x0, y0, z0, T);
+  if (x >= g.nx || y >= g.ny || z >= g.nz) return;
+  if (fast) traceT<true, true, false>(g, bnd_of(g), T, V, k, 3, x, y, z, px, py, pz, bx, by, bz, fx, fy, fz);
+  else traceT<false, true, false>(g, bnd_of(g), T, V, k, 3, x, y, z, px, py, pz, bx, by, bz, fx, fy, fz);
   adv_s1_out(P, g, sidx(g, x, y, z), bx, by, bz, s0, s1, h0, h1);
 }
 void launch_adv_s1(const Fields &f, hipStream_t s) {
