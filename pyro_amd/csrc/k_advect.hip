@@ -220,11 +220,6 @@ __device__ __forceinline__ float triT(const Geo &g, const VTile &T, int c, const
   const float c11 = mixf_(p11.x, p11.y, ax.f);
   return mixf_(mixf_(c00, c10, ay.f), mixf_(c01, c11, ay.f), az.f);
 }
-template <bool FAST>
-__device__ __forceinline__ float triCT(const Geo &g, const VTile &T, int c, const RB &X, float gx, float gy,
-                                       float gz, int hx, int hy, int hz) {
-  return triT<FAST>(g, T, c, X, axis<FAST>(gx, hx), axis<FAST>(gy, hy), axis<FAST>(gz, hz));
-}
 // triC and mmC (tmpl_comp.wgsl:2-25) of one point from the same corners:
 // the sample is returned, (min, max) of the eight corners stored in *mm.
 template <bool FAST>
