@@ -88,6 +88,11 @@ RESTRICT_BYTES_PER_FINE_CELL = 13.5
 PROLONG_BYTES_PER_FINE_CELL = 13.0
 COARSE_ITERS = 80
 COARSE_BYTES_PER_CELL_ITER = 24.0
+# The coarse solve's own bound is the VALU issue rate of its one CU (one
+# wave64 VALU instruction per cycle at 2.4 GHz): the counter run of
+# profiles/pool_2cm_kernel_counters.md reads 25,550 VALU instructions per wave
+# over 12 waves for the 20x40x20 level of the pool at 2 cm.
+COARSE_VALU_PER_WAVE, COARSE_WAVES, CU_CLOCK_HZ = 25550, 12, 2.4e9
 # U-Net FLOPs per level-0 voxel: 2*27*cin*cout per conv at its level's
 # resolution (1/8 per halving) plus the 1x1x1 head: level 0 2*27*(2*16 + 16*16
 # + 48*16) + 2*16, level 1 2*27*(16*32 + 32*32 + 96*32)/8, level 2
@@ -262,6 +267,9 @@ def kernel_rooflines(scene, ck, args):
     add(coarse_name, f"coarsest level {cd[0]}x{cd[1]}x{cd[2]}: {COARSE_ITERS} SOR iterations in one "
         "workgroup (a dependency chain on one CU), 24 B per cell and iteration",
         COARSE_ITERS * COARSE_BYTES_PER_CELL_ITER * ncoarse, coarse, 4)
+    if (cd[0], cd[1], cd[2]) == (20, 40, 20):
+        # share of the launch its one CU spends issuing VALU instructions
+        ents[-1]["one_cu_valu_issue_frac"] = COARSE_VALU_PER_WAVE * COARSE_WAVES / CU_CLOCK_HZ / (coarse * 1e-3)
     ents.sort(key=lambda e: -e["ms_per_step"])
     for e in ents:
         log(f"[bench] {e['kernel']}: {e['avg_launch_us']:.1f} us x {e['launches_per_step']} = "
