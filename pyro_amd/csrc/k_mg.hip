@@ -261,14 +261,18 @@ __global__ __launch_bounds__(NT) void k_mg_prolong(int f0, int f1, int f2, int n
     fv[q] = ff[i];
     pv[q] = pf[i];
   }
+  bool solid = false;
   for (int s = threadIdx.x; s < PL_CN; s += NT) {
     const int u = s % PL_CX, r2 = s / PL_CX, v = r2 % PL_CY, w = r2 / PL_CY;
     const int cx = clampi(cx0 + u, 0, c0 - 1), cy = clampi(cy0 + v, 0, c1 - 1), cz = clampi(cz0 + w, 0, c2 - 1);
     const uint32_t j = (uint32_t)cx + (uint32_t)c0 * ((uint32_t)cy + (uint32_t)c1 * (uint32_t)cz);
     sp[s] = pc[j];
-    sf[s] = fc[j];
+    const uint8_t fl = fc[j];
+    sf[s] = fl;
+    solid |= fl == 1;
   }
-  __syncthreads();
+  // whether any coarse cell of the footprint is solid (block-uniform)
+  solid = __syncthreads_or(solid);
   if (x >= f0 || y >= f1) return;
   // A fine index's fraction is 0.75 (even) or 0.25 (odd), exactly, so the
   // weights are exact: wx wy is formed once per thread, and z = z0 + q (z0 a
@@ -284,6 +288,34 @@ __global__ __launch_bounds__(NT) void k_mg_prolong(int f0, int f1, int f2, int n
     wxy[k] = wx * wy;
   }
   const int sxy = (i0y - cy0) * PL_CX + (i0x - cx0);
+  if (!solid) {
+    // No solid corner anywhere in the block: every weight is the plain
+    // product and they sum to exactly 1 (multiples of 1/64 below 2, added
+    // exactly), so the correction is acc itself; acc is formed with the
+    // weights and in the order of the general loop below.
+    float w8[2][8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      w8[0][k] = wxy[k & 3] * ((k >> 2) ? 0.75f : 1.0f - 0.75f);
+      w8[1][k] = wxy[k & 3] * ((k >> 2) ? 0.25f : 1.0f - 0.25f);
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      const int z = z0 + q;
+      if (z >= f2 || fv[q] == 1) continue;
+      const uint32_t i = (uint32_t)x + (uint32_t)f0 * ((uint32_t)y + (uint32_t)f1 * (uint32_t)z);
+      const int i0z = (int)floorf(((float)z + 0.5f) * 0.5f - 0.5f);
+      float acc = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const int ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
+        const int si = (i0z + oz - cz0) * (PL_CY * PL_CX) + sxy + oy * PL_CX + ox;
+        acc += w8[q & 1][k] * sp[si];
+      }
+      pf[i] = pv[q] + acc;
+    }
+    return;
+  }
 #pragma unroll
   for (int q = 0; q < NQ; q++) {
     const int z = z0 + q;
