@@ -144,7 +144,24 @@ int pyro_sim_device_ptr(PyroSim *sim, int field, void **dptr, size_t *bytes);
  * host_dst NULL only the device buffer (PYRO_FIELD_RENDER) is written.
  * host_dst must stay valid and untouched until pyro_sim_render_wait. */
 int pyro_sim_render_pack(PyroSim *sim, void *host_dst, size_t bytes);
+/* Blocks until the last hand-off has landed: its copy to host_dst, or with
+ * host_dst NULL the pack itself (for a consumer of the device buffer). */
 int pyro_sim_render_wait(PyroSim *sim);
+/* The renderer's volumes as memory another API imports (the zero-copy form
+ * of the same hand-off, render.rs:255-277: the renderer binds the packed
+ * volumes instead of uploading them).  From the first call on, the packed
+ * volumes (PYRO_FIELD_RENDER, 24 N bytes at offset 0) live in one
+ * exportable device allocation of *bytes bytes (N x 24 rounded up to the
+ * allocation granularity); each call returns a new POSIX file descriptor
+ * for it (a dma-buf on this platform), which the caller owns and closes.
+ * Vulkan imports it with VkImportMemoryFdInfoKHR.  Each
+ * pyro_sim_render_pack(sim, NULL, 0) then rewrites it in stream order;
+ * pyro_sim_render_wait before the importer reads it.  The allocation lives
+ * until pyro_sim_destroy (importers keep their own reference). */
+int pyro_sim_render_export(PyroSim *sim, int *fd, uint64_t *bytes);
+/* Imports an allocation exported as above (fd, alloc_bytes as returned),
+ * maps it and copies its first `bytes` bytes to host_dst; fd stays open. */
+int pyro_shared_read(int fd, uint64_t alloc_bytes, void *host_dst, size_t bytes);
 /* Pinned host memory for the hand-off (hipHostMalloc / hipHostFree). */
 int pyro_host_alloc(size_t bytes, void **out);
 int pyro_host_free(void *p);

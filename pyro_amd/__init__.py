@@ -6,9 +6,9 @@ read_stats / read_scalars) over the C ABI of ``include/pyro_hip.h``.
 """
 from ._native import NativeLibraryMissing, PyroError
 from .scene import Params, Scene, material_index, scene_hash
-from .sim import FIELDS, STAGES, PinnedBuffer, Sim, Stats, level_field
+from .sim import FIELDS, STAGES, PinnedBuffer, Sim, Stats, level_field, shared_read
 
 __all__ = [
     "NativeLibraryMissing", "PyroError", "Params", "Scene", "material_index", "scene_hash",
-    "FIELDS", "STAGES", "PinnedBuffer", "Sim", "Stats", "level_field",
+    "FIELDS", "STAGES", "PinnedBuffer", "Sim", "Stats", "level_field", "shared_read",
 ]

@@ -63,6 +63,8 @@ def hip() -> C.CDLL:
         lib.pyro_voxelize_noalloc.argtypes = [P, sz, P, f32, f32, u32, P]
         lib.pyro_sim_render_pack.argtypes = [P, P, sz]
         lib.pyro_sim_render_wait.argtypes = [P]
+        lib.pyro_sim_render_export.argtypes = [P, C.POINTER(i32), C.POINTER(C.c_uint64)]
+        lib.pyro_shared_read.argtypes = [i32, C.c_uint64, P, sz]
         lib.pyro_host_alloc.argtypes = [sz, C.POINTER(P)]
         lib.pyro_host_free.argtypes = [P]
         lib.pyro_sim_device_ptr.argtypes = [P, i32, C.POINTER(P), C.POINTER(sz)]

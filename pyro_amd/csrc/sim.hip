@@ -67,6 +67,12 @@ struct PyroSim {
   hipStream_t copy_stream = nullptr;
   hipEvent_t ev_packed = nullptr, ev_copied = nullptr;
   bool copy_issued = false;
+  bool pack_issued = false, last_copied = false;  // what pyro_sim_render_wait waits for
+  // the volumes as an exportable allocation (pyro_sim_render_export): the
+  // physical allocation, its mapping and size; render then points at it
+  hipMemGenericAllocationHandle_t render_vmm = nullptr;
+  void *render_vmm_ptr = nullptr;
+  size_t render_vmm_size = 0;
 };
 
 namespace {
@@ -495,8 +501,10 @@ int pyro_sim_render_pack(PyroSim *s, void *host_dst, size_t bytes) {
   if (s->copy_issued) HIPCHK(hipStreamWaitEvent(s->stream, s->ev_copied, 0));
   launch_render_pack(fields_of(s), s->render, s->stream);
   HIPCHK(hipGetLastError());
-  if (!host_dst) return 0;
   HIPCHK(hipEventRecord(s->ev_packed, s->stream));
+  s->pack_issued = true;
+  s->last_copied = host_dst != nullptr;
+  if (!host_dst) return 0;
   HIPCHK(hipStreamWaitEvent(s->copy_stream, s->ev_packed, 0));
   HIPCHK(hipMemcpyAsync(host_dst, s->render, need, hipMemcpyDeviceToHost, s->copy_stream));
   HIPCHK(hipEventRecord(s->ev_copied, s->copy_stream));
@@ -506,10 +514,107 @@ int pyro_sim_render_pack(PyroSim *s, void *host_dst, size_t bytes) {
 
 int pyro_sim_render_wait(PyroSim *s) {
   if (!s) return fail(PYRO_E_INVALID, "null sim");
-  if (!s->copy_issued) return 0;
-  hipError_t e = hipEventSynchronize(s->ev_copied);
-  if (e != hipSuccess) return fail(PYRO_E_HIP, std::string("render hand-off copy: ") + hipGetErrorString(e));
+  if (!s->pack_issued) return 0;
+  hipError_t e = hipEventSynchronize(s->last_copied ? s->ev_copied : s->ev_packed);
+  if (e != hipSuccess) return fail(PYRO_E_HIP, std::string("render hand-off: ") + hipGetErrorString(e));
   return 0;
+}
+
+namespace {
+hipMemAllocationProp export_prop(int dev) {
+  hipMemAllocationProp prop = {};
+  prop.type = hipMemAllocationTypePinned;
+  prop.requestedHandleType = hipMemHandleTypePosixFileDescriptor;
+  prop.location.type = hipMemLocationTypeDevice;
+  prop.location.id = dev;
+  return prop;
+}
+// reserve, map and open for read/write an allocation of `size` bytes
+int map_allocation(hipMemGenericAllocationHandle_t h, size_t size, int dev, void **out) {
+  void *p = nullptr;
+  hipError_t e = hipMemAddressReserve(&p, size, 0, nullptr, 0);
+  if (e != hipSuccess) return fail(PYRO_E_HIP, std::string("hipMemAddressReserve: ") + hipGetErrorString(e));
+  e = hipMemMap(p, size, 0, h, 0);
+  if (e != hipSuccess) {
+    (void)hipMemAddressFree(p, size);
+    return fail(PYRO_E_HIP, std::string("hipMemMap: ") + hipGetErrorString(e));
+  }
+  hipMemAccessDesc acc = {};
+  acc.location.type = hipMemLocationTypeDevice;
+  acc.location.id = dev;
+  acc.flags = hipMemAccessFlagsProtReadWrite;
+  e = hipMemSetAccess(p, size, &acc, 1);
+  if (e != hipSuccess) {
+    (void)hipMemUnmap(p, size);
+    (void)hipMemAddressFree(p, size);
+    return fail(PYRO_E_HIP, std::string("hipMemSetAccess: ") + hipGetErrorString(e));
+  }
+  *out = p;
+  return 0;
+}
+}  // namespace
+
+int pyro_sim_render_export(PyroSim *s, int *fd, uint64_t *bytes) {
+  if (!s || !fd || !bytes) return fail(PYRO_E_INVALID, "pyro_sim_render_export: null argument");
+  if (s->in_step) return fail(PYRO_E_STATE, "render_export inside a step");
+  *fd = -1;
+  *bytes = 0;
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (!s->render_vmm) {
+    int vmm = 0;
+    HIPCHK(hipDeviceGetAttribute(&vmm, hipDeviceAttributeVirtualMemoryManagementSupported, dev));
+    if (!vmm) return fail(PYRO_E_HIP, "pyro_sim_render_export: the device has no virtual memory management");
+    const hipMemAllocationProp prop = export_prop(dev);
+    size_t gran = 0;
+    HIPCHK(hipMemGetAllocationGranularity(&gran, &prop, hipMemAllocationGranularityMinimum));
+    const size_t need = (size_t)s->g.n * 24;
+    const size_t size = gran ? (need + gran - 1) / gran * gran : need;
+    // packs and copies still in flight use the current buffer
+    if (s->copy_stream) HIPCHK(hipStreamSynchronize(s->copy_stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    hipMemGenericAllocationHandle_t h = nullptr;
+    hipError_t e = hipMemCreate(&h, size, &prop, 0);
+    if (e != hipSuccess) return fail(PYRO_E_OOM, std::string("hipMemCreate: ") + hipGetErrorString(e));
+    void *p = nullptr;
+    int rc = map_allocation(h, size, dev, &p);
+    if (rc) {
+      (void)hipMemRelease(h);
+      return rc;
+    }
+    s->render_vmm = h;
+    s->render_vmm_ptr = p;
+    s->render_vmm_size = size;
+    s->render = p;
+  }
+  int f = -1;
+  hipError_t e = hipMemExportToShareableHandle(&f, s->render_vmm, hipMemHandleTypePosixFileDescriptor, 0);
+  if (e != hipSuccess || f < 0)
+    return fail(PYRO_E_HIP, std::string("hipMemExportToShareableHandle: ") + hipGetErrorString(e));
+  *fd = f;
+  *bytes = (uint64_t)s->render_vmm_size;
+  return 0;
+}
+
+int pyro_shared_read(int fd, uint64_t alloc_bytes, void *host_dst, size_t bytes) {
+  if (fd < 0 || !host_dst || !alloc_bytes || bytes > alloc_bytes)
+    return fail(PYRO_E_INVALID, "pyro_shared_read: bad argument");
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  hipMemGenericAllocationHandle_t h = nullptr;
+  hipError_t e = hipMemImportFromShareableHandle(&h, reinterpret_cast<void *>(static_cast<intptr_t>(fd)),
+                                                 hipMemHandleTypePosixFileDescriptor);
+  if (e != hipSuccess) return fail(PYRO_E_HIP, std::string("hipMemImportFromShareableHandle: ") + hipGetErrorString(e));
+  void *p = nullptr;
+  int rc = map_allocation(h, (size_t)alloc_bytes, dev, &p);
+  if (!rc) {
+    e = hipMemcpy(host_dst, p, bytes, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = fail(PYRO_E_HIP, std::string("pyro_shared_read: ") + hipGetErrorString(e));
+    (void)hipMemUnmap(p, (size_t)alloc_bytes);
+    (void)hipMemAddressFree(p, (size_t)alloc_bytes);
+  }
+  (void)hipMemRelease(h);
+  return rc;
 }
 
 int pyro_host_alloc(size_t bytes, void **out) {
@@ -536,6 +641,11 @@ int pyro_sim_destroy(PyroSim *s) {
   if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
   drop_graphs(s);
   for (void *p : s->allocs) (void)hipFree(p);
+  if (s->render_vmm) {
+    (void)hipMemUnmap(s->render_vmm_ptr, s->render_vmm_size);
+    (void)hipMemAddressFree(s->render_vmm_ptr, s->render_vmm_size);
+    (void)hipMemRelease(s->render_vmm);
+  }
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
   return 0;

@@ -272,6 +272,28 @@ class Sim:
     def render_wait(self) -> None:
         _native.check_hip(_native.hip().pyro_sim_render_wait(self._h))
 
+    def render_pack_device(self) -> None:
+        """Pack into the device buffer only (the exported one after
+        render_export), stream-ordered; render_wait() before it is read."""
+        _native.check_hip(_native.hip().pyro_sim_render_pack(self._h, None, 0))
+
+    def render_export(self) -> tuple[int, int]:
+        """A new file descriptor of the exportable allocation that holds the
+        packed volumes (pyro_sim_render_export) and the allocation's size in
+        bytes; the caller closes the descriptor."""
+        fd, nbytes = C.c_int(-1), C.c_uint64(0)
+        _native.check_hip(_native.hip().pyro_sim_render_export(self._h, C.byref(fd), C.byref(nbytes)))
+        return fd.value, nbytes.value
+
+
+def shared_read(fd: int, alloc_bytes: int, dims) -> np.ndarray:
+    """The packed volumes read back through an exported descriptor
+    (pyro_shared_read: import, map, copy), as (3, nz, ny, nx, 4) float16."""
+    nx, ny, nz = dims
+    out = np.zeros((3, nz, ny, nx, 4), np.float16)
+    _native.check_hip(_native.hip().pyro_shared_read(fd, alloc_bytes, out.ctypes.data, out.nbytes))
+    return out
+
 
 class PinnedBuffer:
     """Page-locked host memory (pyro_host_alloc) for the renderer hand-off,

@@ -3,11 +3,15 @@ pack :16-37) against the oracle's pack: the f32 values of the oracle rounded
 to float16 by numpy (round to nearest even) equal the device's half4 volumes
 bit for bit, including f16 overflow to infinity, subnormal results and
 values that are not representable; the asynchronous copy into pinned memory
-holds the state at the pack while the steps enqueued after it run.
+holds the state at the pack while the steps enqueued after it run; the
+exported allocation (pyro_sim_render_export) read back through its file
+descriptor holds the same volumes.
 
 Packs are compared after at least one step: before it the reference's flags
 are still zero (sim.rs:130; the solid pass fills them, combust.wgsl:131),
 while this library derives them from SOL at creation."""
+
+import os
 
 import numpy as np
 import pytest
@@ -127,3 +131,43 @@ def test_pack_rejects_wrong_size():
     assert rc == -1 and b"3 x N x 8" in _native.hip().pyro_last_error()
     with pytest.raises(ValueError):
         g.render_pack(type("Fake", (), {"nbytes": 8, "ptr": None})())
+
+
+def test_exported_volumes_match_the_pack():
+    """The zero-copy hand-off: the volumes packed into the exported
+    allocation, imported again through its descriptor (as a Vulkan renderer
+    would), equal the oracle's pack; a second descriptor of the same
+    allocation sees the next pack; the host hand-off keeps working."""
+    o, g = twin(mini_scene(), seed=15)
+    for k in range(2):
+        o.step(0.004, False)
+        g.step(0.004, False)
+    try:
+        fd, size = g.render_export()
+    except pa.PyroError as e:
+        pytest.skip(f"pyro_sim_render_export unavailable on this device: {e}")
+    fds = [fd]
+    try:
+        nx, ny, nz = g.dims
+        print(f"[render] export fd={fd} bytes={size} dims={g.dims}", flush=True)
+        assert size >= 24 * nx * ny * nz
+        g.render_pack_device()
+        g.render_wait()
+        shared = pa.shared_read(fd, size, g.dims)
+        _assert_pack_equal(o.render_pack(), shared)
+        assert np.array_equal(_bits16(g.read_field("render")), _bits16(shared))
+        for k in range(2):
+            o.step(0.004, False)
+            g.step(0.004, False)
+        fd2, size2 = g.render_export()
+        fds.append(fd2)
+        assert size2 == size
+        g.render_pack_device()
+        g.render_wait()
+        want = o.render_pack()
+        _assert_pack_equal(want, pa.shared_read(fd2, size, g.dims))
+        _assert_pack_equal(want, pa.shared_read(fd, size, g.dims))
+        _assert_pack_equal(want, g.render_pack())
+    finally:
+        for f in fds:
+            os.close(f)
