@@ -415,6 +415,26 @@ int pyro_sim_create(const void *params, size_t params_bytes, const float *sol_in
   return 0;
 }
 
+namespace {
+// The voxelization into device memory dout, stream-ordered on st and
+// finished when it returns: shapes uploaded, the kernel run, st
+// synchronised (no device-wide synchronisation, so other streams of the
+// process keep running), the shape buffer freed.
+int voxelize_on(const struct PyroShape *shapes, size_t count, const uint32_t dims[3], float dx, float t0,
+                uint32_t seed, float4 *dout, hipStream_t st) {
+  PyroShape *dsh = nullptr;
+  int rc = 0;
+  if (count && (hipMalloc(&dsh, count * sizeof(PyroShape)) != hipSuccess ||
+                hipMemcpyAsync(dsh, shapes, count * sizeof(PyroShape), hipMemcpyHostToDevice, st) != hipSuccess))
+    rc = fail(PYRO_E_HIP, "pyro_voxelize: shape upload failed");
+  if (!rc && launch_voxelize(dsh, (int)count, dims, dx, t0, seed, dout, st))
+    rc = fail(PYRO_E_HIP, "pyro_voxelize: kernel launch failed");
+  if (hipStreamSynchronize(st) != hipSuccess && !rc) rc = fail(PYRO_E_HIP, "pyro_voxelize: kernel failed");
+  if (dsh) (void)hipFree(dsh);
+  return rc;
+}
+}  // namespace
+
 int pyro_voxelize(const struct PyroShape *shapes, size_t count, const uint32_t dims[3], float dx, float t0,
                   uint32_t seed, float *dst, int dst_on_device) {
   if (!dims || !dst || (count && !shapes)) return fail(PYRO_E_INVALID, "pyro_voxelize: null argument");
@@ -422,23 +442,21 @@ int pyro_voxelize(const struct PyroShape *shapes, size_t count, const uint32_t d
   if (!n64 || n64 * 16ull >= (1ull << 32) || count > (1u << 20)) return fail(PYRO_E_INVALID, "pyro_voxelize: bad size");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(PYRO_E_NODEVICE, "no HIP device available");
-  PyroShape *dsh = nullptr;
+  hipStream_t st = nullptr;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
+    return fail(PYRO_E_HIP, "pyro_voxelize: hipStreamCreate failed");
   float4 *dout = nullptr;
-  auto done = [&](int rc) {
-    if (dsh) (void)hipFree(dsh);
-    if (dout && !dst_on_device) (void)hipFree(dout);
-    return rc;
-  };
-  if (count && (hipMalloc(&dsh, count * sizeof(PyroShape)) != hipSuccess ||
-                hipMemcpy(dsh, shapes, count * sizeof(PyroShape), hipMemcpyHostToDevice) != hipSuccess))
-    return done(fail(PYRO_E_HIP, "pyro_voxelize: shape upload failed"));
+  int rc = 0;
   if (dst_on_device) dout = reinterpret_cast<float4 *>(dst);
-  else if (hipMalloc(&dout, (size_t)n64 * 16) != hipSuccess) return done(fail(PYRO_E_OOM, "pyro_voxelize: hipMalloc failed"));
-  if (launch_voxelize(dsh, (int)count, dims, dx, t0, seed, dout, nullptr) || hipDeviceSynchronize() != hipSuccess)
-    return done(fail(PYRO_E_HIP, "pyro_voxelize: kernel failed"));
-  if (!dst_on_device && hipMemcpy(dst, dout, (size_t)n64 * 16, hipMemcpyDeviceToHost) != hipSuccess)
-    return done(fail(PYRO_E_HIP, "pyro_voxelize: download failed"));
-  return done(0);
+  else if (hipMalloc(&dout, (size_t)n64 * 16) != hipSuccess) rc = fail(PYRO_E_OOM, "pyro_voxelize: hipMalloc failed");
+  if (!rc) rc = voxelize_on(shapes, count, dims, dx, t0, seed, dout, st);
+  if (!rc && !dst_on_device &&
+      (hipMemcpyAsync(dst, dout, (size_t)n64 * 16, hipMemcpyDeviceToHost, st) != hipSuccess ||
+       hipStreamSynchronize(st) != hipSuccess))
+    rc = fail(PYRO_E_HIP, "pyro_voxelize: download failed");
+  if (dout && !dst_on_device) (void)hipFree(dout);
+  (void)hipStreamDestroy(st);
+  return rc;
 }
 
 int pyro_voxelize_noalloc(const struct PyroShape *shapes, size_t count, const uint32_t dims[3], float dx, float t0,
@@ -473,8 +491,9 @@ int pyro_sim_voxelize(PyroSim *s, const struct PyroShape *shapes, size_t count, 
   if (!s || (count && !shapes)) return fail(PYRO_E_INVALID, "pyro_sim_voxelize: null argument");
   int rc = sync_check(s);
   if (rc) return rc;
+  if (count > (1u << 20)) return fail(PYRO_E_INVALID, "pyro_sim_voxelize: too many shapes");
   const uint32_t dims[3] = {s->hp.nx, s->hp.ny, s->hp.nz};
-  rc = pyro_voxelize(shapes, count, dims, s->hp.dx, t0, seed, reinterpret_cast<float *>(s->sol[s->sol_cur]), 1);
+  rc = voxelize_on(shapes, count, dims, s->hp.dx, t0, seed, static_cast<float4 *>(s->sol[s->sol_cur]), s->stream);
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(s->sol[1 - s->sol_cur], s->sol[s->sol_cur], (size_t)s->g.n * 16, hipMemcpyDeviceToDevice,
                         s->stream));
