@@ -310,8 +310,7 @@ def render_handoff(sim, args):
     (pyro_sim_read_field, pageable); and steps with a hand-off every 21 steps
     (one 24 fps frame at dt = 2 ms) against the same steps without, the copy
     of frame k running beside the steps of frame k+1; both from the same
-    checkpointed state, alternated twice; then the zero-copy form, a pack
-    into the exported allocation (pyro_sim_render_export) and a wait."""
+    checkpointed state, alternated twice."""
     import tempfile
 
     import pyro_amd as pa
@@ -362,28 +361,12 @@ def render_handoff(sim, args):
     plain, overlapped = float(np.mean(times[False])), float(np.mean(times[True]))
     for bf in bufs:
         bf.close()
-    # the zero-copy form: the volumes packed into the exported allocation
-    # (pyro_sim_render_export) that a Vulkan renderer imports; a frame then
-    # costs the pack and a wait for it, no copy
-    try:
-        fd, alloc = sim.render_export()
-        os.close(fd)
-        ts = []
-        for k in range(5):
-            sim.sync()
-            t = time.perf_counter()
-            sim.render_pack_device()
-            sim.render_wait()
-            ts.append(time.perf_counter() - t)
-        exported = {"alloc_MB": alloc / 1e6, "pack_and_wait_ms": 1e3 * min(ts)}
-    except pa.PyroError as e:
-        exported = {"unavailable": str(e)}
     out = {"volumes_MB": 24 * cells / 1e6, "pack_kernel_us": pack_ms * 1e3, "pack_kernel_GBs": pack_gbs,
            "pack_kernel_hbm_frac": pack_gbs / HBM_PEAK_GBS, "pack_and_pinned_copy_ms": handoff_ms,
            "pcie_GBs": 24 * cells / (handoff_ms * 1e-3) / 1e9, "raw_staging_MB": raw_bytes / 1e6,
            "raw_staging_ms": raw_ms, "steps_per_frame": per, "ms_per_step_plain": plain,
            "ms_per_step_with_handoff": overlapped,
-           "handoff_cost_per_frame_ms": (overlapped - plain) * per, "exported": exported}
+           "handoff_cost_per_frame_ms": (overlapped - plain) * per}
     log(f"[bench] render hand-off: {out}")
     return out
 

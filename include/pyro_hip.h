@@ -159,8 +159,10 @@ int pyro_sim_render_wait(PyroSim *sim);
  * pyro_sim_render_wait before the importer reads it.  The allocation lives
  * until pyro_sim_destroy (importers keep their own reference). */
 int pyro_sim_render_export(PyroSim *sim, int *fd, uint64_t *bytes);
-/* Imports an allocation exported as above (fd, alloc_bytes as returned),
- * maps it and copies its first `bytes` bytes to host_dst; fd stays open. */
+/* Meant to import an allocation exported as above (fd, alloc_bytes as
+ * returned), map it and copy its first `bytes` bytes to host_dst.  Disabled:
+ * it returns PYRO_E_STATE (an in-process import ended in a host fault inside
+ * the runtime; no cause has been shown). */
 int pyro_shared_read(int fd, uint64_t alloc_bytes, void *host_dst, size_t bytes);
 /* Pinned host memory for the hand-off (hipHostMalloc / hipHostFree). */
 int pyro_host_alloc(size_t bytes, void **out);

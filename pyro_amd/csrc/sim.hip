@@ -616,24 +616,15 @@ int pyro_sim_render_export(PyroSim *s, int *fd, uint64_t *bytes) {
 }
 
 int pyro_shared_read(int fd, uint64_t alloc_bytes, void *host_dst, size_t bytes) {
-  if (fd < 0 || !host_dst || !alloc_bytes || bytes > alloc_bytes)
-    return fail(PYRO_E_INVALID, "pyro_shared_read: bad argument");
-  int dev = 0;
-  HIPCHK(hipGetDevice(&dev));
-  hipMemGenericAllocationHandle_t h = nullptr;
-  hipError_t e = hipMemImportFromShareableHandle(&h, reinterpret_cast<void *>(static_cast<intptr_t>(fd)),
-                                                 hipMemHandleTypePosixFileDescriptor);
-  if (e != hipSuccess) return fail(PYRO_E_HIP, std::string("hipMemImportFromShareableHandle: ") + hipGetErrorString(e));
-  void *p = nullptr;
-  int rc = map_allocation(h, (size_t)alloc_bytes, dev, &p);
-  if (!rc) {
-    e = hipMemcpy(host_dst, p, bytes, hipMemcpyDeviceToHost);
-    if (e != hipSuccess) rc = fail(PYRO_E_HIP, std::string("pyro_shared_read: ") + hipGetErrorString(e));
-    (void)hipMemUnmap(p, (size_t)alloc_bytes);
-    (void)hipMemAddressFree(p, (size_t)alloc_bytes);
-  }
-  (void)hipMemRelease(h);
-  return rc;
+  // Importing an exported descriptor back into this process (import, map,
+  // copy) ended in a host segmentation fault inside the runtime when run
+  // after a long sequence of other work; no cause has been shown, so the
+  // import is switched off and this entry point only reports that.
+  (void)fd;
+  (void)alloc_bytes;
+  (void)host_dst;
+  (void)bytes;
+  return fail(PYRO_E_STATE, "pyro_shared_read is disabled: importing an exported descriptor in-process is not verified");
 }
 
 int pyro_host_alloc(size_t bytes, void **out) {
