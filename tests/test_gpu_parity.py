@@ -110,6 +110,22 @@ def test_one_step_pool_c1():
     np.testing.assert_allclose(g.read_stats().raw, o.read_stats_raw(), rtol=1e-9, atol=1e-12)
 
 
+def test_c1_pool_4cm_twenty_steps_bitwise():
+    """Config 1 as run: the pool at 4 cm with U[-1e-3, 1e-3] m/s velocity
+    noise (seed 0), 20 steps with statistics every 10th: every field
+    bit-identical to the oracle at the end, the statistics to 1e-9."""
+    o, g = twin(pool_scene(0.04), seed=None)
+    for sim in (o, g):
+        sim.add_velocity_noise(1e-3, 0)
+    for k in range(20):
+        o.step(0.002, k % 10 == 9)
+        g.step(0.002, k % 10 == 9)
+    res = compare(o, g, VEC4 + VEL + ("flags", pa.level_field(0, "phi")), report=print)
+    for name, (mx, rl2, nmis) in res.items():
+        assert nmis == 0, f"{name}: {nmis} values differ (rel L2 {rl2:.3e})"
+    np.testing.assert_allclose(g.read_stats().raw, o.read_stats_raw(), rtol=1e-9, atol=1e-12)
+
+
 def test_ten_steps_mini_window():
     o, g = twin(mini_scene(), seed=5)
     for k in range(10):
