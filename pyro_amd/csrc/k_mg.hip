@@ -529,9 +529,12 @@ __device__ __forceinline__ void sweep_cells(float *sp, const uint8_t *sc, const 
   bool redo = false;
 #pragma unroll
   for (int q = 0; q < N; q++) {
-    float sum = 0.0f;
+    // the sum starts with its first term: it then differs from +0 + ... only
+    // in the sign of a zero sum (all six terms -0), and the update maps a
+    // quotient of either sign of zero to the same value (cen + (q - cen) w)
+    float sum = nb[q][0];
 #pragma unroll
-    for (int k = 0; k < 6; k++) sum += nb[q][k];
+    for (int k = 1; k < 6; k++) sum += nb[q][k];
     xs[q] = sum - hb[q];
     if (ICNT) {
       const float2 cr = ct[code[q] & 63u];
