@@ -56,3 +56,19 @@ def test_bad_params_size_rejected():
     sol = (C.c_float * 4)()
     rc = lib.pyro_sim_create(buf, 100, sol, None, 4, 0.0, 0, C.byref(h))
     assert rc == -1 and b"576" in lib.pyro_last_error()
+
+
+def test_render_export_argument_checks_without_gpu():
+    """The exported hand-off rejects null arguments before touching a device,
+    and the in-process import reports that it is switched off."""
+    import ctypes as C
+
+    from pyro_amd import _native
+
+    lib = _native.hip()
+    fd, nbytes = C.c_int(5), C.c_uint64(7)
+    assert lib.pyro_sim_render_export(None, C.byref(fd), C.byref(nbytes)) == -1
+    assert b"null argument" in lib.pyro_last_error()
+    buf = (C.c_uint8 * 16)()
+    assert lib.pyro_shared_read(3, 16, buf, 16) == -5
+    assert b"disabled" in lib.pyro_last_error()
