@@ -9,8 +9,9 @@
   (relative L2 over fluid cells) of the oracle's FP64 solution; the
   reference V-cycle, which grows without bound on this hierarchy, is stopped
   as diverged long before its iteration cap;
-- the bonfire at 10 cm (160 x 240 x 160 = 6.14 M cells): three steps from a
-  hot state bit-identical to the oracle, then one step whose pressure solve
+- the bonfire at 10 cm (160 x 240 x 160 = 6.14 M cells): a ten-step window
+  from a hot state bit-identical to the oracle; three steps from another,
+  bit-identical, then one step whose pressure solve
   is MGPCG to 1e-4, whose reported residual an FP64 CPU residual of the same
   phi and b confirms.
 """
@@ -89,6 +90,19 @@ def test_c3_256_reference_vcycle_stops_as_diverged(c3):
     print(info)
     assert info.diverged and info.iterations < 400
     assert np.isfinite(info.rel_resid)
+
+
+def test_c5_bonfire_10cm_ten_step_window():
+    """Ten full steps of the bonfire at 10 cm (dt = 5 ms) from a hot, moving
+    state, the last with statistics: every field bit-identical to the
+    oracle, the statistics to 1e-9."""
+    sc = pa.Scene.load(str(SCENES / "bonfire.toml"))
+    o, g = twin(sc, seed=2, vel_amp=3.0)
+    for k in range(10):
+        o.step(0.005, k == 9)
+        g.step(0.005, k == 9)
+    _bitwise(o, g, VEC4 + VEL + ("flags", pa.level_field(0, "phi"), pa.level_field(0, "b")))
+    np.testing.assert_allclose(g.read_stats().raw, o.read_stats_raw(), rtol=1e-9, atol=1e-12)
 
 
 def test_c5_bonfire_10cm_steps_bitwise_then_mgpcg_step():
