@@ -1,6 +1,7 @@
 """Short driver for rocprofv3 runs: ``python -m pyro_amd.profile [--steps N] [--sweeps R]``.
 
-Builds the pool scene at 2 cm (160 x 320 x 160), runs N full steps (every
+Builds the pool scene at 2 cm (160 x 320 x 160; ``--scene bonfire --cell
+0.10 --dt 0.005`` for the bonfire at 10 cm), runs N full steps (every
 kernel of the step appears in the trace), then R launches of the level-0
 red-black sweep and R launches of the plain copy kernel over 64 Mi floats
 (the calibration for FETCH_SIZE / WRITE_SIZE counters).  Optional phases:
@@ -25,7 +26,9 @@ def main() -> None:
     import pyro_amd as pa
 
     ap = argparse.ArgumentParser()
+    ap.add_argument("--scene", default="pool", help="scene file in pyro_amd/scenes (without .toml)")
     ap.add_argument("--cell", type=float, default=0.02)
+    ap.add_argument("--dt", type=float, default=0.002)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--sweeps", type=int, default=20)
     ap.add_argument("--copies", type=int, default=10)
@@ -35,12 +38,12 @@ def main() -> None:
     ap.add_argument("--noise", type=float, default=1e-3,
                     help="amplitude (m/s) of the uniform velocity noise the run starts from")
     args = ap.parse_args()
-    s = pa.Scene.load(str(ROOT / "pyro_amd" / "scenes" / "pool.toml"))
+    s = pa.Scene.load(str(ROOT / "pyro_amd" / "scenes" / f"{args.scene}.toml"))
     s.set("domain.cell", args.cell)
     sim = pa.Sim(s.params(), s.voxelize(), None, 4, s.get("ignition.until"))
     sim.add_velocity_noise(args.noise, 0)
     for k in range(args.steps):
-        sim.step(0.002, k == args.steps - 1)
+        sim.step(args.dt, k == args.steps - 1)
     sim.sync()
     print(f"[profile] {args.steps} steps done", flush=True)
     ms = sim.time_kernel("mg_smooth", 0, args.sweeps)
@@ -51,7 +54,7 @@ def main() -> None:
     if args.solves:
         from pyro_amd.poisson import Poisson
 
-        sim.begin_step(0.002, False)
+        sim.begin_step(args.dt, False)
         for stg in pa.STAGES[: pa.STAGES.index("divergence") + 1]:
             sim.run_stage(stg)
         P = Poisson(sim.read_field(pa.level_field(0, "flags")), s.grid().dx)
